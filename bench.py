@@ -1,0 +1,129 @@
+#!/usr/bin/env python3
+"""Rule-evaluated pairs/s of the plain blocking job on one MI355X.
+
+Workload c2: 100,000 x 100,000 synthetic rows (5 columns), one rule
+year equal AND JW(title) >= 0.89 AND JW(author) >= 0.85, one partition.
+A step = one blocking job over the resident tables: submit, run every
+kernel, copy the matched pairs back.  Inputs are uploaded before the timed
+region; per-table device structures (signatures, join index) are built
+by the warmup jobs, and the cold first job is reported separately.
+
+Prints one JSON line (see the README of the bench contract in DESIGN.md).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# int32 lane-ops per candidate pair of the bound filter stage (the dominant
+# kernel): 8 sum-of-absolute-differences over 32 byte-class counters packed
+# 4 per dword, 2 for the common-prefix length, 2 for min/max of the lengths,
+# 3 for the bound (match bound, S+T and S*T products), 1 compare.
+FILTER_OPS_PER_CANDIDATE = 16
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 int32 lanes/clk x 2.4 GHz
+
+
+def cpu_baseline(cfg, L, R, plan, budget_rows):
+    """The CPU oracle (faithful restatement, every pair interpreted) on every
+    k-th left row against all right rows, on all host cores."""
+    from tests import helpers as H
+    n_threads = min(16, os.cpu_count() or 1)
+    step = max(1, cfg["n_l"] // budget_rows)
+    t = time.perf_counter()
+    pairs, _, cnt = H.run_oracle(plan, L, R, l_step=step, audit=False)
+    dt = time.perf_counter() - t
+    return {"value": cnt["pairs"] / dt, "unit": "pairs/s", "cores": n_threads,
+            "kind": "port",
+            "sample": "every %dth left row (%d rows) x all %d right rows = %d pairs, %.1f s"
+                      % (step, (cfg["n_l"] + step - 1) // step, cfg["n_r"], cnt["pairs"], dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-rows", type=int, default=2000,
+                    help="left rows in the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: only --gpus 1 is supported; this build measures one MI355X "
+              "(see DESIGN.md for how the job shards across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import hyperblocker_amd as hb
+    from hyperblocker_amd import datagen
+
+    L, R, dup, cfg = datagen.make(args.workload)
+    plan = hb.HostPlan.from_yaml(cfg["rules"])
+    ctx = hb.Context(0)
+    tl = ctx.upload(hb.ColumnTable(L))
+    tr = ctx.upload(hb.ColumnTable(R))
+    dp = ctx.plan(plan)
+    ctx.synchronize()
+
+    def one():
+        job = ctx.submit(dp, tl, tr)
+        pairs, audit, stats = job.results()
+        job.release()
+        return pairs, stats
+
+    t = time.perf_counter()
+    pairs, cold = one()
+    cold_s = time.perf_counter() - t
+    for _ in range(max(0, args.warmup - 1)):
+        one()
+    ctx.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pairs, st = one()
+        stats.append(st)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+
+    n_pairs = cfg["n_l"] * cfg["n_r"]
+    ms_per_step = dt * 1e3 / args.steps
+    value = n_pairs * args.steps / dt
+    filt_ms = float(np.mean([s["filter_ms"] for s in stats]))
+    cand = stats[-1]["candidates"]
+    achieved = cand * FILTER_OPS_PER_CANDIDATE / (filt_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "k_filter_tiles+k_filter_small",
+                "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                "ops_per_candidate": FILTER_OPS_PER_CANDIDATE, "candidates_per_launch": cand,
+                "avg_launch_ms": filt_ms}
+    cpu = None
+    if args.cpu_rows > 0:
+        cpu = cpu_baseline(cfg, L, R, plan, args.cpu_rows)
+    out = {
+        "metric": "rule-evaluated pairs/sec on 1 MI355X",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic (seeded generator, workload %s)" % args.workload,
+        "config": {"workload": "%s: %d x %d rows, 1 rule (year eq, JW title>=0.89, "
+                               "JW author>=0.85), n_partitions=1"
+                               % (args.workload, cfg["n_l"], cfg["n_r"]),
+                   "pairs_per_step": n_pairs, "matches": int(len(pairs))},
+        "stage_ms": {k: float(np.mean([s[k] for s in stats]))
+                     for k in ("device_ms", "join_ms", "filter_ms", "exact_ms")},
+        "cold_first_job_ms": cold_s * 1e3,
+        "filtered": stats[-1]["filtered"], "jw_exact": stats[-1]["jw_exact"],
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    dp.release()
+    tl.release()
+    tr.release()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,300 @@
+"""Python host interface of the MI355X blocking path (mirrors the reference's
+HyperBlocker facade, core/hyperblocker.cuh:41-191, and its components).
+
+All work goes through the C ABI in libhyperblocker_amd.so; nothing here
+computes predicates.
+"""
+import ctypes as C
+import itertools
+import os
+
+import numpy as np
+
+from . import capi
+from .capi import check, lib
+
+_job_ids = itertools.count(1)
+
+
+# ---- host side: rules, CSV, partitions ----------------------------------------
+
+class HostPlan:
+    """SerializedExecutionPlan built on the host
+    (ExecutionPlanGenerator, core/components/execution_plan_generator.h:64-155)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self.view = capi.SerializedPlan()
+        check(lib().hb_host_plan_view(self._h, C.byref(self.view)))
+
+    @classmethod
+    def from_rule_dir(cls, rule_dir):
+        h = C.c_void_p()
+        check(lib().hb_rules_load(os.fsencode(rule_dir), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_yaml(cls, docs):
+        arr = (C.c_char_p * len(docs))(*[d.encode() if isinstance(d, str) else d for d in docs])
+        h = C.c_void_p()
+        check(lib().hb_rules_parse(len(docs), arr, C.byref(h)))
+        return cls(h.value)
+
+    @property
+    def n_rules(self):
+        return self.view.n_rules
+
+    @property
+    def pred_index(self):
+        return [self.view.pred_index[i] for i in range(self.view.length)]
+
+    @property
+    def pred_type(self):
+        return b"".join(self.view.pred_type[i] for i in range(self.view.length)).decode()
+
+    @property
+    def pred_threshold(self):
+        return [self.view.pred_threshold[i] for i in range(self.view.length)]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hb_host_plan_free(self._h)
+            self._h = None
+
+
+class ArrayPlan:
+    """A plan given directly as the three SerializedExecutionPlan arrays."""
+
+    def __init__(self, n_rules, pred_index, pred_type, pred_threshold):
+        n = len(pred_index)
+        if not (len(pred_type) == n == len(pred_threshold)):
+            raise ValueError("plan arrays differ in length")
+        self._idx = (C.c_int32 * n)(*pred_index)
+        pt = pred_type.encode() if isinstance(pred_type, str) else bytes(pred_type)
+        self._typ = C.create_string_buffer(pt, n)
+        self._thr = (C.c_float * n)(*pred_threshold)
+        self.view = capi.SerializedPlan(n_rules, n, self._idx,
+                                        C.cast(self._typ, C.POINTER(C.c_char)), self._thr)
+        self.n_rules = n_rules
+        self.pred_index = list(pred_index)
+        self.pred_type = pt.decode()
+        self.pred_threshold = list(pred_threshold)
+
+
+class HostTable:
+    """A CSV table read like rapidcsv::Document(path, LabelParams(-1, -1))
+    (core/components/data_mngr.h:48-51)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self.view = capi.TableView()
+        check(lib().hb_host_table_view(self._h, C.byref(self.view)))
+
+    @classmethod
+    def from_csv(cls, path, sep=","):
+        h = C.c_void_p()
+        check(lib().hb_csv_load(os.fsencode(path), sep.encode()[0:1], C.byref(h)))
+        return cls(h.value)
+
+    @property
+    def n_rows(self):
+        return self.view.n_rows
+
+    @property
+    def n_cols(self):
+        return self.view.n_cols
+
+    def cell(self, row, col):
+        v = lib().hb_host_table_cell(self._h, row, col)
+        if v is None:
+            raise IndexError((row, col))
+        return v
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().hb_host_table_free(self._h)
+            self._h = None
+
+
+class ColumnTable:
+    """Host table held as fixed-width numpy bytes columns ('S' dtype, itemsize
+    > longest cell)."""
+
+    def __init__(self, columns):
+        self.columns = [np.ascontiguousarray(c) for c in columns]
+        rows = {len(c) for c in self.columns}
+        if len(rows) != 1:
+            raise ValueError("ragged columns")
+        self.n_rows = rows.pop()
+        self.n_cols = len(self.columns)
+        for c in self.columns:
+            if c.dtype.kind != "S":
+                raise ValueError("columns must be numpy bytes ('S') arrays")
+        self._bases = (C.c_char_p * self.n_cols)(
+            *[C.cast(c.ctypes.data, C.c_char_p) for c in self.columns])
+        self._strides = (C.c_int64 * self.n_cols)(*[c.dtype.itemsize for c in self.columns])
+        self.view = capi.TableView(self.n_rows, self.n_cols, self._bases, self._strides)
+
+    def cell(self, row, col):
+        return bytes(self.columns[col][row])
+
+
+def partition(l_table, r_table, plan, n_partitions, prefix_idx=capi.CHECK_POINT):
+    """DataPartitioning (core/components/data_mngr.h:81-222).  Returns a list of
+    (l_rows, r_rows) int64 arrays, empty-side partitions dropped."""
+    h = C.c_void_p()
+    check(lib().hb_partition(l_table._h, r_table._h, C.byref(plan.view), n_partitions,
+                             prefix_idx, C.byref(h)))
+    try:
+        out = []
+        for k in range(lib().hb_partitions_count(h)):
+            lp, rp = C.POINTER(C.c_int64)(), C.POINTER(C.c_int64)()
+            nl, nr = C.c_int64(), C.c_int64()
+            check(lib().hb_partitions_get(h, k, C.byref(lp), C.byref(nl), C.byref(rp),
+                                          C.byref(nr)))
+            out.append((np.ctypeslib.as_array(lp, (nl.value,)).copy() if nl.value else
+                        np.zeros(0, np.int64),
+                        np.ctypeslib.as_array(rp, (nr.value,)).copy() if nr.value else
+                        np.zeros(0, np.int64)))
+        return out
+    finally:
+        lib().hb_partitions_free(h)
+
+
+def write_pairs(path, l_table, r_table, pairs, l_rows=None, r_rows=None, append=False):
+    """WriteMatch (core/components/host_reducer.h:80-90)."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    lr = None if l_rows is None else np.ascontiguousarray(l_rows, dtype=np.int64)
+    rr = None if r_rows is None else np.ascontiguousarray(r_rows, dtype=np.int64)
+    check(lib().hb_write_pairs(os.fsencode(path), 1 if append else 0, l_table._h, r_table._h,
+                               None if lr is None else lr.ctypes.data_as(C.POINTER(C.c_int64)),
+                               None if rr is None else rr.ctypes.data_as(C.POINTER(C.c_int64)),
+                               pairs.ctypes.data_as(C.POINTER(C.c_uint32)), len(pairs)))
+
+
+def filter_coefs(threshold):
+    out = (C.c_float * 4)()
+    check(lib().hb_filter_coefs(threshold, out))
+    return [out[i] for i in range(4)]
+
+
+# ---- device side -----------------------------------------------------------------
+
+class Context:
+    """One GPU (hb_ctx_create)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().hb_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h and self._h.value:
+            check(lib().hb_ctx_destroy(self._h))
+            self._h = None
+
+    def synchronize(self):
+        check(lib().hb_ctx_synchronize(self._h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, table, rows=None):
+        return DeviceTable(self, table, rows)
+
+    def plan(self, host_plan):
+        return DevicePlan(self, host_plan)
+
+    def submit(self, plan, l_table, r_table, job_id=None):
+        jid = next(_job_ids) if job_id is None else job_id
+        check(lib().hb_job_submit(self._h, plan._h, l_table._h, r_table._h, jid))
+        return Job(self, jid)
+
+    def block(self, plan, l_table, r_table):
+        """Synchronous blocking of l_table x r_table: (pairs uint32[n,2] in no
+        particular order, audit structured array, stats dict)."""
+        job = self.submit(plan, l_table, r_table)
+        try:
+            return job.results()
+        finally:
+            job.release()
+
+
+class DeviceTable:
+    def __init__(self, ctx, table, rows=None):
+        self.ctx = ctx
+        self.host = table  # keeps the source buffers alive for the caller
+        h = C.c_void_p()
+        if rows is None:
+            check(lib().hb_table_upload(ctx._h, C.byref(table.view), C.byref(h)))
+            self.n_rows = table.view.n_rows
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.int64)
+            check(lib().hb_table_upload_rows(ctx._h, C.byref(table.view),
+                                             rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             len(rows), C.byref(h)))
+            self.n_rows = len(rows)
+        self._h = h
+
+    def release(self):
+        if self._h and self._h.value and self.ctx._h:
+            check(lib().hb_table_release(self.ctx._h, self._h))
+        self._h = None
+
+
+class DevicePlan:
+    def __init__(self, ctx, host_plan):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().hb_plan_create(ctx._h, C.byref(host_plan.view), C.byref(h)))
+        self._h = h
+
+    def release(self):
+        if self._h and self._h.value and self.ctx._h:
+            check(lib().hb_plan_release(self.ctx._h, self._h))
+        self._h = None
+
+
+class Job:
+    def __init__(self, ctx, job_id):
+        self.ctx = ctx
+        self.id = job_id
+
+    def done(self):
+        return check(lib().hb_job_query(self.ctx._h, self.id)) == capi.HB_OK
+
+    def wait(self):
+        check(lib().hb_job_wait(self.ctx._h, self.id))
+
+    def stats(self):
+        s = capi.JobStats()
+        check(lib().hb_job_stats_get(self.ctx._h, self.id, C.byref(s)))
+        return s.as_dict()
+
+    def results(self):
+        p = C.POINTER(C.c_uint32)()
+        a = C.POINTER(capi.AuditEntry)()
+        n, na = C.c_uint64(), C.c_uint64()
+        check(lib().hb_job_results(self.ctx._h, self.id, C.byref(p), C.byref(n), C.byref(a),
+                                   C.byref(na)))
+        pairs = (np.ctypeslib.as_array(p, (n.value * 2,)).reshape(-1, 2).copy()
+                 if n.value else np.zeros((0, 2), np.uint32))
+        audit = (np.frombuffer(C.string_at(a, na.value * capi.AUDIT_DTYPE.itemsize),
+                               dtype=capi.AUDIT_DTYPE).copy()
+                 if na.value else np.zeros(0, capi.AUDIT_DTYPE))
+        return pairs, audit, self.stats()
+
+    def release(self):
+        check(lib().hb_job_release(self.ctx._h, self.id))
+
+
+def sort_pairs(pairs):
+    """(l, r) pairs sorted lexicographically."""
+    pairs = np.asarray(pairs).reshape(-1, 2)
+    if len(pairs) == 0:
+        return pairs.astype(np.uint32)
+    order = np.lexsort((pairs[:, 1], pairs[:, 0]))
+    return pairs[order].astype(np.uint32)

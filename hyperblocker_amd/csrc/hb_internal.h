@@ -1,0 +1,128 @@
+// Internal layout shared by the host runtime and the gfx950 kernels.
+//
+// Device-resident table, per column c (structure of arrays):
+//   bytes[c] : n_rows x width[c] bytes, each cell NUL padded (every byte after
+//              the first NUL is zero), width a multiple of 16 and at least
+//              the longest cell + 1
+//   len[c]   : uint16 byte length of each cell (up to its first NUL)
+//   sig[c]   : 48-byte filter signature per row (SigRow), built on first use
+//              by a similarity predicate
+//   code[c]  : uint32 dictionary code per row, built on first use by an
+//              equality predicate; equal codes <=> equal byte strings
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#else
+#include <hip/hip_runtime_api.h>
+#endif
+
+namespace hb {
+
+// Filter signature of one cell.
+//   h[8] : 32 byte-sized counters, counter b holds #bytes c with (c & 31) == b
+//          (counter b lives in byte (b & 3) of word b >> 2)
+//   pre  : first 4 bytes of the cell, NUL padded, little endian
+//   len  : bits 0-15 length; bit 31 set when the length exceeds 255 (the
+//          counters could saturate, so the filter passes such pairs on)
+struct __attribute__((aligned(16))) SigRow {
+  uint32_t h[8];
+  uint32_t pre;
+  uint32_t len;
+  uint32_t pad[2];
+};
+static_assert(sizeof(SigRow) == 48, "SigRow layout");
+
+constexpr uint32_t kLongFlag = 0x80000000u;
+constexpr int kMaxEq = 4;        // equality predicates per rule beyond the join key
+constexpr int kTileA = 256;      // rows held in lanes per work item
+constexpr int kTileB = 2048;     // rows streamed through LDS per work item
+constexpr int kChunkB = 256;     // LDS chunk of streamed rows
+constexpr uint64_t kSmallGroupPairs = 16384;  // groups at or below go to the list path
+
+struct DevColumn {
+  const uint8_t *bytes;
+  const uint16_t *len;
+  uint32_t width;
+};
+
+// Join groups.  Run g of equal key in the sorted left keys: left positions
+// [l_off, l_off + l_cnt) of perm_l, right positions [r_off, r_off + r_cnt)
+// of perm_r.  Arrays hold n_l + 1 entries; entries at and beyond *n_runs are
+// zero so the exclusive scans give totals at index n_l.
+struct GroupArrays {
+  uint32_t *n_runs;        // [1]
+  uint32_t *l_off, *l_cnt, *r_off, *r_cnt;
+  uint64_t *small_pairs;   // pairs of groups with <= kSmallGroupPairs, else 0
+  uint64_t *pair_off;      // exclusive scan of small_pairs
+  uint64_t *items;         // work items of the other groups, else 0
+  uint64_t *item_off;      // exclusive scan of items
+  unsigned long long *joined;  // += pairs sharing the join key
+  uint32_t cap;            // n_l: array entries - 1
+};
+
+struct EqCheck {  // equality predicates checked per pair (beyond the join key)
+  int32_t n;
+  const uint32_t *code_l[kMaxEq];
+  const uint32_t *code_r[kMaxEq];
+};
+
+struct StageOut {  // pair list output with capacity
+  uint32_t *pairs;                  // 2 * cap uint32
+  unsigned long long *count;
+  unsigned long long cap;
+  unsigned long long *overflow;     // set to 1 when count exceeded cap
+};
+
+struct AuditOut {
+  void *entries;  // hb_audit_entry[cap]
+  unsigned long long *count;
+  unsigned long long cap;
+};
+
+// Kernel launch wrappers (hb_kernels.hip).  All are asynchronous on stream.
+void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint16_t *len,
+                      SigRow *sig, hipStream_t stream);
+void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
+void launch_run_flags(const uint32_t *sorted_keys, uint32_t n, uint32_t *flags,
+                      hipStream_t stream);
+void launch_run_info(const uint32_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
+                     uint32_t n_l, const uint32_t *sorted_r, uint32_t n_r, GroupArrays g,
+                     hipStream_t stream);
+void launch_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g, hipStream_t stream);
+
+// First stage over join groups: remaining equalities + bound filter of the
+// first similarity predicate (has_sim = 0: every equal pair is emitted).
+void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t *perm_r,
+                          const SigRow *sig_l, const SigRow *sig_r, EqCheck eq, int has_sim,
+                          const float coef[4], StageOut out, int n_cu, hipStream_t stream);
+
+// Exact Jaro-Winkler on a pair list (count read from the device).
+void launch_exact(const uint32_t *pairs, const unsigned long long *n_pairs,
+                  unsigned long long in_cap, DevColumn col_l, DevColumn col_r, float threshold,
+                  int32_t slot, StageOut out, AuditOut audit, unsigned long long *jw_count,
+                  int n_cu, hipStream_t stream);
+
+// Append a pair list to the job output, dropping pairs already present
+// (device hash set keyed by (l, r); empty slots hold ~0).
+void launch_merge_unique(const uint32_t *pairs, const unsigned long long *n_pairs,
+                         unsigned long long in_cap, unsigned long long *set,
+                         unsigned long long set_cap, StageOut out, int n_cu, hipStream_t stream);
+void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long long v,
+                     hipStream_t stream);
+
+// Jaro-Winkler from counts, one thread per (m, S, T, p) tuple (tests).
+void launch_jw_from_counts(const int32_t *mstp, int64_t n, double *out, double *out_fused,
+                           hipStream_t stream);
+
+// hb_sort.hip: rocPRIM wrappers.  temp may be null to query temp_bytes.
+hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in,
+                          uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
+                          uint32_t n, hipStream_t stream);
+hipError_t exscan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out,
+                      uint32_t n, hipStream_t stream);
+hipError_t exscan_u64(void *temp, size_t &temp_bytes, const uint64_t *in, uint64_t *out,
+                      uint32_t n, hipStream_t stream);
+
+}  // namespace hb

@@ -1,0 +1,243 @@
+/*
+ * hyperblocker_amd.h -- C ABI of the MI355X (gfx950) rule-based blocking path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an hb_status (0 = HB_OK).  No call truncates output silently: a
+ * result that does not fit grows its buffer, and an input the reference
+ * would misread is rejected with an error.
+ *
+ * Reference interfaces replaced (paths relative to the HyperBlocker tree):
+ *   - table layout      core/data_structures/table.h:16-51 (SerializedTable)
+ *   - execution plan    core/components/execution_plan_generator.h:25-30
+ *                       (SerializedExecutionPlan) and :64-155
+ *                       (ExecutionPlanGenerator, GetExecutionPlan)
+ *   - rule files        core/common/yaml_config.h:19-57 (convert<Rule>)
+ *   - CSV load          core/components/data_mngr.h:46-79 (DataMngr)
+ *   - partitioning      core/components/data_mngr.h:81-222 (DataPartitioning)
+ *   - submit/complete   core/components/host_producer.h:109-215 (AsyncSubmit)
+ *                       core/components/host_reducer.h:32-68 (Run: polling)
+ *   - kernel            core/gpu/global_func.cuh:104-185 (blocking_kernel)
+ *   - output            core/components/host_reducer.h:80-90 (WriteMatch)
+ */
+#ifndef HYPERBLOCKER_AMD_H_
+#define HYPERBLOCKER_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t hb_status;
+#define HB_OK 0
+#define HB_NOT_READY 1        /* job still running (hb_job_query) */
+#define HB_ERR_INVALID_ARG -1 /* null/out-of-range argument */
+#define HB_ERR_PLAN -2        /* malformed plan or rule */
+#define HB_ERR_OOM -3         /* device or pinned allocation failed */
+#define HB_ERR_CAPACITY -4    /* a configured size limit was exceeded */
+#define HB_ERR_HIP -5         /* HIP runtime error */
+#define HB_ERR_NOT_FOUND -6   /* unknown job id */
+#define HB_ERR_IO -7          /* file could not be read or written */
+#define HB_ERR_PARSE -8       /* CSV / YAML / flag syntax error */
+#define HB_ERR_NO_DEVICE -9   /* no HIP device */
+
+/* core/common/types.h:8-13 */
+#define HB_EQUALITIES 'e'
+#define HB_SIM 's'
+#define HB_CHECK_POINT 2147483647
+#define HB_CHECK_POINT_CHAR ' '
+
+typedef struct hb_ctx hb_ctx;
+typedef struct hb_table hb_table;
+typedef struct hb_plan hb_plan;
+typedef struct hb_host_table hb_host_table;
+typedef struct hb_host_plan hb_host_plan;
+typedef struct hb_partitions hb_partitions;
+
+/* Column-addressed view of a host table.  Cell (r, c) is the byte string at
+ * col_base[c] + r * col_stride[c], ending at the first NUL or after
+ * col_stride[c] bytes, whichever comes first. */
+typedef struct {
+  int64_t n_rows;
+  int32_t n_cols;
+  const char *const *col_base;
+  const int64_t *col_stride;
+} hb_table_view;
+
+/* Same fields as SerializedTable (core/data_structures/table.h:16-51):
+ * cell (r, c) at data + r * aligned_tuple_size + col_offset[c], col_size[c]
+ * bytes, NUL padded. */
+typedef struct {
+  size_t n_rows;
+  size_t n_cols;
+  size_t aligned_tuple_size;
+  const size_t *col_size;
+  const size_t *col_offset;
+  const char *data;
+} hb_serialized_table;
+
+/* Same fields as SerializedExecutionPlan
+ * (core/components/execution_plan_generator.h:25-30).  Per rule: slot pairs
+ * (left column, right column) of type 'e' or 's', then one checkpoint slot
+ * (HB_CHECK_POINT, ' ', 1).  A similarity predicate uses the threshold of its
+ * left slot, compared as double (core/gpu/global_func.cuh:139-145). */
+typedef struct {
+  int32_t n_rules;
+  int32_t length;
+  const int32_t *pred_index;
+  const char *pred_type;
+  const float *pred_threshold;
+} hb_serialized_plan;
+
+/* One similarity evaluation whose Jaro-Winkler value lies within 1e-6 of its
+ * threshold.  slot = plan slot of the predicate's left column.
+ * flags bit0: a fused multiply-add in the Winkler step would decide the
+ * other way; bit1: the shorter string has >= 257 bytes (the reference's
+ * 256-entry match array is exceeded there). */
+typedef struct {
+  uint32_t l, r;
+  int32_t slot;
+  int32_t flags;
+  double jw;
+} hb_audit_entry;
+
+typedef struct {
+  uint64_t pairs;      /* rule-evaluated pairs: n_l * n_r */
+  uint64_t candidates; /* pairs passing a rule's equalities, summed over rules */
+  uint64_t filtered;   /* candidates the bound filter sent to exact JW */
+  uint64_t jw_exact;   /* exact JW evaluations */
+  uint64_t matches;    /* emitted pairs */
+  uint64_t audit;      /* audit entries */
+  double device_ms;    /* first kernel start to last kernel end */
+  double join_ms;      /* equality join construction, summed over rules */
+  double filter_ms;    /* equality + bound filter stage, summed over rules */
+  double exact_ms;     /* exact Jaro-Winkler stages, summed over rules */
+  int32_t retries;     /* capacity regrowth reruns */
+  int32_t n_rules;
+} hb_job_stats;
+
+/* ---- context -------------------------------------------------------------
+ * One context per GPU (hipSetDevice(device) is made current by every call
+ * that touches the device).  Thread-safe: calls may come from any thread. */
+hb_status hb_ctx_create(int32_t device, hb_ctx **out);
+hb_status hb_ctx_destroy(hb_ctx *ctx);
+/* Waits until all work of every stream on the context's device is done. */
+hb_status hb_ctx_synchronize(hb_ctx *ctx);
+/* Text of the last error on this thread (never NULL). */
+const char *hb_last_error(void);
+/* Library build id, e.g. "hyperblocker_amd 0.1 gfx950". */
+const char *hb_version(void);
+
+/* ---- tables --------------------------------------------------------------
+ * Uploading copies the cells into library-owned pinned staging, starts the
+ * host-to-device copy on the context's copy stream and returns; the caller
+ * may reuse its buffers immediately.  Jobs that use the table wait for the
+ * copy on the device.  A table may be shared by any number of jobs and
+ * stays resident until released (releasing waits for those jobs). */
+hb_status hb_table_upload(hb_ctx *ctx, const hb_table_view *view, hb_table **out);
+hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *t,
+                                     hb_table **out);
+/* Uploads rows[0..n_rows) of the view, in that order, as a new table: the
+ * per-partition upload of the streamed form (the reference re-serialises
+ * each partition, core/components/data_mngr.h:209-215, and copies it in
+ * AsyncSubmit, host_producer.h:137-152). */
+hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *view, const int64_t *rows,
+                               int64_t n_rows, hb_table **out);
+hb_status hb_table_info(const hb_table *t, int64_t *n_rows, int32_t *n_cols);
+hb_status hb_table_release(hb_ctx *ctx, hb_table *t);
+
+/* ---- plans ---------------------------------------------------------------
+ * Validates and copies the plan: rejects an empty rule, a missing trailing
+ * checkpoint, an unknown predicate type, mismatched slot types and negative
+ * column indices.  Column indices are checked against the tables at submit. */
+hb_status hb_plan_create(hb_ctx *ctx, const hb_serialized_plan *plan, hb_plan **out);
+hb_status hb_plan_release(hb_ctx *ctx, hb_plan *plan);
+
+/* ---- jobs ----------------------------------------------------------------
+ * hb_job_submit enqueues the blocking of every pair (l, r) of L x R under the
+ * plan (AsyncSubmit + blocking_kernel) and returns without waiting.  job_id
+ * is chosen by the caller and must be unused.  Results are library-owned
+ * until hb_job_release. */
+hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
+                        const hb_table *r, uint64_t job_id);
+hb_status hb_job_query(hb_ctx *ctx, uint64_t job_id); /* HB_OK or HB_NOT_READY */
+hb_status hb_job_wait(hb_ctx *ctx, uint64_t job_id);
+/* (l_row, r_row) pairs as 2*n_pairs uint32, sorted by (l, r), each pair
+ * once.  Rows index the job's L and R tables. */
+hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs,
+                         uint64_t *n_pairs, const hb_audit_entry **audit,
+                         uint64_t *n_audit);
+hb_status hb_job_stats_get(hb_ctx *ctx, uint64_t job_id, hb_job_stats *out);
+hb_status hb_job_release(hb_ctx *ctx, uint64_t job_id);
+
+/* Synchronous convenience: submit + wait + copy results into malloc'd
+ * arrays the caller frees with hb_free. */
+hb_status hb_block(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
+                   const hb_table *r, uint32_t **pairs, uint64_t *n_pairs,
+                   hb_audit_entry **audit, uint64_t *n_audit, hb_job_stats *stats);
+void hb_free(void *p);
+
+/* ---- host front end (no device needed) ---------------------------------- */
+
+/* ExecutionPlanGenerator(rule_dir) + GetExecutionPlan(kEqualitiesFirst)
+ * (execution_plan_generator.h:64-155): loads every regular file of rule_dir
+ * (names sorted), decodes each as a rule (yaml_config.h:19-57), orders the
+ * rules by number of equality entries, descending (stable), and serialises
+ * equalities first.  The returned view stays valid until hb_host_plan_free. */
+hb_status hb_rules_load(const char *rule_dir, hb_host_plan **out);
+/* Same, from YAML documents given as strings (one rule per string). */
+hb_status hb_rules_parse(int32_t n_docs, const char *const *docs, hb_host_plan **out);
+hb_status hb_host_plan_view(const hb_host_plan *p, hb_serialized_plan *view);
+void hb_host_plan_free(hb_host_plan *p);
+
+/* rapidcsv::Document(path, LabelParams(-1, -1), SeparatorParams(sep))
+ * (data_mngr.h:48-51): every line is a data row (no header), cells split on
+ * sep outside double quotes, quotes removed ("" -> "), CR of CRLF dropped. */
+hb_status hb_csv_load(const char *path, char sep, hb_host_table **out);
+hb_status hb_host_table_view(const hb_host_table *t, hb_table_view *view);
+/* Cell text (NUL-terminated, library-owned). */
+const char *hb_host_table_cell(const hb_host_table *t, int64_t row, int32_t col);
+void hb_host_table_free(hb_host_table *t);
+
+/* DataPartitioning for two tables (data_mngr.h:81-222): bucket by
+ * std::hash<std::string> of the column in plan slot 0 when that slot is an
+ * equality (the same column index on both sides), optional first-character
+ * sub-bucket (prefix_idx; pass HB_CHECK_POINT for none), partitions with an
+ * empty side dropped.  Partition k holds its L and R row numbers in input
+ * order. */
+hb_status hb_partition(const hb_host_table *l, const hb_host_table *r,
+                       const hb_serialized_plan *plan, uint64_t n_partitions,
+                       int64_t prefix_idx, hb_partitions **out);
+int64_t hb_partitions_count(const hb_partitions *p);
+hb_status hb_partitions_get(const hb_partitions *p, int64_t k, const int64_t **l_rows,
+                            int64_t *n_l, const int64_t **r_rows, int64_t *n_r);
+void hb_partitions_free(hb_partitions *p);
+
+/* WriteMatch (host_reducer.h:80-90): one line "<L[l][0]>,<R[r][0]>\n" per
+ * pair, rows mapped through l_rows / r_rows when not NULL. */
+hb_status hb_write_pairs(const char *path, int32_t append, const hb_host_table *l,
+                         const hb_host_table *r, const int64_t *l_rows,
+                         const int64_t *r_rows, const uint32_t *pairs, uint64_t n_pairs);
+
+/* ---- filter coefficients (exposed for host-side verification) -----------
+ * The bound filter rejects a candidate pair with inner length S, outer
+ * length T, common prefix p < 4 and match upper bound m when
+ * float(m * (S + T)) < coef[p] * float(S * T).  coef[p] is chosen so that
+ * every rejected pair has Jaro-Winkler below threshold - 2e-6. */
+hb_status hb_filter_coefs(float threshold, float coef[4]);
+
+/* ---- device arithmetic check ---------------------------------------------
+ * Evaluates on the GPU, for n tuples mstp[4*i..4*i+3] = (m, S, T, p), the
+ * Jaro tail of lev_jaro_ratio (device_func.cuh:127-128) followed by the
+ * Winkler step of lev_jaro_winkler_ratio (device_func.cuh:229-230), exactly
+ * as the blocking kernels do: out[i] unfused as written, out_fused[i] with
+ * the last multiply-add fused.  Host arrays; synchronous. */
+hb_status hb_jw_from_counts(hb_ctx *ctx, const int32_t *mstp, int64_t n, double *out,
+                            double *out_fused);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYPERBLOCKER_AMD_H_ */
