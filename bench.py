@@ -42,15 +42,38 @@ def cpu_baseline(cfg, L, R, plan, budget_rows):
                       % (step, (cfg["n_l"] + step - 1) // step, cfg["n_r"], cnt["pairs"], dt)}
 
 
+def copy_calibration(n_bytes):
+    """Ten device-to-device copies of n_bytes: under rocprofv3 --pmc, the
+    FETCH_SIZE / WRITE_SIZE this copy kernel reports per dispatch against its
+    known n_bytes gives the scale applied to the blocking kernels' counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(n_bytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(n_bytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(n_bytes)) == 0
+    for _ in range(10):
+        assert hip.hipMemcpy(dst, src, C.c_size_t(n_bytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n_bytes, "copies": 10}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--cpu-rows", type=int, default=2000,
+    ap.add_argument("--cpu-rows", type=int, default=20000,
                     help="left rows in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 10 device-to-device copies of BYTES (counter scale)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py: only --gpus 1 is supported; this build measures one MI355X "
               "(see DESIGN.md for how the job shards across GPUs)", file=sys.stderr)

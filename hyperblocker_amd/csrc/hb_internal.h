@@ -26,25 +26,31 @@ namespace hb {
 //   pre  : first 4 bytes of the cell, NUL padded, little endian
 //   len  : bits 0-15 length; bit 31 set when the length exceeds 255 (the
 //          counters could saturate, so the filter passes such pairs on)
+//   rlen : 1/length rounded to float when 1 <= length <= 255, else +inf; an
+//          infinite term makes the bound's left-hand side infinite (or NaN),
+//          so the pair always passes
 struct __attribute__((aligned(16))) SigRow {
   uint32_t h[8];
   uint32_t pre;
   uint32_t len;
-  uint32_t pad[2];
+  float rlen;
+  uint32_t pad;
 };
 static_assert(sizeof(SigRow) == 48, "SigRow layout");
 
 constexpr uint32_t kLongFlag = 0x80000000u;
 constexpr int kMaxEq = 4;        // equality predicates per rule beyond the join key
-constexpr int kTileA = 256;      // rows held in lanes per work item
-constexpr int kTileB = 2048;     // rows streamed through LDS per work item
-constexpr int kChunkB = 256;     // LDS chunk of streamed rows
-constexpr uint64_t kSmallGroupPairs = 16384;  // groups at or below go to the list path
+
+// Bit-planes of a similarity column (positions 0..32*nwc-1, nwc <= 8):
+// uint32 planes[row][w][k], bit b of word (w, k) is bit k of byte 32w+b.
+constexpr int kMaxPlaneWords = 8;
 
 struct DevColumn {
   const uint8_t *bytes;
   const uint16_t *len;
   uint32_t width;
+  const uint32_t *planes;
+  uint32_t nwc;
 };
 
 // Join groups.  Run g of equal key in the sorted left keys: left positions
@@ -54,12 +60,22 @@ struct DevColumn {
 struct GroupArrays {
   uint32_t *n_runs;        // [1]
   uint32_t *l_off, *l_cnt, *r_off, *r_cnt;
-  uint64_t *small_pairs;   // pairs of groups with <= kSmallGroupPairs, else 0
+  uint64_t *small_pairs;   // pairs of groups with <= small_pairs_max pairs, else 0
   uint64_t *pair_off;      // exclusive scan of small_pairs
   uint64_t *items;         // work items of the other groups, else 0
   uint64_t *item_off;      // exclusive scan of items
   unsigned long long *joined;  // += pairs sharing the join key
   uint32_t cap;            // n_l: array entries - 1
+  uint32_t tile_a;         // rows of the larger side per work item (256 * rows per lane)
+  uint32_t tile_b;         // rows of the smaller side per work item
+  uint32_t small_pairs_max;  // groups with at most this many pairs take the list path
+};
+
+// Launch shape of the filter stage (chosen per device by the autotuner).
+struct FilterShape {
+  int rows_per_lane;   // 1 or 2
+  uint32_t tile_b;     // multiple of 256
+  unsigned grid_per_cu;
 };
 
 struct EqCheck {  // equality predicates checked per pair (beyond the join key)
@@ -84,6 +100,8 @@ struct AuditOut {
 // Kernel launch wrappers (hb_kernels.hip).  All are asynchronous on stream.
 void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint16_t *len,
                       SigRow *sig, hipStream_t stream);
+void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
+                         uint32_t *planes, hipStream_t stream);
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
 void launch_run_flags(const uint32_t *sorted_keys, uint32_t n, uint32_t *flags,
                       hipStream_t stream);
@@ -92,17 +110,26 @@ void launch_run_info(const uint32_t *sorted_l, const uint32_t *flags, const uint
                      hipStream_t stream);
 void launch_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g, hipStream_t stream);
 
+// Signatures and extra equality codes copied into join order (element i of
+// the outputs belongs to row perm[i]; codes kMaxEq per row).  side 0 = left.
+void launch_gather_join(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
+                        EqCheck eq, int side, uint32_t *jcode, hipStream_t stream);
+
 // First stage over join groups: remaining equalities + bound filter of the
 // first similarity predicate (has_sim = 0: every equal pair is emitted).
+// Inputs are in join order (launch_gather_join).
 void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t *perm_r,
-                          const SigRow *sig_l, const SigRow *sig_r, EqCheck eq, int has_sim,
-                          const float coef[4], StageOut out, int n_cu, hipStream_t stream);
+                          const SigRow *jsig_l, const SigRow *jsig_r, const uint32_t *jcode_l,
+                          const uint32_t *jcode_r, int n_eq, int has_sim, const float coef[4],
+                          StageOut out, FilterShape shape, int n_cu, hipStream_t stream);
 
-// Exact Jaro-Winkler on a pair list (count read from the device).
+// Exact Jaro-Winkler on a pair list (count read from the device): one pair
+// per lane, then one wave per pair for the pairs moved to `defer` (shorter
+// string longer than 256 bytes).
 void launch_exact(const uint32_t *pairs, const unsigned long long *n_pairs,
                   unsigned long long in_cap, DevColumn col_l, DevColumn col_r, float threshold,
                   int32_t slot, StageOut out, AuditOut audit, unsigned long long *jw_count,
-                  int n_cu, hipStream_t stream);
+                  StageOut defer, int n_cu, hipStream_t stream);
 
 // Append a pair list to the job output, dropping pairs already present
 // (device hash set keyed by (l, r); empty slots hold ~0).

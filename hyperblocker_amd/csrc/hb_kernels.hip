@@ -9,15 +9,18 @@
 //      (hb_sort.hip); each run of one code on the left meets the run of the
 //      same code on the right (k_run_info).  A rule without equalities is one
 //      group covering L x R.
-//   2. k_filter_groups: pairs of every group are checked for the remaining
-//      equalities (integer code compares) and an upper bound of the first
-//      similarity predicate (byte-class histograms, v_sad_u8); survivors are
-//      compacted with wave ballots into a pair list.
-//   3. k_exact: each surviving pair gets the exact greedy Jaro match count
-//      (one wave per pair: a v_cmp per outer character gives the equal
-//      positions of the inner string as a 64-bit mask, the greedy pick runs on
-//      the scalar unit) and the reference's fp32/fp64 tail; later similarity
-//      predicates of the rule run as further k_exact stages on the list.
+//   2. k_filter_tiles / k_filter_small: pairs of every group are checked for
+//      the remaining equalities (integer code compares) and an upper bound of
+//      the first similarity predicate (byte-class histograms, v_sad_u8);
+//      survivors are compacted with wave ballots into a pair list.  Large
+//      groups run as tiles (256 rows in lanes x up to 1024 rows read as
+//      wave-uniform scalar loads, four per step), small groups as explicit
+//      pair lists.  Signatures are first gathered into join order.
+//   3. k_exact_lanes: each surviving pair (one per lane) gets the exact greedy
+//      Jaro match count from bit-planes of its shorter string, then the
+//      reference's fp32/fp64 tail.  Shorter strings above 256 bytes go to
+//      k_exact (one wave per pair).  Later similarity predicates of the rule
+//      run as further exact stages on the shrinking list.
 //
 // Floating point: contraction is off for this file, fp32 '/' is correctly
 // rounded (hipcc default), so every operation of the Jaro/Winkler tail
@@ -64,6 +67,63 @@ __device__ __forceinline__ void wave_append(bool ok, uint32_t l, uint32_t r, con
       *out.overflow = 1ull;
     }
   }
+}
+
+// Survivor buffer of one work-group in LDS.  Waves append with one LDS
+// atomic each; the group moves the buffer to the global list with a single
+// global atomic (block_flush), so a busy list counter is not contended by
+// every wave.  Appends past the buffer go to the global list directly.
+constexpr uint32_t kBlockOut = 1024;
+struct BlockOut {
+  uint32_t pairs[2 * kBlockOut];
+  uint32_t n;
+  unsigned long long base;
+};
+
+__device__ __forceinline__ void block_append(bool ok, uint32_t l, uint32_t r, BlockOut &bo,
+                                             const StageOut &out) {
+  const uint64_t mask = __ballot(ok);
+  if (mask == 0) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(&bo.n, (uint32_t)__popcll(mask));
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+  if (ok) {
+    const uint32_t pos = base + rank_in(mask);
+    if (pos < kBlockOut) {
+      bo.pairs[2 * pos] = l;
+      bo.pairs[2 * pos + 1] = r;
+    } else {
+      const unsigned long long g = atomicAdd(out.count, 1ull);
+      if (g < out.cap) {
+        out.pairs[2 * g] = l;
+        out.pairs[2 * g + 1] = r;
+      } else {
+        *out.overflow = 1ull;
+      }
+    }
+  }
+}
+
+// Called by every thread of the group.  Leaves bo.n == 0.
+__device__ __forceinline__ void block_flush(BlockOut &bo, const StageOut &out) {
+  __syncthreads();
+  const uint32_t n = min(bo.n, kBlockOut);
+  if (threadIdx.x == 0) bo.base = n ? atomicAdd(out.count, (unsigned long long)n) : 0ull;
+  __syncthreads();
+  const unsigned long long base = bo.base;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned long long pos = base + i;
+    if (pos < out.cap) {
+      out.pairs[2 * pos] = bo.pairs[2 * i];
+      out.pairs[2 * pos + 1] = bo.pairs[2 * i + 1];
+    } else {
+      *out.overflow = 1ull;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) bo.n = 0;
+  __syncthreads();
 }
 
 // lev_jaro_ratio tail, device_func.cuh:127-128.
@@ -153,8 +213,29 @@ __global__ void k_build_sig(const uint8_t *__restrict__ bytes, uint32_t width, i
   for (int j = 0; j < 8; j++) s.h[j] = h[j];
   s.pre = pre;
   s.len = L | (L > 255u ? kLongFlag : 0u);
-  s.pad[0] = s.pad[1] = 0;
+  s.rlen = (L >= 1u && L <= 255u) ? 1.0f / (float)L : INFINITY;
+  s.pad = 0;
   sig_out[r] = s;
+}
+
+__global__ void k_build_planes(const uint8_t *__restrict__ bytes, uint32_t width, int64_t n,
+                               uint32_t nwc, uint32_t *__restrict__ planes) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint8_t *cell = bytes + r * (int64_t)width;
+  uint32_t *dst = planes + r * (int64_t)nwc * 8;
+  for (uint32_t w = 0; w < nwc; w++) {
+    uint32_t P[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = 0; b < 32; b++) {
+      const uint32_t pos = 32 * w + b;
+      const uint32_t c = pos < width ? cell[pos] : 0u;
+#pragma unroll
+      for (int k = 0; k < 8; k++) P[k] |= ((c >> k) & 1u) << b;
+    }
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst + 8 * w);
+    d4[0] = make_uint4(P[0], P[1], P[2], P[3]);
+    d4[1] = make_uint4(P[4], P[5], P[6], P[7]);
+  }
 }
 
 __global__ void k_iota(uint32_t *out, uint32_t n) {
@@ -195,9 +276,9 @@ __device__ __forceinline__ void set_group_work(GroupArrays g, uint32_t gi, uint3
                                                uint32_t r_cnt) {
   const uint64_t pairs = (uint64_t)l_cnt * r_cnt;
   uint64_t small = 0, items = 0;
-  if (pairs > kSmallGroupPairs) {
+  if (pairs > g.small_pairs_max) {
     const uint32_t a = max(l_cnt, r_cnt), b = min(l_cnt, r_cnt);
-    items = (uint64_t)((a + kTileA - 1) / kTileA) * ((b + kTileB - 1) / kTileB);
+    items = (uint64_t)((a + g.tile_a - 1) / g.tile_a) * ((b + g.tile_b - 1) / g.tile_b);
   } else {
     small = pairs;
   }
@@ -260,23 +341,60 @@ __device__ __forceinline__ uint32_t find_group(const uint64_t *off, uint32_t n, 
 
 // ---- stage 1: equalities + bound filter over join groups ------------------
 
+// Signatures and extra equality codes arrive in join order: element i of
+// jsig_l / jcode_l belongs to left row perm_l[i] (likewise on the right), so
+// the rows of a join group are contiguous.  jcode_* hold kMaxEq codes per
+// row, row-major.
 struct FilterArgs {
   GroupArrays g;
   const uint32_t *perm_l, *perm_r;
-  const SigRow *sig_l, *sig_r;
-  EqCheck eq;
-  int32_t has_sim;
+  int32_t n_eq;
   float c0, c1, c2, c3;
   StageOut out;
 };
 
-// Work item = kTileA rows of the larger side in lanes (A) x up to kTileB
-// rows of the other side (B), streamed through LDS in chunks of kChunkB.
-__global__ __launch_bounds__(256) void k_filter_tiles(FilterArgs a) {
-  __shared__ SigRow s_sig[kChunkB];
-  __shared__ uint32_t s_row[kChunkB];
-  __shared__ uint32_t s_code[kMaxEq][kChunkB];
+// The bound test of may_pass for a lane-held A signature against a B row
+// that is the same for the whole wave (scalar registers), in the form
+//   reject  <=>  t * (1/la + 1/lb) < 2 * coef
+// with t = la + lb - SAD = 2 * sum over classes of min(count_a, count_b);
+// this is m_ub * (S + T) < coef * S * T divided by S * T / 2 (S + T = la + lb,
+// S * T = la * lb).  coef is coef[0] when the first bytes differ and coef[3]
+// (the smallest of coef[1..3], so never stricter) when only a later one of
+// the first 4 bytes differs; with no difference there the pair passes.  The
+// reciprocals and the two float operations add at most 1.8e-7 relative
+// error to the left side, inside the coefficients' 1e-6 relative margin.
+// An empty or over-long cell has rlen = +inf, which makes the left side
+// infinite or NaN and passes the pair.
+__device__ __forceinline__ bool bound_pass(const uint32_t (&ah)[8], uint32_t apre, uint32_t la,
+                                           float rla, const SigRow &b, float k0, float k3) {
+  uint32_t sad = __builtin_amdgcn_sad_u8(ah[0], b.h[0], 0u);
+#pragma unroll
+  for (int k = 1; k < 8; k++) sad = __builtin_amdgcn_sad_u8(ah[k], b.h[k], sad);
+  const uint32_t t = la + (b.len & 0xffffu) - sad;
+  const uint32_t x = apre ^ b.pre;
+  const float k = (x & 0xffu) ? k0 : (x ? k3 : -INFINITY);
+  return !((float)t * (rla + b.rlen) < k);
+}
+
+// Work item = 256*AR rows of the larger side in lanes (A, AR rows per lane)
+// x up to tile_b rows of the other side (B).  B rows are staged through LDS
+// in chunks of 256 and read as wave-wide broadcasts, two at a time, so each
+// LDS round trip feeds 2*AR independent pair tests.  NEQ = equality
+// predicates checked per pair beyond the join key.
+template <int NEQ, bool SIM, int AR>
+__global__ __launch_bounds__(256) void k_filter_tiles(const FilterArgs a,
+                                                      const SigRow *__restrict__ jsig_l,
+                                                      const SigRow *__restrict__ jsig_r,
+                                                      const uint32_t *__restrict__ jcode_l,
+                                                      const uint32_t *__restrict__ jcode_r,
+                                                      uint32_t tile_b) {
+  constexpr uint32_t kChunk = 256;
+  __shared__ SigRow s_sig[SIM ? kChunk + 1 : 1];
+  __shared__ uint32_t s_code[NEQ > 0 ? NEQ : 1][kChunk + 1];
+  __shared__ BlockOut bo;
   const uint32_t tid = threadIdx.x;
+  if (tid == 0) bo.n = 0;
+  const float k0 = 2.0f * a.c0, k3 = 2.0f * a.c3;
   const uint32_t n_runs = *a.g.n_runs;
   const uint64_t total = a.g.item_off[a.g.cap];
   for (uint64_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -288,61 +406,105 @@ __global__ __launch_bounds__(256) void k_filter_tiles(FilterArgs a) {
     const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
     const uint32_t *perm_a = a_is_l ? a.perm_l : a.perm_r;
     const uint32_t *perm_b = a_is_l ? a.perm_r : a.perm_l;
-    const SigRow *sig_a = a_is_l ? a.sig_l : a.sig_r;
-    const SigRow *sig_b = a_is_l ? a.sig_r : a.sig_l;
-    const uint32_t n_bseg = (b_cnt + kTileB - 1) / kTileB;
+    const SigRow *sig_a = (a_is_l ? jsig_l : jsig_r) + a_off;
+    const SigRow *sig_b = (a_is_l ? jsig_r : jsig_l) + b_off;
+    const uint32_t *code_a = (a_is_l ? jcode_l : jcode_r) + (uint64_t)a_off * kMaxEq;
+    const uint32_t *code_b = (a_is_l ? jcode_r : jcode_l) + (uint64_t)b_off * kMaxEq;
+    const uint32_t n_bseg = (b_cnt + tile_b - 1) / tile_b;
     const uint64_t local = item - a.g.item_off[gi];
     const uint32_t a_tile = (uint32_t)(local / n_bseg), b_seg = (uint32_t)(local % n_bseg);
-    const uint32_t a_idx = a_tile * kTileA + tid;
-    const bool a_valid = a_idx < a_cnt;
-    const uint32_t a_row = a_valid ? perm_a[a_off + a_idx] : 0u;
-    uint32_t ah[8] = {0, 0, 0, 0, 0, 0, 0, 0}, apre = 0, alen = 0;
-    if (a_valid && a.has_sim) {
-      const uint4 *src = reinterpret_cast<const uint4 *>(sig_a + a_row);
-      const uint4 v0 = src[0], v1 = src[1], v2 = src[2];
-      ah[0] = v0.x; ah[1] = v0.y; ah[2] = v0.z; ah[3] = v0.w;
-      ah[4] = v1.x; ah[5] = v1.y; ah[6] = v1.z; ah[7] = v1.w;
-      apre = v2.x;
-      alen = v2.y;
+    bool a_valid[AR];
+    uint32_t a_row[AR], ah[AR][8], apre[AR], la[AR], acode[AR][NEQ > 0 ? NEQ : 1];
+    float rla[AR];
+#pragma unroll
+    for (int r = 0; r < AR; r++) {
+      const uint32_t a_idx = a_tile * 256 * AR + r * 256 + tid;
+      a_valid[r] = a_idx < a_cnt;
+      a_row[r] = a_valid[r] ? perm_a[a_off + a_idx] : 0u;
+#pragma unroll
+      for (int k = 0; k < 8; k++) ah[r][k] = 0;
+      apre[r] = 0;
+      la[r] = 0;
+      rla[r] = INFINITY;
+      if (SIM && a_valid[r]) {
+        const SigRow s = sig_a[a_idx];
+#pragma unroll
+        for (int k = 0; k < 8; k++) ah[r][k] = s.h[k];
+        apre[r] = s.pre;
+        la[r] = s.len & 0xffffu;
+        rla[r] = s.rlen;
+      }
+#pragma unroll
+      for (int k = 0; k < NEQ; k++)
+        acode[r][k] = a_valid[r] ? code_a[(uint64_t)a_idx * kMaxEq + k] : 0u;
     }
-    uint32_t acode[kMaxEq];
-#pragma unroll
-    for (int k = 0; k < kMaxEq; k++)
-      acode[k] = (k < a.eq.n && a_valid) ? (a_is_l ? a.eq.code_l[k] : a.eq.code_r[k])[a_row] : 0u;
-    const uint32_t b_begin = b_seg * kTileB;
-    const uint32_t b_end = min(b_begin + kTileB, b_cnt);
-    for (uint32_t cb = b_begin; cb < b_end; cb += kChunkB) {
+    const uint32_t b_begin = b_seg * tile_b;
+    const uint32_t b_end = min(b_begin + tile_b, b_cnt);
+    for (uint32_t cb = b_begin; cb < b_end; cb += kChunk) {
+      const uint32_t n_b = min(kChunk, b_end - cb);
       __syncthreads();
-      const uint32_t j = cb + tid;
-      if (j < b_end) {
-        const uint32_t b_row = perm_b[b_off + j];
-        s_row[tid] = b_row;
-        if (a.has_sim) s_sig[tid] = sig_b[b_row];
+      if (bo.n >= kBlockOut / 2) block_flush(bo, a.out);  // bo.n is group-uniform here
+      if (tid < n_b) {
+        if (SIM) s_sig[tid] = sig_b[cb + tid];
 #pragma unroll
-        for (int k = 0; k < kMaxEq; k++)
-          if (k < a.eq.n) s_code[k][tid] = (a_is_l ? a.eq.code_r[k] : a.eq.code_l[k])[b_row];
+        for (int k = 0; k < NEQ; k++) s_code[k][tid] = code_b[(uint64_t)(cb + tid) * kMaxEq + k];
+      }
+      if (tid == 0) {  // odd chunk: a padding row no lane can pass
+        if (SIM) {
+          SigRow z = {};
+          z.rlen = INFINITY;
+          s_sig[n_b] = z;
+        }
+#pragma unroll
+        for (int k = 0; k < NEQ; k++) s_code[k][n_b] = 0;
       }
       __syncthreads();
-      const uint32_t n_b = min((uint32_t)kChunkB, b_end - cb);
-      for (uint32_t jj = 0; jj < n_b; jj++) {
-        bool ok = a_valid;
+      for (uint32_t jj = 0; jj < n_b; jj += 2) {
+        const bool second = jj + 1 < n_b;
+        bool ok[2][AR];
 #pragma unroll
-        for (int k = 0; k < kMaxEq; k++)
-          if (k < a.eq.n) ok = ok && (acode[k] == s_code[k][jj]);
-        if (a.has_sim) {
-          const SigRow &b = s_sig[jj];
-          const uint32_t bh[8] = {b.h[0], b.h[1], b.h[2], b.h[3], b.h[4], b.h[5], b.h[6], b.h[7]};
-          ok = ok && may_pass(ah, apre, alen, bh, b.pre, b.len, a.c0, a.c1, a.c2, a.c3);
+        for (int u = 0; u < 2; u++) {
+          const SigRow b = SIM ? s_sig[jj + u] : SigRow{};
+#pragma unroll
+          for (int r = 0; r < AR; r++) {
+            bool o = a_valid[r] && (u == 0 || second);
+#pragma unroll
+            for (int k = 0; k < NEQ; k++) o &= (acode[r][k] == s_code[k][jj + u]);
+            if (SIM) o &= bound_pass(ah[r], apre[r], la[r], rla[r], b, k0, k3);
+            ok[u][r] = o;
+          }
         }
-        const uint32_t b_row = s_row[jj];
-        wave_append(ok, a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, a.out);
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+          for (int r = 0; r < AR; r++) any |= ok[u][r];
+        if (__ballot(any)) {
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const uint32_t b_row = perm_b[b_off + cb + min(jj + u, n_b - 1)];
+#pragma unroll
+            for (int r = 0; r < AR; r++)
+              block_append(ok[u][r], a_is_l ? a_row[r] : b_row, a_is_l ? b_row : a_row[r], bo,
+                           a.out);
+          }
+        }
       }
     }
   }
+  block_flush(bo, a.out);
 }
 
 // Pairs of small groups, one pair per lane.
-__global__ __launch_bounds__(256) void k_filter_small(FilterArgs a) {
+template <int NEQ, bool SIM>
+__global__ __launch_bounds__(256) void k_filter_small(const FilterArgs a,
+                                                      const SigRow *__restrict__ jsig_l,
+                                                      const SigRow *__restrict__ jsig_r,
+                                                      const uint32_t *__restrict__ jcode_l,
+                                                      const uint32_t *__restrict__ jcode_r) {
+  __shared__ BlockOut bo;
+  if (threadIdx.x == 0) bo.n = 0;
+  __syncthreads();
   const uint32_t n_runs = *a.g.n_runs;
   const uint64_t total = a.g.pair_off[a.g.cap];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -352,24 +514,39 @@ __global__ __launch_bounds__(256) void k_filter_small(FilterArgs a) {
   for (uint64_t q0 = base0; q0 < total; q0 += stride) {
     const uint64_t q = q0 + lane_id();
     bool ok = q < total;
-    uint32_t l_row = 0, r_row = 0;
+    uint32_t l_row = 0, r_row = 0, li = 0, ri = 0;
     if (ok) {
       const uint32_t gi = find_group(a.g.pair_off, n_runs, q);
       const uint64_t local = q - a.g.pair_off[gi];
       const uint32_t r_cnt = a.g.r_cnt[gi];
-      const uint32_t li = (uint32_t)(local / r_cnt), ri = (uint32_t)(local % r_cnt);
-      l_row = a.perm_l[a.g.l_off[gi] + li];
-      r_row = a.perm_r[a.g.r_off[gi] + ri];
+      li = a.g.l_off[gi] + (uint32_t)(local / r_cnt);
+      ri = a.g.r_off[gi] + (uint32_t)(local % r_cnt);
+      l_row = a.perm_l[li];
+      r_row = a.perm_r[ri];
 #pragma unroll
-      for (int k = 0; k < kMaxEq; k++)
-        if (k < a.eq.n) ok = ok && (a.eq.code_l[k][l_row] == a.eq.code_r[k][r_row]);
+      for (int k = 0; k < NEQ; k++)
+        ok = ok && (jcode_l[(uint64_t)li * kMaxEq + k] == jcode_r[(uint64_t)ri * kMaxEq + k]);
     }
-    if (ok && a.has_sim) {
-      const SigRow sa = a.sig_l[l_row], sb = a.sig_r[r_row];
+    if (SIM && ok) {
+      const SigRow sa = jsig_l[li], sb = jsig_r[ri];
       ok = may_pass(sa.h, sa.pre, sa.len, sb.h, sb.pre, sb.len, a.c0, a.c1, a.c2, a.c3);
     }
-    wave_append(ok, l_row, r_row, a.out);
+    block_append(ok, l_row, r_row, bo, a.out);
   }
+  block_flush(bo, a.out);  // every wave of the group reaches this point
+}
+
+// Copies per-row signatures and equality codes into join order.
+__global__ void k_gather_join(const uint32_t *__restrict__ perm, uint32_t n,
+                              const SigRow *__restrict__ sig, SigRow *__restrict__ jsig,
+                              EqCheck eq, int side, uint32_t *__restrict__ jcode) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t row = perm[i];
+  if (sig) jsig[i] = sig[row];
+  for (int k = 0; k < eq.n; k++)
+    jcode[(uint64_t)i * kMaxEq + k] = (side == 0 ? eq.code_l[k] : eq.code_r[k])[row];
+  if (eq.n == 3) jcode[(uint64_t)i * kMaxEq + 3] = jcode[(uint64_t)i * kMaxEq + 2];
 }
 
 // ---- exact Jaro-Winkler -----------------------------------------------------
@@ -451,8 +628,194 @@ struct ExactArgs {
   StageOut out;
   AuditOut audit;
   unsigned long long *jw_count;
+  StageOut defer;  // pairs whose shorter string exceeds 256 bytes
 };
 
+// Decision, output and audit of one evaluated pair (lane-local).
+__device__ __forceinline__ void emit_result(const ExactArgs &a, uint32_t l, uint32_t r, int m,
+                                            int S, int T, int p, bool ok) {
+  const double thr = (double)a.threshold;
+  const float jaro = jaro_value(m, S, T);
+  const double jw = winkler(jaro, p);
+  wave_append(ok && !(jw < thr), l, r, a.out);
+  if (ok && fabs(jw - thr) < 1e-6) {
+    const double jf = winkler_fused(jaro, p);
+    const unsigned long long pos = atomicAdd(a.audit.count, 1ull);
+    if (pos < a.audit.cap) {
+      hb_audit_entry *e = reinterpret_cast<hb_audit_entry *>(a.audit.entries) + pos;
+      e->l = l;
+      e->r = r;
+      e->slot = a.slot;
+      e->flags = (((jw < thr) != (jf < thr)) ? 1 : 0) | (S >= 257 ? 2 : 0);
+      e->jw = jw;
+    } else {
+      *a.out.overflow = 1ull;
+    }
+  }
+}
+
+// Common prefix length of two cells up to n bytes, one lane, 16 bytes at a
+// time (cells are 16-byte aligned and NUL padded to a multiple of 16).
+__device__ __forceinline__ int lane_prefix(const uint8_t *a, const uint8_t *b, int n) {
+  const uint4 *pa = reinterpret_cast<const uint4 *>(a);
+  const uint4 *pb = reinterpret_cast<const uint4 *>(b);
+  for (int base = 0; base < n; base += 16) {
+    const uint4 x = pa[base >> 4], y = pb[base >> 4];
+    const uint32_t d[4] = {x.x ^ y.x, x.y ^ y.y, x.z ^ y.z, x.w ^ y.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (d[q]) return min(n, base + 4 * q + ((__ffs(d[q]) - 1) >> 3));
+    }
+  }
+  return n;
+}
+
+// Greedy match count (device_func.cuh:81-111) for one pair per lane, inner
+// strings of at most 32*NW bytes.  The inner string is held as 8 bit-planes
+// (plane k, word w: bit b set iff bit k of inner byte 32w+b is set), so the
+// positions equal to an outer byte c are the AND over k of
+// (plane k XNOR bit k of c).  `win` is the window [i-h, i+h] of outer
+// position i, advanced by a one-bit left shift per step; `avail` holds inner
+// positions not yet matched.  The lowest candidate is taken with x & -x.
+template <int NW, bool HIGH>
+__device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, int h, int n_out,
+                                           const uint8_t *outer, int n_steps) {
+  uint32_t avail[NW], win[NW];
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    const int lo = 32 * w;
+    avail[w] = S >= lo + 32 ? ~0u : (S > lo ? ((1u << (S - lo)) - 1u) : 0u);
+    win[w] = h >= lo + 31 ? ~0u : (h >= lo ? (~0u >> (31 - (h - lo))) : 0u);
+  }
+  int m = 0;
+  const uint4 *o16 = reinterpret_cast<const uint4 *>(outer);
+  for (int base = 0; base < n_steps; base += 16) {
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (base < n_out) v = o16[base >> 4];
+    const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int i = base + q;
+      const uint32_t c = (vw[q >> 2] >> (8 * (q & 3))) & 0xffu;
+      // -1 where this lane still has outer positions, else 0
+      const uint32_t live = (uint32_t)(-(int)(i < n_out));
+      uint32_t sb[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) sb[k] = (uint32_t)__builtin_amdgcn_sbfe((int)c, k, 1);
+      const uint32_t ascii = HIGH ? ~0u : (uint32_t)((int)(c >> 7) - 1);
+      uint32_t cand[NW];
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        uint32_t e = avail[w] & win[w] & live & ascii;
+#pragma unroll
+        for (int k = 0; k < (HIGH ? 8 : 7); k++) e &= ~(P[w][k] ^ sb[k]);
+        cand[w] = e;
+      }
+      // lowest set bit of the NW-word candidate mask
+      uint32_t found = 0;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const uint32_t low = found ? 0u : (cand[w] & (0u - cand[w]));
+        avail[w] ^= low;
+        found |= low;
+      }
+      m += found != 0;
+      // advance the window: positions shift up by one, and position 0 stays
+      // inside while i + 1 <= h
+#pragma unroll
+      for (int w = NW - 1; w > 0; w--) win[w] = __builtin_amdgcn_alignbit(win[w], win[w - 1], 31);
+      win[0] = (win[0] << 1) | (uint32_t)(i + 1 <= h);
+    }
+  }
+  return m;
+}
+
+// Words w < nwc of the row's planes; words beyond the column's stored width
+// are zero (those positions lie past the inner string).
+template <int NW>
+__device__ __forceinline__ void load_planes(uint32_t (&P)[NW][8], const uint32_t *row_planes,
+                                            int nwc) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(row_planes);
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    uint4 x = make_uint4(0, 0, 0, 0), y = make_uint4(0, 0, 0, 0);
+    if (w < nwc) {
+      x = src[2 * w];
+      y = src[2 * w + 1];
+    }
+    P[w][0] = x.x; P[w][1] = x.y; P[w][2] = x.z; P[w][3] = x.w;
+    P[w][4] = y.x; P[w][5] = y.y; P[w][6] = y.z; P[w][7] = y.w;
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ int lane_matches(const uint32_t *row_planes, int nwc, int S, int T,
+                                            const uint8_t *outer, bool live) {
+  uint32_t P[NW][8];
+  load_planes<NW>(P, row_planes, nwc);
+  const int h = (S + 1) >> 1;
+  const int n_out = live ? min(T, S + h) : 0;
+  // uniform trip count: the largest n_out of the wave
+  int n_steps = n_out;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n_steps = max(n_steps, __shfl_xor(n_steps, off));
+  uint32_t hi = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) hi |= P[w][7];
+  if (__ballot(hi != 0))
+    return lane_greedy<NW, true>(P, S, h, n_out, outer, n_steps);
+  return lane_greedy<NW, false>(P, S, h, n_out, outer, n_steps);
+}
+
+// One pair per lane.  Pairs whose shorter string is longer than 256 bytes
+// are deferred to k_exact (one wave per pair, 257-slot semantics).
+__global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
+  const unsigned long long n_in = *a.n_pairs;
+  const unsigned long long n_use = n_in < a.in_cap ? n_in : a.in_cap;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  unsigned long long evals = 0;
+  for (uint64_t q0 = base0; q0 < n_use; q0 += stride) {
+    const uint64_t idx = q0 + lane_id();
+    bool act = idx < n_use;
+    uint32_t l = 0, r = 0;
+    int la = 0, lb = 0;
+    if (act) {
+      l = a.pairs[2 * idx];
+      r = a.pairs[2 * idx + 1];
+      la = a.col_l.len[l];
+      lb = a.col_r.len[r];
+    }
+    const int S = min(la, lb), T = max(la, lb);
+    const bool deferred = act && S > 256;
+    wave_append(deferred, l, r, a.defer);
+    act = act && !deferred;
+    const bool swap = la > lb;  // device_func.cuh:71-79: left stays inner on ties
+    const uint8_t *ca = a.col_l.bytes + (uint64_t)l * a.col_l.width;
+    const uint8_t *cb = a.col_r.bytes + (uint64_t)r * a.col_r.width;
+    const uint8_t *outer = swap ? ca : cb;
+    const uint32_t *planes = swap ? a.col_r.planes + (uint64_t)r * a.col_r.nwc * 8
+                                  : a.col_l.planes + (uint64_t)l * a.col_l.nwc * 8;
+    const int nwc = (int)(swap ? a.col_r.nwc : a.col_l.nwc);
+    const int p = act ? lane_prefix(ca, cb, S) : 0;
+    const bool live = act && S > 0;
+    int m = 0;
+    const bool gt32 = __ballot(live && S > 32) != 0, gt64 = __ballot(live && S > 64) != 0;
+    const bool gt128 = __ballot(live && S > 128) != 0;
+    if (!gt32) m = lane_matches<1>(planes, nwc, S, T, outer, live);
+    else if (!gt64) m = lane_matches<2>(planes, nwc, S, T, outer, live);
+    else if (!gt128) m = lane_matches<4>(planes, nwc, S, T, outer, live);
+    else m = lane_matches<8>(planes, nwc, S, T, outer, live);
+    if (!live) m = 0;
+    emit_result(a, l, r, m, S, T, p, act);
+    evals += act;
+  }
+  for (int off = 32; off > 0; off >>= 1) evals += __shfl_xor(evals, off);
+  if (lane_id() == 0 && evals) atomicAdd(a.jw_count, evals);
+}
+
+// One wave per pair (scalar greedy); used for pairs deferred by
+// k_exact_lanes, i.e. shorter strings of more than 256 bytes.
 __global__ __launch_bounds__(256) void k_exact(ExactArgs a) {
   const uint32_t lane = lane_id();
   const unsigned long long n_in = *a.n_pairs;
@@ -587,6 +950,13 @@ void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint
                      bytes, width, n_rows, len, sig);
 }
 
+void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
+                         uint32_t *planes, hipStream_t stream) {
+  if (n_rows <= 0) return;
+  hipLaunchKernelGGL(k_build_planes, dim3(grid_for((uint64_t)n_rows, 256)), dim3(256), 0, stream,
+                     bytes, width, n_rows, nwc, planes);
+}
+
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream) {
   if (!n) return;
   hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256)), dim3(256), 0, stream, out, n);
@@ -618,35 +988,72 @@ void launch_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g, hipStream_t 
   hipLaunchKernelGGL(k_single_group, dim3(1), dim3(64), 0, stream, n_l, n_r, g);
 }
 
+void launch_gather_join(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
+                        EqCheck eq, int side, uint32_t *jcode, hipStream_t stream) {
+  if (!n || (!sig && eq.n == 0)) return;
+  hipLaunchKernelGGL(k_gather_join, dim3(grid_for(n, 256)), dim3(256), 0, stream, perm, n, sig,
+                     jsig, eq, side, jcode);
+}
+
 void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t *perm_r,
-                          const SigRow *sig_l, const SigRow *sig_r, EqCheck eq, int has_sim,
-                          const float coef[4], StageOut out, int n_cu, hipStream_t stream) {
+                          const SigRow *jsig_l, const SigRow *jsig_r, const uint32_t *jcode_l,
+                          const uint32_t *jcode_r, int n_eq, int has_sim, const float coef[4],
+                          StageOut out, FilterShape shape, int n_cu, hipStream_t stream) {
   FilterArgs a;
   a.g = g;
   a.perm_l = perm_l;
   a.perm_r = perm_r;
-  a.sig_l = sig_l;
-  a.sig_r = sig_r;
-  a.eq = eq;
-  a.has_sim = has_sim;
+  a.n_eq = n_eq;
   a.c0 = coef[0];
   a.c1 = coef[1];
   a.c2 = coef[2];
   a.c3 = coef[3];
   a.out = out;
-  const unsigned grid = (unsigned)(n_cu > 0 ? n_cu : 256) * 4;
-  hipLaunchKernelGGL(k_filter_tiles, dim3(grid), dim3(256), 0, stream, a);
-  hipLaunchKernelGGL(k_filter_small, dim3(grid), dim3(256), 0, stream, a);
+  // The tile grid holds up to 12 work-groups per CU so that a whole job's
+  // work items are resident at once; surplus groups find no item and exit.
+  const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
+  const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 4;
+  const uint32_t tile_b = shape.tile_b;
+#define HB_FILTER(NEQ, SIM)                                                                   \
+  do {                                                                                        \
+    if (shape.rows_per_lane == 2)                                                             \
+      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 2>), dim3(grid), dim3(256), 0, stream, a,  \
+                         jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                           \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 1>), dim3(grid), dim3(256), 0, stream, a,  \
+                         jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                           \
+    hipLaunchKernelGGL((k_filter_small<NEQ, SIM>), dim3(grid_small), dim3(256), 0, stream, a, \
+                       jsig_l, jsig_r, jcode_l, jcode_r);                                     \
+  } while (0)
+  // Three extra equalities run as four: the gathered codes repeat the third.
+  const int neq = n_eq == 3 ? 4 : n_eq;
+  if (has_sim) {
+    switch (neq) {
+    case 0: HB_FILTER(0, true); break;
+    case 1: HB_FILTER(1, true); break;
+    case 2: HB_FILTER(2, true); break;
+    default: HB_FILTER(4, true); break;
+    }
+  } else {
+    switch (neq) {
+    case 0: HB_FILTER(0, false); break;
+    case 1: HB_FILTER(1, false); break;
+    case 2: HB_FILTER(2, false); break;
+    default: HB_FILTER(4, false); break;
+    }
+  }
+#undef HB_FILTER
 }
 
 void launch_exact(const uint32_t *pairs, const unsigned long long *n_pairs,
                   unsigned long long in_cap, DevColumn col_l, DevColumn col_r, float threshold,
                   int32_t slot, StageOut out, AuditOut audit, unsigned long long *jw_count,
-                  int n_cu, hipStream_t stream) {
+                  StageOut defer, int n_cu, hipStream_t stream) {
   ExactArgs a;
   a.pairs = pairs;
   a.n_pairs = n_pairs;
   a.in_cap = in_cap;
+  a.defer = defer;
   a.col_l = col_l;
   a.col_r = col_r;
   a.threshold = threshold;
@@ -654,8 +1061,13 @@ void launch_exact(const uint32_t *pairs, const unsigned long long *n_pairs,
   a.out = out;
   a.audit = audit;
   a.jw_count = jw_count;
-  const unsigned grid = (unsigned)(n_cu > 0 ? n_cu : 256) * 8;
-  hipLaunchKernelGGL(k_exact, dim3(grid), dim3(256), 0, stream, a);
+  const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
+  hipLaunchKernelGGL(k_exact_lanes, dim3(cus * 8), dim3(256), 0, stream, a);
+  ExactArgs b = a;
+  b.pairs = defer.pairs;
+  b.n_pairs = defer.count;
+  b.in_cap = defer.cap;
+  hipLaunchKernelGGL(k_exact, dim3(cus * 4), dim3(256), 0, stream, b);
 }
 
 void launch_merge_unique(const uint32_t *pairs, const unsigned long long *n_pairs,

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -145,6 +146,7 @@ struct Job;
 struct hb_ctx {
   int device = 0;
   int n_cu = 256;
+  hb::FilterShape shape{2, 512, 12};  // filter launch shape; HB_FILTER_SHAPE overrides
   hipStream_t copy_stream = nullptr;
   std::vector<hipStream_t> streams;
   std::atomic<uint64_t> next_stream{0};
@@ -170,6 +172,8 @@ struct Column {
   Lazy sig;
   uint16_t *d_len = nullptr;
   hb::SigRow *d_sig = nullptr;
+  uint32_t *d_planes = nullptr;  // bit-planes of the first 32*nwc bytes
+  uint32_t nwc = 0;
   // equality predicates
   Lazy code;
   uint32_t *d_code = nullptr;
@@ -309,7 +313,10 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   const size_t n = (size_t)std::max<int64_t>(t->n_rows, 1);
   HIP_OK(hipMalloc(&col.d_len, n * sizeof(uint16_t)));
   HIP_OK(hipMalloc(&col.d_sig, n * sizeof(hb::SigRow)));
+  col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
+  HIP_OK(hipMalloc(&col.d_planes, n * col.nwc * 32));
   hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
+  hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
   HIP_OK(hipGetLastError());
   return record_lazy(col.sig, stream);
 }
@@ -405,6 +412,8 @@ hb_status enqueue_job(Job *j) {
   list[0] = (uint32_t *)p;
   HB_TRY(job_alloc(j, j->list_cap * 8, &p));
   list[1] = (uint32_t *)p;
+  HB_TRY(job_alloc(j, j->list_cap * 8, &p));
+  uint32_t *d_defer = (uint32_t *)p;
   HB_TRY(job_alloc(j, j->out_cap * 8, &p));
   j->d_out = (uint32_t *)p;
   HB_TRY(job_alloc(j, j->audit_cap * sizeof(hb_audit_entry), &p));
@@ -426,8 +435,22 @@ hb_status enqueue_job(Job *j) {
   iota_l = (uint32_t *)p;
   HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * 4, &p));
   iota_r = (uint32_t *)p;
+  // Signatures and extra equality codes in join order, reused by every rule.
+  hb::SigRow *jsig_l, *jsig_r;
+  uint32_t *jcode_l, *jcode_r;
+  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_l, 1) * sizeof(hb::SigRow), &p));
+  jsig_l = (hb::SigRow *)p;
+  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * sizeof(hb::SigRow), &p));
+  jsig_r = (hb::SigRow *)p;
+  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_l, 1) * 4 * hb::kMaxEq, &p));
+  jcode_l = (uint32_t *)p;
+  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * 4 * hb::kMaxEq, &p));
+  jcode_r = (uint32_t *)p;
   hb::GroupArrays g;
   g.cap = n_l;
+  g.tile_a = 256u * (uint32_t)ctx->shape.rows_per_lane;
+  g.tile_b = ctx->shape.tile_b;
+  g.small_pairs_max = 16384;
   HB_TRY(job_alloc(j, 4 * sizeof(uint32_t), &p));
   g.n_runs = (uint32_t *)p;
   uint32_t *u32arr[4];
@@ -519,6 +542,17 @@ hb_status enqueue_job(Job *j) {
       sig_r = R->cols[rule.sims[0].cr].d_sig;
       for (int q = 0; q < 4; q++) coef[q] = filter_coef((double)rule.sims[0].thr, q);
     }
+    // Join order: gathered copies, or the rows themselves without a join.
+    const hb::SigRow *jl = sig_l, *jr = sig_r;
+    if (!rule.eq.empty()) {
+      hb::launch_gather_join(perm_l, n_l, sig_l, jsig_l, eq, 0, jcode_l, st);
+      hb::launch_gather_join(perm_r, n_r, sig_r, jsig_r, eq, 1, jcode_r, st);
+      HIP_OK(hipGetLastError());
+      if (sig_l) {
+        jl = jsig_l;
+        jr = jsig_r;
+      }
+    }
     hb::StageOut prev{};
     HB_TRY(mark(j, st));
     for (int s = 0; s < n_stages; s++) {
@@ -534,16 +568,20 @@ hb_status enqueue_job(Job *j) {
         stage++;
       }
       if (s == 0) {
-        hb::launch_filter_groups(gu, perm_l, perm_r, sig_l, sig_r, eq,
-                                 rule.sims.empty() ? 0 : 1, coef, o, ctx->n_cu, st);
+        hb::launch_filter_groups(gu, perm_l, perm_r, jl, jr, jcode_l, jcode_r, eq.n,
+                                 rule.sims.empty() ? 0 : 1, coef, o, ctx->shape, ctx->n_cu, st);
       } else {
         const auto &sim = rule.sims[s - 1];
         const Column &cl = L->cols[sim.cl];
         const Column &cr = R->cols[sim.cr];
-        hb::DevColumn dl{cl.d_bytes, cl.d_len, cl.width};
-        hb::DevColumn dr{cr.d_bytes, cr.d_len, cr.width};
+        hb::DevColumn dl{cl.d_bytes, cl.d_len, cl.width, cl.d_planes, cl.nwc};
+        hb::DevColumn dr{cr.d_bytes, cr.d_len, cr.width, cr.d_planes, cr.nwc};
+        if (stage >= j->n_counters)
+          return fail(HB_ERR_INVALID_ARG, "internal: stage counters exhausted");
+        hb::StageOut defer{d_defer, j->d_cnt + stage, j->list_cap, j->d_cnt + kCntOverflow};
+        stage++;
         hb::launch_exact(prev.pairs, prev.count, prev.cap, dl, dr, sim.thr, sim.slot, o, audit,
-                         j->d_cnt + kCntJw, ctx->n_cu, st);
+                         j->d_cnt + kCntJw, defer, ctx->n_cu, st);
       }
       HIP_OK(hipGetLastError());
       prev = o;
@@ -688,6 +726,12 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, device));
   ctx->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (const char *env = std::getenv("HB_FILTER_SHAPE")) {
+    int rpl = 0, tb = 0, gpc = 0;
+    if (std::sscanf(env, "%d,%d,%d", &rpl, &tb, &gpc) == 3 && (rpl == 1 || rpl == 2) &&
+        tb >= 256 && tb % 256 == 0 && gpc >= 1 && gpc <= 64)
+      ctx->shape = hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc};
+  }
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; i++) {
     hipStream_t s;
@@ -826,6 +870,7 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
     if (c.h_bytes) (void)hipHostFree(c.h_bytes);
     (void)hipFree(c.d_len);
     (void)hipFree(c.d_sig);
+    (void)hipFree(c.d_planes);
     (void)hipFree(c.d_code);
     if (c.h_code) (void)hipHostFree(c.h_code);
     (void)hipFree(c.d_sorted);
@@ -923,7 +968,7 @@ hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, con
   while (sc < 2 * j->out_cap) sc <<= 1;
   j->set_cap = sc;
   j->n_counters = kCntStage0;
-  for (const PlanRule &rule : plan->rules) j->n_counters += 1 + (int)rule.sims.size();
+  for (const PlanRule &rule : plan->rules) j->n_counters += 1 + 2 * (int)rule.sims.size();
   HIP_OK(hipHostMalloc((void **)&j->h_cnt, (size_t)j->n_counters * sizeof(unsigned long long),
                        hipHostMallocDefault));
   HIP_OK(hipEventCreate(&j->ev_start));
