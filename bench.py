@@ -8,7 +8,7 @@ kernel, copy the matched pairs back.  Inputs are uploaded before the timed
 region; per-table device structures (signatures, join index) are built
 by the warmup jobs, and the cold first job is reported separately.
 
-Prints one JSON line (see the README of the bench contract in DESIGN.md).
+Prints one JSON line.
 """
 import argparse
 import json
@@ -19,12 +19,27 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# int32 lane-ops per candidate pair of the bound filter stage (the dominant
-# kernel): 8 sum-of-absolute-differences over 32 byte-class counters packed
-# 4 per dword, 2 for the common-prefix length, 2 for min/max of the lengths,
-# 3 for the bound (match bound, S+T and S*T products), 1 compare.
-FILTER_OPS_PER_CANDIDATE = 16
+# Algorithmic int32 lane-ops per candidate pair of the bound filter (the
+# dominant kernel): the test "SAD of the 32 byte-class counters > allowance
+# for (first-bytes class, S, T)" needs 8 v_sad_u8 (32 counters, 4 per
+# dword), 1 XOR of the 4-byte prefixes, 2 compares for the prefix class,
+# 1 select of the allowance and 1 final compare.
+FILTER_OPS_PER_CANDIDATE = 13
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 int32 lanes/clk x 2.4 GHz
+HBM_PEAK_GBS = 8000.0
+# Filter-stage HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# scaled by a copy kernel of known size (summary and commands in this file).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_filter_tiles_c2_hbm_traffic.txt")
+
+
+def measured_traffic(workload):
+    """traffic_bytes_per_launch recorded by the PMC runs, or None."""
+    if workload != "c2" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(line.split("=")[1])
+    return None
 
 
 def cpu_baseline(cfg, L, R, plan, budget_rows):
@@ -117,11 +132,14 @@ def main():
     filt_ms = float(np.mean([s["filter_ms"] for s in stats]))
     cand = stats[-1]["candidates"]
     achieved = cand * FILTER_OPS_PER_CANDIDATE / (filt_ms * 1e-3) / 1e12
+    traffic = measured_traffic(args.workload)
     roofline = {"bound": "valu", "kernel": "k_filter_tiles+k_filter_small",
                 "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
-                "frac": achieved / VALU_PEAK_TOPS, "traffic": None,
+                "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
                 "ops_per_candidate": FILTER_OPS_PER_CANDIDATE, "candidates_per_launch": cand,
-                "avg_launch_ms": filt_ms}
+                "avg_launch_ms": filt_ms,
+                "hbm_gbs": None if traffic is None else traffic / (filt_ms * 1e-3) / 1e9,
+                "hbm_peak_gbs": HBM_PEAK_GBS}
     cpu = None
     if args.cpu_rows > 0:
         cpu = cpu_baseline(cfg, L, R, plan, args.cpu_rows)

@@ -163,8 +163,9 @@ hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
                         const hb_table *r, uint64_t job_id);
 hb_status hb_job_query(hb_ctx *ctx, uint64_t job_id); /* HB_OK or HB_NOT_READY */
 hb_status hb_job_wait(hb_ctx *ctx, uint64_t job_id);
-/* (l_row, r_row) pairs as 2*n_pairs uint32, sorted by (l, r), each pair
- * once.  Rows index the job's L and R tables. */
+/* (l_row, r_row) pairs as 2*n_pairs uint32, each matching pair exactly once,
+ * in unspecified order (the reference's order is that of its atomicAdd).
+ * Rows index the job's L and R tables. */
 hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs,
                          uint64_t *n_pairs, const hb_audit_entry **audit,
                          uint64_t *n_audit);
@@ -221,10 +222,14 @@ hb_status hb_write_pairs(const char *path, int32_t append, const hb_host_table *
                          const int64_t *r_rows, const uint32_t *pairs, uint64_t n_pairs);
 
 /* ---- filter coefficients (exposed for host-side verification) -----------
- * The bound filter rejects a candidate pair with inner length S, outer
- * length T, common prefix p < 4 and match upper bound m when
- * float(m * (S + T)) < coef[p] * float(S * T).  coef[p] is chosen so that
- * every rejected pair has Jaro-Winkler below threshold - 2e-6. */
+ * The bound filter rejects a candidate pair of cell lengths S and T (1..255)
+ * with match upper bound m (sum over 32 byte classes of the smaller count)
+ * when, in float arithmetic, (2m) * (1/S + 1/T) < 2 * coef[q], where q = 0
+ * if the first bytes differ, q = 3 if a later one of the first 4 bytes
+ * differs, and no pair with equal first 4 bytes is rejected.  coef[p] is the
+ * Jaro bound 3 * J - 1 at which a common prefix of p (the largest allowed by
+ * q) could still reach threshold - 2e-6, lowered by 1e-6 relative, so every
+ * rejected pair has Jaro-Winkler below threshold - 2e-6. */
 hb_status hb_filter_coefs(float threshold, float coef[4]);
 
 /* ---- device arithmetic check ---------------------------------------------

@@ -173,6 +173,11 @@ double hbo_jw_from_counts(int32_t m, int32_t S, int32_t T, int32_t p) {
   return winkler(jaro, (size_t)p, 0);
 }
 
+void hbo_jw_from_counts_batch(const int32_t *mstp, int64_t n, double *out) {
+  for (int64_t i = 0; i < n; i++)
+    out[i] = hbo_jw_from_counts(mstp[4 * i], mstp[4 * i + 1], mstp[4 * i + 2], mstp[4 * i + 3]);
+}
+
 /* Second formulation: for every outer position i < min(T, S + h), take the
  * lowest free inner position j with s[j] == t[i] inside the window
  * [max(0, i-h), min(i+h, S-1)], where for S >= 257 loop 1 (i < h) stops

@@ -80,6 +80,9 @@ void hbo_jaro_counts(const char *a, const char *b, int32_t *m, int32_t *S,
  * function of the counts (device_func.cuh:113-128, 222-230). */
 double hbo_jw_from_counts(int32_t m, int32_t S, int32_t T, int32_t p);
 
+/* hbo_jw_from_counts over n tuples mstp[4*i..4*i+3] = (m, S, T, p). */
+void hbo_jw_from_counts_batch(const int32_t *mstp, int64_t n, double *out);
+
 /* An independently structured formulation of the same greedy matching: a
  * symmetric window of radius h around each outer position and a bitmask of
  * free inner positions.  Used only to cross-check hbo_jaro. */

@@ -73,6 +73,9 @@ def lib():
         L.hbo_jw_work.restype = C.c_uint64
         L.hbo_jw_from_counts.argtypes = [C.c_int32] * 4
         L.hbo_jw_from_counts.restype = C.c_double
+        L.hbo_jw_from_counts_batch.argtypes = [C.POINTER(C.c_int32), C.c_int64,
+                                               C.POINTER(C.c_double)]
+        L.hbo_jw_from_counts_batch.restype = None
         L.hbo_jaro_counts.argtypes = [C.c_char_p, C.c_char_p] + [C.POINTER(C.c_int32)] * 4
         L.hbo_jaro_counts.restype = None
         L.hbo_plan_check.argtypes = [C.POINTER(_Plan)]
@@ -124,6 +127,15 @@ def jw_work(a, b):
 
 def jw_from_counts(m, S, T, p):
     return float(lib().hbo_jw_from_counts(m, S, T, p))
+
+
+def jw_from_counts_batch(mstp):
+    """mstp: int32 array [n, 4] of (m, S, T, p) -> float64 [n]."""
+    t = np.ascontiguousarray(np.asarray(mstp, dtype=np.int32).reshape(-1, 4))
+    out = np.zeros(len(t), np.float64)
+    lib().hbo_jw_from_counts_batch(t.ctypes.data_as(C.POINTER(C.c_int32)), len(t),
+                                   out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
 
 
 def jaro_counts(a, b):

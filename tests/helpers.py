@@ -24,6 +24,43 @@ def columns_from_rows(rows):
     return cols
 
 
+def host_columns(table):
+    """HostTable (CSV read by the product reader) -> numpy 'S' columns."""
+    return columns_from_rows([tuple(table.cell(r, c) for c in range(table.n_cols))
+                              for r in range(table.n_rows)])
+
+
+FIXTURES = {
+    # name: (left csv, right csv, rule dir), all under tests/golden
+    "dblp_acm": ("dblp_acm/tableA.csv", "dblp_acm/tableB.csv", "dblp_acm/rules"),
+    "dblp_google_self": ("dblp_google/tableA.csv", "dblp_google/tableA.csv",
+                         "dblp_google/rules"),
+}
+
+
+def load_fixture(name):
+    """(HostTable L, HostTable R, HostPlan) of a CSV fixture."""
+    lp, rp, rd = (os.path.join(GOLDEN, x) for x in FIXTURES[name])
+    return hb.HostTable.from_csv(lp), hb.HostTable.from_csv(rp), hb.HostPlan.from_rule_dir(rd)
+
+
+def read_pairs(path):
+    a = np.loadtxt(path, dtype=np.uint32, ndmin=2)
+    return a.reshape(-1, 2)
+
+
+def audit_to_json(a):
+    return [[int(e["l"]), int(e["r"]), int(e["slot"]), int(e["flags"]), float(e["jw"]).hex()]
+            for e in sort_audit(a)]
+
+
+def audit_from_json(rows):
+    out = np.zeros(len(rows), dtype=O.AUDIT_DTYPE)
+    for i, (l, r, slot, flags, jw) in enumerate(rows):
+        out[i] = (l, r, slot, flags, float.fromhex(jw))
+    return out
+
+
 def oracle_plan(plan):
     return O.Plan(plan.n_rules, plan.pred_index, plan.pred_type.encode(), plan.pred_threshold)
 
