@@ -196,6 +196,18 @@ class Context:
     def synchronize(self):
         check(lib().hb_ctx_synchronize(self._h))
 
+    @property
+    def filter_shape(self):
+        """(rows per lane, B rows per work item, work-groups per CU) of the
+        bound-filter kernel."""
+        v = [C.c_int32() for _ in range(3)]
+        check(lib().hb_ctx_get_filter_shape(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    @filter_shape.setter
+    def filter_shape(self, shape):
+        check(lib().hb_ctx_set_filter_shape(self._h, *[int(x) for x in shape]))
+
     def __enter__(self):
         return self
 

@@ -26,7 +26,8 @@ CHECK_POINT = 2147483647
 
 # Every symbol include/hyperblocker_amd.h declares.
 EXPORTS = [
-    "hb_ctx_create", "hb_ctx_destroy", "hb_ctx_synchronize", "hb_last_error", "hb_version",
+    "hb_ctx_create", "hb_ctx_destroy", "hb_ctx_synchronize", "hb_ctx_set_filter_shape",
+    "hb_ctx_get_filter_shape", "hb_last_error", "hb_version",
     "hb_table_upload", "hb_table_upload_serialized", "hb_table_upload_rows",
     "hb_table_info", "hb_table_release",
     "hb_plan_create", "hb_plan_release",
@@ -99,6 +100,8 @@ def lib():
         "hb_ctx_create": ([C.c_int32, PVP], st),
         "hb_ctx_destroy": ([VP], st),
         "hb_ctx_synchronize": ([VP], st),
+        "hb_ctx_set_filter_shape": ([VP, C.c_int32, C.c_int32, C.c_int32], st),
+        "hb_ctx_get_filter_shape": ([VP] + [C.POINTER(C.c_int32)] * 3, st),
         "hb_last_error": ([], C.c_char_p),
         "hb_version": ([], C.c_char_p),
         "hb_table_upload": ([VP, C.POINTER(TableView), PVP], st),

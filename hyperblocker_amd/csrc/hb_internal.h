@@ -73,7 +73,7 @@ struct GroupArrays {
 
 // Launch shape of the filter stage (chosen per device by the autotuner).
 struct FilterShape {
-  int rows_per_lane;   // 1 or 2
+  int rows_per_lane;   // 1, 2 or 4
   uint32_t tile_b;     // multiple of 256
   unsigned grid_per_cu;
 };
@@ -103,10 +103,18 @@ void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
                          uint32_t *planes, hipStream_t stream);
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
-void launch_run_flags(const uint32_t *sorted_keys, uint32_t n, uint32_t *flags,
+// Join order of a table for one rule: keys (code << 8) | min(length, 255)
+// (code or length may be absent: null pointer), sorted with the row ids.
+void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,
+                        hipStream_t stream);
+void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
+                       hipStream_t stream);
+// Runs of equal code (key >> 8) in the sorted left keys, matched with the
+// sorted right keys.
+void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
                       hipStream_t stream);
-void launch_run_info(const uint32_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
-                     uint32_t n_l, const uint32_t *sorted_r, uint32_t n_r, GroupArrays g,
+void launch_run_info(const uint64_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
+                     uint32_t n_l, const uint64_t *sorted_r, uint32_t n_r, GroupArrays g,
                      hipStream_t stream);
 void launch_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g, hipStream_t stream);
 
@@ -147,6 +155,10 @@ void launch_jw_from_counts(const int32_t *mstp, int64_t n, double *out, double *
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in,
                           uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint32_t n, hipStream_t stream);
+// Stable radix sort of (key, value) pairs on key bits [0, end_bit).
+hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const uint64_t *keys_in,
+                          uint64_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
+                          uint32_t n, int end_bit, hipStream_t stream);
 hipError_t exscan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out,
                       uint32_t n, hipStream_t stream);
 hipError_t exscan_u64(void *temp, size_t &temp_bytes, const uint64_t *in, uint64_t *out,

@@ -249,25 +249,46 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long n, unsigned
     p[i] = v;
 }
 
-__global__ void k_run_flags(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *flags) {
+// Join-order keys: (dictionary code << 8) | min(length, 255) of the rule's
+// first similarity column (0 without one), or the length alone when the rule
+// has no equality.  Sorting them groups rows by code and orders each group by
+// length.
+__global__ void k_layout_keys(const uint32_t *__restrict__ code, const uint16_t *__restrict__ len,
+                              uint32_t n, uint64_t *__restrict__ keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+  if (i >= n) return;
+  const uint64_t c = code ? (uint64_t)code[i] : 0ull;
+  const uint32_t l = len ? min((uint32_t)len[i], 255u) : 0u;
+  keys[i] = (c << 8) | l;
 }
 
-__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t *a, uint32_t lo, uint32_t hi,
-                                                    uint32_t key) {
+__global__ void k_gather_sig(const uint32_t *__restrict__ perm, uint32_t n,
+                             const SigRow *__restrict__ sig, SigRow *__restrict__ jsig) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) jsig[i] = sig[perm[i]];
+}
+
+__global__ void k_run_flags(const uint64_t *__restrict__ keys, uint32_t n, uint32_t *flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = (i == 0 || (keys[i] >> 8) != (keys[i - 1] >> 8)) ? 1u : 0u;
+}
+
+// First index in [lo, hi) whose code (key >> 8) exceeds `code`.
+__device__ __forceinline__ uint32_t upper_bound_code(const uint64_t *a, uint32_t lo, uint32_t hi,
+                                                     uint64_t code) {
   while (lo < hi) {
     const uint32_t mid = lo + ((hi - lo) >> 1);
-    if (a[mid] <= key) lo = mid + 1; else hi = mid;
+    if ((a[mid] >> 8) <= code) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
 
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t lo, uint32_t hi,
-                                                    uint32_t key) {
+// First index in [lo, hi) whose code is not below `code`.
+__device__ __forceinline__ uint32_t lower_bound_code(const uint64_t *a, uint32_t lo, uint32_t hi,
+                                                     uint64_t code) {
   while (lo < hi) {
     const uint32_t mid = lo + ((hi - lo) >> 1);
-    if (a[mid] < key) lo = mid + 1; else hi = mid;
+    if ((a[mid] >> 8) < code) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
@@ -286,9 +307,9 @@ __device__ __forceinline__ void set_group_work(GroupArrays g, uint32_t gi, uint3
   g.items[gi] = items;
 }
 
-__global__ void k_run_info(const uint32_t *__restrict__ sl, const uint32_t *__restrict__ flags,
+__global__ void k_run_info(const uint64_t *__restrict__ sl, const uint32_t *__restrict__ flags,
                            const uint32_t *__restrict__ run_id, uint32_t n_l,
-                           const uint32_t *__restrict__ sr, uint32_t n_r, GroupArrays g) {
+                           const uint64_t *__restrict__ sr, uint32_t n_r, GroupArrays g) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n_l) return;
   const uint32_t n_runs = n_l ? run_id[n_l - 1] + flags[n_l - 1] : 0u;
@@ -300,10 +321,10 @@ __global__ void k_run_info(const uint32_t *__restrict__ sl, const uint32_t *__re
   }
   if (i < n_l && flags[i]) {
     const uint32_t gi = run_id[i];
-    const uint32_t key = sl[i];
-    const uint32_t l_end = upper_bound_u32(sl, i, n_l, key);
-    const uint32_t r_lo = lower_bound_u32(sr, 0, n_r, key);
-    const uint32_t r_hi = upper_bound_u32(sr, r_lo, n_r, key);
+    const uint64_t code = sl[i] >> 8;
+    const uint32_t l_end = upper_bound_code(sl, i, n_l, code);
+    const uint32_t r_lo = lower_bound_code(sr, 0, n_r, code);
+    const uint32_t r_hi = upper_bound_code(sr, r_lo, n_r, code);
     g.l_off[gi] = i;
     g.l_cnt[gi] = l_end - i;
     g.r_off[gi] = r_lo;
@@ -353,44 +374,91 @@ struct FilterArgs {
   StageOut out;
 };
 
-// The bound test of may_pass for a lane-held A signature against a B row
-// that is the same for the whole wave (scalar registers), in the form
-//   reject  <=>  t * (1/la + 1/lb) < 2 * coef
-// with t = la + lb - SAD = 2 * sum over classes of min(count_a, count_b);
-// this is m_ub * (S + T) < coef * S * T divided by S * T / 2 (S + T = la + lb,
-// S * T = la * lb).  coef is coef[0] when the first bytes differ and coef[3]
-// (the smallest of coef[1..3], so never stricter) when only a later one of
-// the first 4 bytes differs; with no difference there the pair passes.  The
-// reciprocals and the two float operations add at most 1.8e-7 relative
-// error to the left side, inside the coefficients' 1e-6 relative margin.
-// An empty or over-long cell has rlen = +inf, which makes the left side
-// infinite or NaN and passes the pair.
-__device__ __forceinline__ bool bound_pass(const uint32_t (&ah)[8], uint32_t apre, uint32_t la,
-                                           float rla, const SigRow &b, float k0, float k3) {
-  uint32_t sad = __builtin_amdgcn_sad_u8(ah[0], b.h[0], 0u);
+// The bound test of may_pass in allowance form, for a lane-held A row and a
+// B row shared by the wave.  may_pass rejects when
+//   t * (1/la + 1/lb) < 2 * coef,   t = la + lb - SAD = 2 * sum of min counts
+// (m_ub * (S + T) < coef * S * T divided by S * T / 2), evaluated in float as
+// (float)t * (rla + rlb) < k.  For fixed lengths the left side does not
+// decrease with t, so the rejected t are exactly those below
+//   T* = min { t : !((float)t * (rla + rlb) < k) },
+// i.e. the pair is rejected iff SAD > allow = la + lb - T*.  neg_allowance
+// returns -allow (two's complement) found by that same float comparison, so
+// decisions equal the direct form bit for bit; seeding the SAD chain with it
+// leaves SAD - allow, positive iff rejected.  An empty or over-long cell has
+// rlen = +inf and is never rejected, nor is any pair when k <= 0.
+constexpr int32_t kPassAllow = 1 << 30;
+
+__device__ __forceinline__ uint32_t neg_allowance(int32_t tmax, float s, float q, float k) {
+  // t * s crosses k within 1 of ceil(q), q ~ k / s: |k / s| <= 511 and q and
+  // every float operation are off by at most 2^-22 relative, so one step
+  // either way lands on T* (clamped to [1, tmax + 1]; T* >= 1 as k > 0).
+  if (!(k > 0.0f)) return (uint32_t)(-kPassAllow);
+  int32_t t = (int32_t)ceilf(q);
+  t = max(1, min(t, tmax + 1));
+  if (!((float)(t - 1) * s < k))
+    t -= 1;
+  else if (t <= tmax && (float)t * s < k)
+    t += 1;
+  return (uint32_t)(t - tmax);
+}
+
+// Negated allowances of one A row for one B length, both prefix classes.
+__device__ __forceinline__ void allowances(uint32_t la, float rla, uint32_t lb, float rlb,
+                                           float k0, float k3, uint32_t &nal0, uint32_t &nal3) {
+  const float s = rla + rlb;
+  if (!(s < INFINITY)) {  // an empty or over-long cell: never rejected
+    nal0 = nal3 = (uint32_t)(-kPassAllow);
+    return;
+  }
+  const float inv = __builtin_amdgcn_rcpf(s);
+  const int32_t tmax = (int32_t)((la & 0xffffu) + (lb & 0xffffu));
+  nal0 = neg_allowance(tmax, s, k0 * inv, k0);
+  nal3 = neg_allowance(tmax, s, k3 * inv, k3);
+}
+
+// coef[0] applies when the first bytes differ, coef[3] (the smallest of
+// coef[1..3], so never stricter) when only a later one of the first 4 bytes
+// differs, and nothing is rejected when the first 4 bytes are equal.  With
+// the prefixes byte-swapped (first byte most significant), x = pre_a ^ pre_b
+// is >= 2^24 exactly when the first bytes differ.
+__device__ __forceinline__ bool bound_keep(const uint32_t (&ah)[8], uint32_t apre_be,
+                                           uint32_t nal0, uint32_t nal3, const uint32_t (&bh)[8],
+                                           uint32_t bpre_be) {
+  const uint32_t x = apre_be ^ bpre_be;
+  uint32_t acc = x >= 0x01000000u ? nal0 : nal3;
 #pragma unroll
-  for (int k = 1; k < 8; k++) sad = __builtin_amdgcn_sad_u8(ah[k], b.h[k], sad);
-  const uint32_t t = la + (b.len & 0xffffu) - sad;
-  const uint32_t x = apre ^ b.pre;
-  const float k = (x & 0xffu) ? k0 : (x ? k3 : -INFINITY);
-  return !((float)t * (rla + b.rlen) < k);
+  for (int k = 0; k < 8; k++) acc = __builtin_amdgcn_sad_u8(ah[k], bh[k], acc);
+  return (int32_t)acc <= 0 || x == 0u;
 }
 
 // Work item = 256*AR rows of the larger side in lanes (A, AR rows per lane)
 // x up to tile_b rows of the other side (B).  B rows are staged through LDS
 // in chunks of 256 and read as wave-wide broadcasts, two at a time, so each
-// LDS round trip feeds 2*AR independent pair tests.  NEQ = equality
-// predicates checked per pair beyond the join key.
-template <int NEQ, bool SIM, int AR>
-__global__ __launch_bounds__(256) void k_filter_tiles(const FilterArgs a,
+// LDS round trip feeds 2*AR independent pair tests.  Inside a join group the
+// rows are ordered by length, so a chunk splits into a few runs of equal B
+// length; the per-lane allowances are computed once per run and the rows of
+// a run are tested without further branching.  NEQ = equality predicates
+// checked per pair beyond the join key.
+// B row as staged in LDS: byte-class counters, byte-swapped prefix, length.
+struct __attribute__((aligned(8))) BRow {
+  uint32_t h[8];
+  uint32_t pre;
+  uint32_t len;
+};
+
+template <int NEQ, bool SIM, int AR, int BU>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6 : 8))) void k_filter_tiles(
+    const FilterArgs a,
                                                       const SigRow *__restrict__ jsig_l,
                                                       const SigRow *__restrict__ jsig_r,
                                                       const uint32_t *__restrict__ jcode_l,
                                                       const uint32_t *__restrict__ jcode_r,
                                                       uint32_t tile_b) {
   constexpr uint32_t kChunk = 256;
-  __shared__ SigRow s_sig[SIM ? kChunk + 1 : 1];
+  __shared__ BRow s_b[SIM ? kChunk + 1 : 1];
   __shared__ uint32_t s_code[NEQ > 0 ? NEQ : 1][kChunk + 1];
+  __shared__ unsigned long long s_starts[kChunk / 64];  // bit i: B row i starts a run
+  __shared__ float s_rlen[SIM ? kChunk + 1 : 1];
   __shared__ BlockOut bo;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) bo.n = 0;
@@ -414,29 +482,30 @@ __global__ __launch_bounds__(256) void k_filter_tiles(const FilterArgs a,
     const uint64_t local = item - a.g.item_off[gi];
     const uint32_t a_tile = (uint32_t)(local / n_bseg), b_seg = (uint32_t)(local % n_bseg);
     bool a_valid[AR];
-    uint32_t a_row[AR], ah[AR][8], apre[AR], la[AR], acode[AR][NEQ > 0 ? NEQ : 1];
+    uint32_t a_idx[AR], ah[AR][8], apre[AR], la[AR], acode[AR][NEQ > 0 ? NEQ : 1];
+    uint32_t nal0[AR], nal3[AR];
     float rla[AR];
 #pragma unroll
     for (int r = 0; r < AR; r++) {
-      const uint32_t a_idx = a_tile * 256 * AR + r * 256 + tid;
-      a_valid[r] = a_idx < a_cnt;
-      a_row[r] = a_valid[r] ? perm_a[a_off + a_idx] : 0u;
+      a_idx[r] = a_tile * 256 * AR + r * 256 + tid;
+      a_valid[r] = a_idx[r] < a_cnt;
 #pragma unroll
       for (int k = 0; k < 8; k++) ah[r][k] = 0;
       apre[r] = 0;
       la[r] = 0;
       rla[r] = INFINITY;
+      nal0[r] = nal3[r] = (uint32_t)(-kPassAllow);
       if (SIM && a_valid[r]) {
-        const SigRow s = sig_a[a_idx];
+        const SigRow s = sig_a[a_idx[r]];
 #pragma unroll
         for (int k = 0; k < 8; k++) ah[r][k] = s.h[k];
-        apre[r] = s.pre;
-        la[r] = s.len & 0xffffu;
+        apre[r] = __builtin_bswap32(s.pre);
+        la[r] = s.len;
         rla[r] = s.rlen;
       }
 #pragma unroll
       for (int k = 0; k < NEQ; k++)
-        acode[r][k] = a_valid[r] ? code_a[(uint64_t)a_idx * kMaxEq + k] : 0u;
+        acode[r][k] = a_valid[r] ? code_a[(uint64_t)a_idx[r] * kMaxEq + k] : 0u;
     }
     const uint32_t b_begin = b_seg * tile_b;
     const uint32_t b_end = min(b_begin + tile_b, b_cnt);
@@ -444,51 +513,90 @@ __global__ __launch_bounds__(256) void k_filter_tiles(const FilterArgs a,
       const uint32_t n_b = min(kChunk, b_end - cb);
       __syncthreads();
       if (bo.n >= kBlockOut / 2) block_flush(bo, a.out);  // bo.n is group-uniform here
+      uint32_t my_len = 0;
       if (tid < n_b) {
-        if (SIM) s_sig[tid] = sig_b[cb + tid];
+        if (SIM) {
+          const SigRow bs = sig_b[cb + tid];
+          BRow br;
+#pragma unroll
+          for (int k = 0; k < 8; k++) br.h[k] = bs.h[k];
+          br.pre = __builtin_bswap32(bs.pre);
+          br.len = my_len = bs.len;
+          s_b[tid] = br;
+          s_rlen[tid] = bs.rlen;
+        }
 #pragma unroll
         for (int k = 0; k < NEQ; k++) s_code[k][tid] = code_b[(uint64_t)(cb + tid) * kMaxEq + k];
       }
-      if (tid == 0) {  // odd chunk: a padding row no lane can pass
-        if (SIM) {
-          SigRow z = {};
-          z.rlen = INFINITY;
-          s_sig[n_b] = z;
-        }
+      if (tid == 0) {  // odd run end: a padding row, masked out below
+        if (SIM) s_b[n_b] = BRow{};
 #pragma unroll
         for (int k = 0; k < NEQ; k++) s_code[k][n_b] = 0;
       }
       __syncthreads();
-      for (uint32_t jj = 0; jj < n_b; jj += 2) {
-        const bool second = jj + 1 < n_b;
-        bool ok[2][AR];
+      if (SIM) {  // run starts: rows whose length differs from the previous row's
+        const bool start = tid < n_b && (tid == 0 || s_b[tid - 1].len != my_len);
+        const unsigned long long m = __ballot(start);
+        if (lane_id() == 0) s_starts[tid >> 6] = m;
+        __syncthreads();
+      }
+      uint32_t run = 0;
+      while (run < n_b) {  // wave-uniform
+        uint32_t run_end = n_b;
+        if (SIM) {
+          // next set bit of s_starts after `run`
+          for (uint32_t w = (run + 1) >> 6; w < kChunk / 64; w++) {
+            unsigned long long m = __builtin_amdgcn_readfirstlane(
+                                       (uint32_t)s_starts[w]) |
+                                   ((unsigned long long)__builtin_amdgcn_readfirstlane(
+                                        (uint32_t)(s_starts[w] >> 32))
+                                    << 32);
+            if (w == (run + 1) >> 6) m &= ~0ull << ((run + 1) & 63);
+            if (m) {
+              run_end = min(n_b, (w << 6) + (uint32_t)__builtin_ctzll(m));
+              break;
+            }
+          }
+          const uint32_t blen = s_b[run].len;
+          const float rlb = s_rlen[run];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const SigRow b = SIM ? s_sig[jj + u] : SigRow{};
+          for (int r = 0; r < AR; r++) allowances(la[r], rla[r], blen, rlb, k0, k3, nal0[r], nal3[r]);
+        }
+        for (uint32_t jj = run; jj < run_end; jj += BU) {
+          const bool second = jj + 1 < run_end;
+          bool ok[BU][AR];
 #pragma unroll
-          for (int r = 0; r < AR; r++) {
-            bool o = a_valid[r] && (u == 0 || second);
+          for (int u = 0; u < BU; u++) {
+            const BRow b = SIM ? s_b[jj + u] : BRow{};
 #pragma unroll
-            for (int k = 0; k < NEQ; k++) o &= (acode[r][k] == s_code[k][jj + u]);
-            if (SIM) o &= bound_pass(ah[r], apre[r], la[r], rla[r], b, k0, k3);
-            ok[u][r] = o;
+            for (int r = 0; r < AR; r++) {
+              bool o = a_valid[r] && (u == 0 || second);
+#pragma unroll
+              for (int k = 0; k < NEQ; k++) o &= (acode[r][k] == s_code[k][jj + u]);
+              if (SIM) o &= bound_keep(ah[r], apre[r], nal0[r], nal3[r], b.h, b.pre);
+              ok[u][r] = o;
+            }
+          }
+          bool any = false;
+#pragma unroll
+          for (int u = 0; u < BU; u++)
+#pragma unroll
+            for (int r = 0; r < AR; r++) any |= ok[u][r];
+          if (__builtin_amdgcn_ballot_w64(any)) {  // rare: a survivor in this step
+#pragma unroll
+            for (int u = 0; u < BU; u++) {
+#pragma unroll
+              for (int r = 0; r < AR; r++) {
+                if (__builtin_amdgcn_ballot_w64(ok[u][r]) == 0) continue;
+                const uint32_t b_row = perm_b[b_off + cb + min(jj + u, n_b - 1)];
+                const uint32_t a_row = ok[u][r] ? perm_a[a_off + a_idx[r]] : 0u;
+                block_append(ok[u][r], a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, bo,
+                             a.out);
+              }
+            }
           }
         }
-        bool any = false;
-#pragma unroll
-        for (int u = 0; u < 2; u++)
-#pragma unroll
-          for (int r = 0; r < AR; r++) any |= ok[u][r];
-        if (__ballot(any)) {
-#pragma unroll
-          for (int u = 0; u < 2; u++) {
-            const uint32_t b_row = perm_b[b_off + cb + min(jj + u, n_b - 1)];
-#pragma unroll
-            for (int r = 0; r < AR; r++)
-              block_append(ok[u][r], a_is_l ? a_row[r] : b_row, a_is_l ? b_row : a_row[r], bo,
-                           a.out);
-          }
-        }
+        run = run_end;
       }
     }
   }
@@ -970,15 +1078,29 @@ void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long 
   hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(256), 0, stream, p, n, v);
 }
 
-void launch_run_flags(const uint32_t *sorted_keys, uint32_t n, uint32_t *flags,
+void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,
+                        hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_layout_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, code, len, n,
+                     keys);
+}
+
+void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
+                       hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_gather_sig, dim3(grid_for(n, 256)), dim3(256), 0, stream, perm, n, sig,
+                     jsig);
+}
+
+void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
                       hipStream_t stream) {
   if (!n) return;
   hipLaunchKernelGGL(k_run_flags, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted_keys, n,
                      flags);
 }
 
-void launch_run_info(const uint32_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
-                     uint32_t n_l, const uint32_t *sorted_r, uint32_t n_r, GroupArrays g,
+void launch_run_info(const uint64_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
+                     uint32_t n_l, const uint64_t *sorted_r, uint32_t n_r, GroupArrays g,
                      hipStream_t stream) {
   hipLaunchKernelGGL(k_run_info, dim3(grid_for((uint64_t)n_l + 1, 256)), dim3(256), 0, stream,
                      sorted_l, flags, run_id, n_l, sorted_r, n_r, g);
@@ -1009,19 +1131,23 @@ void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t 
   a.c2 = coef[2];
   a.c3 = coef[3];
   a.out = out;
-  // The tile grid holds up to 12 work-groups per CU so that a whole job's
-  // work items are resident at once; surplus groups find no item and exit.
+  // The tile grid launches grid_per_cu work-groups per CU (16 by default,
+  // two rounds of the 8 that fit at once), each taking work items in a
+  // grid-stride loop; surplus groups find no item and exit.
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
   const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 4;
   const uint32_t tile_b = shape.tile_b;
 #define HB_FILTER(NEQ, SIM)                                                                   \
   do {                                                                                        \
-    if (shape.rows_per_lane == 2)                                                             \
-      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 2>), dim3(grid), dim3(256), 0, stream, a,  \
-                         jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                           \
+    if (shape.rows_per_lane == 4)                                                             \
+      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 4, 1>), dim3(grid), dim3(256), 0, stream,  \
+                         a, jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                        \
+    else if (shape.rows_per_lane == 2)                                                        \
+      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 2, 2>), dim3(grid), dim3(256), 0, stream,  \
+                         a, jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                        \
     else                                                                                      \
-      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 1>), dim3(grid), dim3(256), 0, stream, a,  \
-                         jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                           \
+      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 1, 2>), dim3(grid), dim3(256), 0, stream,  \
+                         a, jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                        \
     hipLaunchKernelGGL((k_filter_small<NEQ, SIM>), dim3(grid_small), dim3(256), 0, stream, a, \
                        jsig_l, jsig_r, jcode_l, jcode_r);                                     \
   } while (0)

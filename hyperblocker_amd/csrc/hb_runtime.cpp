@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -106,10 +107,98 @@ class DevicePool {
   std::unordered_map<size_t, std::vector<void *>> free_;
 };
 
+// Pinned host memory cache (hipHostMalloc, mapped into the device's address
+// space), reused across jobs: counters and job outputs live here, so results
+// need no copy after the job and no allocation per job.
+class HostPool {
+ public:
+  ~HostPool() { release_all(); }
+  hb_status alloc(size_t bytes, void **host, void **dev) {
+    bytes = round(bytes);
+    void *p = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto it = free_.find(bytes);
+      if (it != free_.end() && !it->second.empty()) {
+        p = it->second.back();
+        it->second.pop_back();
+      }
+    }
+    if (!p) {
+      hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocMapped);
+      if (e != hipSuccess) {
+        release_all();
+        e = hipHostMalloc(&p, bytes, hipHostMallocMapped);
+        if (e != hipSuccess)
+          return fail(HB_ERR_OOM, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+      }
+    }
+    *host = p;
+    if (dev) {
+      hipError_t e = hipHostGetDevicePointer(dev, p, 0);
+      if (e != hipSuccess) {
+        free(p, bytes);
+        return fail(HB_ERR_HIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
+      }
+    }
+    return HB_OK;
+  }
+  void free(void *p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    free_[round(bytes)].push_back(p);
+  }
+  void release_all() {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &kv : free_)
+      for (void *p : kv.second) (void)hipHostFree(p);
+    free_.clear();
+  }
+
+ private:
+  static size_t round(size_t b) {
+    size_t p = 256;
+    while (p < b) p <<= 1;
+    return p;
+  }
+  std::mutex mu_;
+  std::unordered_map<size_t, std::vector<void *>> free_;
+};
+
+// Timing events, reused across jobs.
+class EventPool {
+ public:
+  ~EventPool() {
+    for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+  }
+  hb_status get(hipEvent_t *e) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (!free_.empty()) {
+        *e = free_.back();
+        free_.pop_back();
+        return HB_OK;
+      }
+    }
+    HIP_OK(hipEventCreate(e));
+    return HB_OK;
+  }
+  void put(hipEvent_t e) {
+    if (!e) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.push_back(e);
+  }
+
+ private:
+  std::mutex mu_;
+  std::vector<hipEvent_t> free_;
+};
+
 // Byte-string -> code dictionary shared by every table of a context, so that
 // codes from any two columns compare equal exactly when the strings do.
 class Dictionary {
  public:
+  size_t size() const { return map_.size(); }
   uint32_t code(std::string_view s) {
     auto it = map_.find(s);
     if (it != map_.end()) return it->second;
@@ -146,11 +235,13 @@ struct Job;
 struct hb_ctx {
   int device = 0;
   int n_cu = 256;
-  hb::FilterShape shape{2, 512, 12};  // filter launch shape; HB_FILTER_SHAPE overrides
+  hb::FilterShape shape{2, 256, 16};  // filter launch shape; HB_FILTER_SHAPE overrides
   hipStream_t copy_stream = nullptr;
   std::vector<hipStream_t> streams;
   std::atomic<uint64_t> next_stream{0};
   DevicePool pool;
+  HostPool hpool;
+  EventPool events;
   Dictionary dict;
   std::mutex mu;  // jobs map
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
@@ -178,9 +269,16 @@ struct Column {
   Lazy code;
   uint32_t *d_code = nullptr;
   uint32_t *h_code = nullptr;  // pinned
-  Lazy index;                  // join index: codes sorted + row permutation
-  uint32_t *d_sorted = nullptr;
-  uint32_t *d_perm = nullptr;
+};
+
+// The rows of a table in join order for one rule shape: sorted by the code
+// of its first equality column, then by the length of its first similarity
+// column (either may be absent, -1).  Kept for the table's lifetime.
+struct Layout {
+  Lazy lz;
+  uint64_t *d_keys = nullptr;     // sorted (code << 8) | min(length, 255)
+  uint32_t *d_perm = nullptr;     // row id of each join-order position
+  hb::SigRow *d_jsig = nullptr;   // similarity signatures in join order
 };
 
 }  // namespace
@@ -190,6 +288,7 @@ struct hb_table {
   int64_t n_rows = 0;
   int32_t n_cols = 0;
   std::vector<Column> cols;
+  std::map<std::pair<int, int>, Layout> layouts;  // (equality column, similarity column)
   hipEvent_t ready = nullptr;
   std::mutex mu;  // guards lazy builds
   std::atomic<int> users{0};
@@ -243,12 +342,14 @@ struct Job {
   // capacities (pairs / entries)
   unsigned long long list_cap = 0, out_cap = 0, audit_cap = 0, set_cap = 0;
   int retries = 0;
+  hb::FilterShape shape{2, 256, 16};  // the context's filter shape at submit
   // device memory
   struct Alloc {
     void *p;
     size_t n;
   };
   std::vector<Alloc> allocs;
+  std::vector<Alloc> host_allocs;       // pinned, from the context's host pool
   unsigned long long *d_cnt = nullptr;
   unsigned long long *h_cnt = nullptr;  // pinned copy of the counters
   int n_counters = 0;
@@ -256,12 +357,10 @@ struct Job {
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> marks;  // per rule: join, filter, exact stage boundaries
   size_t n_marks = 0;
-  uint32_t *d_out = nullptr;
-  hb_audit_entry *d_audit = nullptr;
-  // results
-  bool fetched = false;
-  std::vector<uint32_t> pairs;
-  std::vector<hb_audit_entry> audit;
+  // outputs: pinned host memory written by the kernels through its device
+  // address (d_*), read by the caller at the host address (h_*)
+  uint32_t *d_out = nullptr, *h_out = nullptr;
+  hb_audit_entry *d_audit = nullptr, *h_audit = nullptr;
   hb_job_stats stats{};
 };
 
@@ -270,7 +369,7 @@ namespace {
 hb_status mark(Job *j, hipStream_t st) {
   if (j->n_marks == j->marks.size()) {
     hipEvent_t e;
-    HIP_OK(hipEventCreate(&e));
+    HB_TRY(j->ctx->events.get(&e));
     j->marks.push_back(e);
   }
   HIP_OK(hipEventRecord(j->marks[j->n_marks++], st));
@@ -283,9 +382,31 @@ hb_status job_alloc(Job *j, size_t bytes, void **out) {
   return HB_OK;
 }
 
+hb_status job_alloc_host(Job *j, size_t bytes, void **host, void **dev) {
+  HB_TRY(j->ctx->hpool.alloc(bytes, host, dev));
+  j->host_allocs.push_back({*host, bytes});
+  return HB_OK;
+}
+
+// Returns the job's device and pinned output memory to the context's pools
+// (the job's stream must be idle).
 void job_free_device(Job *j) {
   for (auto &a : j->allocs) j->ctx->pool.free(a.p, a.n);
   j->allocs.clear();
+  for (auto &a : j->host_allocs) j->ctx->hpool.free(a.p, a.n);
+  j->host_allocs.clear();
+}
+
+// Returns everything a finished job holds to the context's pools.
+void job_free_all(Job *j) {
+  job_free_device(j);
+  if (j->h_cnt) j->ctx->hpool.free(j->h_cnt, (size_t)j->n_counters * sizeof(unsigned long long));
+  j->h_cnt = nullptr;
+  j->ctx->events.put(j->ev_start);
+  j->ctx->events.put(j->ev_end);
+  for (hipEvent_t e : j->marks) j->ctx->events.put(e);
+  j->ev_start = j->ev_end = nullptr;
+  j->marks.clear();
 }
 
 hb_status set_device(hb_ctx *ctx) {
@@ -342,29 +463,52 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   return record_lazy(col.code, stream);
 }
 
-hb_status ensure_index(hb_table *t, int c, hipStream_t stream) {
-  HB_TRY(ensure_code(t, c, stream));
-  Column &col = t->cols[c];
+// Builds (once per table and rule shape) the join-order layout: keys from the
+// equality column's dictionary codes and the similarity column's lengths,
+// radix-sorted with the row ids, and the similarity signatures gathered into
+// that order.
+hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, const Layout **out) {
+  if (eq >= 0) HB_TRY(ensure_code(t, eq, stream));
+  if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
+  uint64_t n_codes = 1;
+  if (eq >= 0) {
+    Dictionary &d = t->ctx->dict;
+    std::lock_guard<std::mutex> dl(d.mu);
+    n_codes = std::max<uint64_t>(d.size(), 1);
+  }
   std::lock_guard<std::mutex> lk(t->mu);
-  if (col.index.built) return wait_lazy(col.index, stream);
+  Layout &ly = t->layouts[{eq, sim}];
+  *out = &ly;
+  if (ly.lz.built) return wait_lazy(ly.lz, stream);
   const uint32_t n = (uint32_t)t->n_rows;
-  const size_t nb = std::max<size_t>(n, 1) * sizeof(uint32_t);
-  HIP_OK(hipMalloc(&col.d_sorted, nb));
-  HIP_OK(hipMalloc(&col.d_perm, nb));
+  const size_t n1 = std::max<size_t>(n, 1);
+  HIP_OK(hipMalloc(&ly.d_keys, n1 * sizeof(uint64_t)));
+  HIP_OK(hipMalloc(&ly.d_perm, n1 * sizeof(uint32_t)));
+  if (sim >= 0) HIP_OK(hipMalloc(&ly.d_jsig, n1 * sizeof(hb::SigRow)));
   if (n) {
+    uint64_t *keys_in = nullptr;
     uint32_t *iota = nullptr;
-    HIP_OK(hipMalloc(&iota, nb));
+    HIP_OK(hipMalloc(&keys_in, n1 * sizeof(uint64_t)));
+    HIP_OK(hipMalloc(&iota, n1 * sizeof(uint32_t)));
+    hb::launch_layout_keys(eq >= 0 ? t->cols[eq].d_code : nullptr,
+                           sim >= 0 ? t->cols[sim].d_len : nullptr, n, keys_in, stream);
     hb::launch_iota(iota, n, stream);
+    int end_bit = 8;
+    while (end_bit < 64 && (1ull << (end_bit - 8)) < n_codes) end_bit++;
     size_t tb = 0;
-    HIP_OK(hb::sort_pairs_u32(nullptr, tb, col.d_code, col.d_sorted, iota, col.d_perm, n, stream));
+    HIP_OK(hb::sort_pairs_u64(nullptr, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit,
+                              stream));
     void *tmp = nullptr;
     HIP_OK(hipMalloc(&tmp, std::max<size_t>(tb, 16)));
-    HIP_OK(hb::sort_pairs_u32(tmp, tb, col.d_code, col.d_sorted, iota, col.d_perm, n, stream));
-    HIP_OK(hipStreamSynchronize(stream));  // one-off per column; frees below need it
+    HIP_OK(hb::sort_pairs_u64(tmp, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit, stream));
+    if (sim >= 0) hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(stream));  // one-off per layout; frees below need it
     HIP_OK(hipFree(tmp));
+    HIP_OK(hipFree(keys_in));
     HIP_OK(hipFree(iota));
   }
-  return record_lazy(col.index, stream);
+  return record_lazy(ly.lz, stream);
 }
 
 float filter_coef(double thr, int p) {
@@ -376,6 +520,11 @@ float filter_coef(double thr, int p) {
   return cf;
 }
 
+bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
+  return (rows_per_lane == 1 || rows_per_lane == 2 || rows_per_lane == 4) && tile_b >= 256 &&
+         tile_b <= 65536 && tile_b % 256 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
+}
+
 hb_status enqueue_job(Job *j) {
   hb_ctx *ctx = j->ctx;
   const hb_plan *plan = j->plan;
@@ -385,21 +534,25 @@ hb_status enqueue_job(Job *j) {
 
   HIP_OK(hipStreamWaitEvent(st, L->ready, 0));
   if (R != L) HIP_OK(hipStreamWaitEvent(st, R->ready, 0));
-  // Lazy per-column structures needed by the plan.
+  // Lazy per-column structures and join layouts needed by the plan.
+  std::vector<std::pair<const Layout *, const Layout *>> layouts;
   for (const PlanRule &rule : plan->rules) {
-    for (size_t k = 0; k < rule.eq.size(); k++) {
-      if (k == 0) {
-        HB_TRY(ensure_index(L, rule.eq[k].first, st));
-        HB_TRY(ensure_index(R, rule.eq[k].second, st));
-      } else {
-        HB_TRY(ensure_code(L, rule.eq[k].first, st));
-        HB_TRY(ensure_code(R, rule.eq[k].second, st));
-      }
+    for (size_t k = 1; k < rule.eq.size(); k++) {
+      HB_TRY(ensure_code(L, rule.eq[k].first, st));
+      HB_TRY(ensure_code(R, rule.eq[k].second, st));
     }
     for (const auto &s : rule.sims) {
       HB_TRY(ensure_sig(L, s.cl, st));
       HB_TRY(ensure_sig(R, s.cr, st));
     }
+    const int el = rule.eq.empty() ? -1 : rule.eq[0].first;
+    const int er = rule.eq.empty() ? -1 : rule.eq[0].second;
+    const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
+    const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
+    const Layout *ll = nullptr, *lr = nullptr;
+    HB_TRY(ensure_layout(L, el, sl, st, &ll));
+    HB_TRY(ensure_layout(R, er, sr, st, &lr));
+    layouts.push_back({ll, lr});
   }
 
   // Job memory.
@@ -414,9 +567,12 @@ hb_status enqueue_job(Job *j) {
   list[1] = (uint32_t *)p;
   HB_TRY(job_alloc(j, j->list_cap * 8, &p));
   uint32_t *d_defer = (uint32_t *)p;
-  HB_TRY(job_alloc(j, j->out_cap * 8, &p));
+  void *hp = nullptr;
+  HB_TRY(job_alloc_host(j, j->out_cap * 8, &hp, &p));
+  j->h_out = (uint32_t *)hp;
   j->d_out = (uint32_t *)p;
-  HB_TRY(job_alloc(j, j->audit_cap * sizeof(hb_audit_entry), &p));
+  HB_TRY(job_alloc_host(j, j->audit_cap * sizeof(hb_audit_entry), &hp, &p));
+  j->h_audit = (hb_audit_entry *)hp;
   j->d_audit = (hb_audit_entry *)p;
   unsigned long long *d_set = nullptr;
   const bool multi = plan->rules.size() > 1;
@@ -426,30 +582,21 @@ hb_status enqueue_job(Job *j) {
   }
   // Join scratch sized for the larger of the left tables' row counts.
   const uint32_t cap = std::max<uint32_t>(n_l, 1);
-  uint32_t *flags, *run_id, *iota_l, *iota_r;
+  uint32_t *flags, *run_id;
   HB_TRY(job_alloc(j, (size_t)cap * 4, &p));
   flags = (uint32_t *)p;
   HB_TRY(job_alloc(j, (size_t)cap * 4, &p));
   run_id = (uint32_t *)p;
-  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_l, 1) * 4, &p));
-  iota_l = (uint32_t *)p;
-  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * 4, &p));
-  iota_r = (uint32_t *)p;
-  // Signatures and extra equality codes in join order, reused by every rule.
-  hb::SigRow *jsig_l, *jsig_r;
+  // Extra equality codes in join order (the signatures come with the layout).
   uint32_t *jcode_l, *jcode_r;
-  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_l, 1) * sizeof(hb::SigRow), &p));
-  jsig_l = (hb::SigRow *)p;
-  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * sizeof(hb::SigRow), &p));
-  jsig_r = (hb::SigRow *)p;
   HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_l, 1) * 4 * hb::kMaxEq, &p));
   jcode_l = (uint32_t *)p;
   HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * 4 * hb::kMaxEq, &p));
   jcode_r = (uint32_t *)p;
   hb::GroupArrays g;
   g.cap = n_l;
-  g.tile_a = 256u * (uint32_t)ctx->shape.rows_per_lane;
-  g.tile_b = ctx->shape.tile_b;
+  g.tile_a = 256u * (uint32_t)j->shape.rows_per_lane;
+  g.tile_b = j->shape.tile_b;
   g.small_pairs_max = 16384;
   HB_TRY(job_alloc(j, 4 * sizeof(uint32_t), &p));
   g.n_runs = (uint32_t *)p;
@@ -484,40 +631,30 @@ hb_status enqueue_job(Job *j) {
   HIP_OK(hipEventRecord(j->ev_start, st));
   hb::launch_fill_u64(j->d_cnt, (unsigned long long)j->n_counters, 0ull, st);
   if (multi) hb::launch_fill_u64(d_set, j->set_cap, ~0ull, st);
-  bool iota_done = false;
 
   hb::AuditOut audit{j->d_audit, j->d_cnt + kCntAudit, j->audit_cap};
   hb::StageOut job_out{j->d_out, j->d_cnt + kCntOut, j->out_cap, j->d_cnt + kCntOverflow};
   int stage = kCntStage0;
 
-  for (const PlanRule &rule : plan->rules) {
-    const uint32_t *perm_l, *perm_r;
+  for (size_t ri = 0; ri < plan->rules.size(); ri++) {
+    const PlanRule &rule = plan->rules[ri];
+    const Layout &ll = *layouts[ri].first, &lr = *layouts[ri].second;
+    const uint32_t *perm_l = ll.d_perm, *perm_r = lr.d_perm;
     hb::EqCheck eq{};
     HB_TRY(mark(j, st));
     if (!rule.eq.empty()) {
-      const Column &cl = L->cols[rule.eq[0].first];
-      const Column &cr = R->cols[rule.eq[0].second];
-      perm_l = cl.d_perm;
-      perm_r = cr.d_perm;
       if (n_l) {
-        hb::launch_run_flags(cl.d_sorted, n_l, flags, st);
+        hb::launch_run_flags(ll.d_keys, n_l, flags, st);
         size_t t = tmp_bytes;
         HIP_OK(hb::exscan_u32(tmp, t, flags, run_id, n_l, st));
       }
-      hb::launch_run_info(cl.d_sorted, flags, run_id, n_l, cr.d_sorted, n_r, g, st);
+      hb::launch_run_info(ll.d_keys, flags, run_id, n_l, lr.d_keys, n_r, g, st);
       eq.n = (int32_t)rule.eq.size() - 1;
       for (int k = 1; k < (int)rule.eq.size(); k++) {
         eq.code_l[k - 1] = L->cols[rule.eq[k].first].d_code;
         eq.code_r[k - 1] = R->cols[rule.eq[k].second].d_code;
       }
     } else {
-      if (!iota_done) {
-        hb::launch_iota(iota_l, n_l, st);
-        hb::launch_iota(iota_r, n_r, st);
-        iota_done = true;
-      }
-      perm_l = iota_l;
-      perm_r = iota_r;
       hb::GroupArrays g1 = g;
       g1.cap = 1;
       hb::launch_single_group(n_l, n_r, g1, st);
@@ -535,23 +672,14 @@ hb_status enqueue_job(Job *j) {
     // between the two lists; the last stage of a single-rule plan writes the
     // job output directly.
     const int n_stages = 1 + (int)rule.sims.size();
-    const hb::SigRow *sig_l = nullptr, *sig_r = nullptr;
     float coef[4] = {0, 0, 0, 0};
-    if (!rule.sims.empty()) {
-      sig_l = L->cols[rule.sims[0].cl].d_sig;
-      sig_r = R->cols[rule.sims[0].cr].d_sig;
+    if (!rule.sims.empty())
       for (int q = 0; q < 4; q++) coef[q] = filter_coef((double)rule.sims[0].thr, q);
-    }
-    // Join order: gathered copies, or the rows themselves without a join.
-    const hb::SigRow *jl = sig_l, *jr = sig_r;
-    if (!rule.eq.empty()) {
-      hb::launch_gather_join(perm_l, n_l, sig_l, jsig_l, eq, 0, jcode_l, st);
-      hb::launch_gather_join(perm_r, n_r, sig_r, jsig_r, eq, 1, jcode_r, st);
+    const hb::SigRow *jl = ll.d_jsig, *jr = lr.d_jsig;
+    if (eq.n > 0) {
+      hb::launch_gather_join(perm_l, n_l, nullptr, nullptr, eq, 0, jcode_l, st);
+      hb::launch_gather_join(perm_r, n_r, nullptr, nullptr, eq, 1, jcode_r, st);
       HIP_OK(hipGetLastError());
-      if (sig_l) {
-        jl = jsig_l;
-        jr = jsig_r;
-      }
     }
     hb::StageOut prev{};
     HB_TRY(mark(j, st));
@@ -569,7 +697,7 @@ hb_status enqueue_job(Job *j) {
       }
       if (s == 0) {
         hb::launch_filter_groups(gu, perm_l, perm_r, jl, jr, jcode_l, jcode_r, eq.n,
-                                 rule.sims.empty() ? 0 : 1, coef, o, ctx->shape, ctx->n_cu, st);
+                                 rule.sims.empty() ? 0 : 1, coef, o, j->shape, ctx->n_cu, st);
       } else {
         const auto &sim = rule.sims[s - 1];
         const Column &cl = L->cols[sim.cl];
@@ -728,8 +856,7 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
   ctx->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   if (const char *env = std::getenv("HB_FILTER_SHAPE")) {
     int rpl = 0, tb = 0, gpc = 0;
-    if (std::sscanf(env, "%d,%d,%d", &rpl, &tb, &gpc) == 3 && (rpl == 1 || rpl == 2) &&
-        tb >= 256 && tb % 256 == 0 && gpc >= 1 && gpc <= 64)
+    if (std::sscanf(env, "%d,%d,%d", &rpl, &tb, &gpc) == 3 && valid_filter_shape(rpl, tb, gpc))
       ctx->shape = hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc};
   }
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
@@ -739,6 +866,28 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
     ctx->streams.push_back(s);
   }
   *out = ctx.release();
+  return HB_OK;
+}
+
+hb_status hb_ctx_set_filter_shape(hb_ctx *ctx, int32_t rows_per_lane, int32_t tile_b,
+                                  int32_t groups_per_cu) {
+  if (!ctx) return fail(HB_ERR_INVALID_ARG, "hb_ctx_set_filter_shape: null context");
+  if (!valid_filter_shape(rows_per_lane, tile_b, groups_per_cu))
+    return fail(HB_ERR_INVALID_ARG,
+                "filter shape (%d, %d, %d): rows per lane must be 1, 2 or 4, tile_b a multiple "
+                "of 256 up to 65536, groups per CU 1..64",
+                rows_per_lane, tile_b, groups_per_cu);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->shape = hb::FilterShape{rows_per_lane, (uint32_t)tile_b, (unsigned)groups_per_cu};
+  return HB_OK;
+}
+
+hb_status hb_ctx_get_filter_shape(const hb_ctx *ctx, int32_t *rows_per_lane, int32_t *tile_b,
+                                  int32_t *groups_per_cu) {
+  if (!ctx) return fail(HB_ERR_INVALID_ARG, "hb_ctx_get_filter_shape: null context");
+  if (rows_per_lane) *rows_per_lane = ctx->shape.rows_per_lane;
+  if (tile_b) *tile_b = (int32_t)ctx->shape.tile_b;
+  if (groups_per_cu) *groups_per_cu = (int32_t)ctx->shape.grid_per_cu;
   return HB_OK;
 }
 
@@ -755,17 +904,11 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   (void)hipDeviceSynchronize();
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    for (auto &kv : ctx->jobs) {
-      Job *j = kv.second.get();
-      job_free_device(j);
-      if (j->h_cnt) (void)hipHostFree(j->h_cnt);
-      if (j->ev_start) (void)hipEventDestroy(j->ev_start);
-      if (j->ev_end) (void)hipEventDestroy(j->ev_end);
-      for (hipEvent_t e : j->marks) (void)hipEventDestroy(e);
-    }
+    for (auto &kv : ctx->jobs) job_free_all(kv.second.get());
     ctx->jobs.clear();
   }
   ctx->pool.release_all();
+  ctx->hpool.release_all();
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   delete ctx;
@@ -873,10 +1016,14 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
     (void)hipFree(c.d_planes);
     (void)hipFree(c.d_code);
     if (c.h_code) (void)hipHostFree(c.h_code);
-    (void)hipFree(c.d_sorted);
-    (void)hipFree(c.d_perm);
-    for (Lazy *lz : {&c.sig, &c.code, &c.index})
+    for (Lazy *lz : {&c.sig, &c.code})
       if (lz->ev) (void)hipEventDestroy(lz->ev);
+  }
+  for (auto &kv : t->layouts) {
+    (void)hipFree(kv.second.d_keys);
+    (void)hipFree(kv.second.d_perm);
+    (void)hipFree(kv.second.d_jsig);
+    if (kv.second.lz.ev) (void)hipEventDestroy(kv.second.lz.ev);
   }
   if (t->ready) (void)hipEventDestroy(t->ready);
   delete t;
@@ -969,14 +1116,18 @@ hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, con
   j->set_cap = sc;
   j->n_counters = kCntStage0;
   for (const PlanRule &rule : plan->rules) j->n_counters += 1 + 2 * (int)rule.sims.size();
-  HIP_OK(hipHostMalloc((void **)&j->h_cnt, (size_t)j->n_counters * sizeof(unsigned long long),
-                       hipHostMallocDefault));
-  HIP_OK(hipEventCreate(&j->ev_start));
-  HIP_OK(hipEventCreate(&j->ev_end));
+  {
+    void *hc = nullptr;
+    HB_TRY(ctx->hpool.alloc((size_t)j->n_counters * sizeof(unsigned long long), &hc, nullptr));
+    j->h_cnt = (unsigned long long *)hc;
+  }
+  HB_TRY(ctx->events.get(&j->ev_start));
+  HB_TRY(ctx->events.get(&j->ev_end));
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (ctx->jobs.count(job_id))
       return fail(HB_ERR_INVALID_ARG, "job id %llu already in use", (unsigned long long)job_id);
+    j->shape = ctx->shape;
     ctx->jobs.emplace(job_id, std::move(job));
   }
   std::lock_guard<std::mutex> lk(j->mu);
@@ -1007,22 +1158,12 @@ hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs, u
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
   HB_TRY(job_poll(j, true));
   std::lock_guard<std::mutex> lk(j->mu);
-  if (!j->fetched) {
-    HB_TRY(set_device(ctx));
-    const unsigned long long n = j->h_cnt[kCntOut], na = j->h_cnt[kCntAudit];
-    j->pairs.resize(2 * n);
-    j->audit.resize(na);
-    if (n) HIP_OK(hipMemcpyAsync(j->pairs.data(), j->d_out, n * 8, hipMemcpyDeviceToHost,
-                                 j->stream));
-    if (na) HIP_OK(hipMemcpyAsync(j->audit.data(), j->d_audit, na * sizeof(hb_audit_entry),
-                                  hipMemcpyDeviceToHost, j->stream));
-    HIP_OK(hipStreamSynchronize(j->stream));
-    j->fetched = true;
-  }
-  if (pairs) *pairs = j->pairs.data();
-  if (n_pairs) *n_pairs = j->pairs.size() / 2;
-  if (audit) *audit = j->audit.data();
-  if (n_audit) *n_audit = j->audit.size();
+  // The kernels wrote the outputs into pinned memory; the job's completion
+  // event orders those writes before this read.
+  if (pairs) *pairs = j->h_out;
+  if (n_pairs) *n_pairs = j->h_cnt[kCntOut];
+  if (audit) *audit = j->h_audit;
+  if (n_audit) *n_audit = j->h_cnt[kCntAudit];
   return HB_OK;
 }
 
@@ -1048,11 +1189,7 @@ hb_status hb_job_release(hb_ctx *ctx, uint64_t job_id) {
   }
   HB_TRY(set_device(ctx));
   HIP_OK(hipStreamSynchronize(job->stream));
-  job_free_device(job.get());
-  if (job->h_cnt) (void)hipHostFree(job->h_cnt);
-  (void)hipEventDestroy(job->ev_start);
-  (void)hipEventDestroy(job->ev_end);
-  for (hipEvent_t e : job->marks) (void)hipEventDestroy(e);
+  job_free_all(job.get());
   return HB_OK;
 }
 

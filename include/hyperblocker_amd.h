@@ -124,6 +124,16 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out);
 hb_status hb_ctx_destroy(hb_ctx *ctx);
 /* Waits until all work of every stream on the context's device is done. */
 hb_status hb_ctx_synchronize(hb_ctx *ctx);
+/* Launch shape of the bound-filter kernel on this context: rows of the
+ * larger side per lane (1, 2 or 4), rows of the other side per work item
+ * (a multiple of 256, at most 65536) and work-groups per compute unit
+ * (1..64).  Applies to jobs submitted afterwards; results do not depend on
+ * it.  No counterpart in the reference (its launch is fixed at 64 x 64
+ * threads, host_producer.h:114-115); used by the autotuner. */
+hb_status hb_ctx_set_filter_shape(hb_ctx *ctx, int32_t rows_per_lane, int32_t tile_b,
+                                  int32_t groups_per_cu);
+hb_status hb_ctx_get_filter_shape(const hb_ctx *ctx, int32_t *rows_per_lane, int32_t *tile_b,
+                                  int32_t *groups_per_cu);
 /* Text of the last error on this thread (never NULL). */
 const char *hb_last_error(void);
 /* Library build id, e.g. "hyperblocker_amd 0.1 gfx950". */

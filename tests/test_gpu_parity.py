@@ -121,6 +121,36 @@ def test_adversarial_parity(ctx, name, seed, alphabet, max_len, long_frac):
     H.assert_same_audit(audit, want_audit, name)
 
 
+FILTER_SHAPES = [(1, 256, 4), (2, 256, 16), (2, 512, 12), (4, 256, 8), (4, 1024, 6)]
+
+
+@pytest.mark.parametrize("shape", FILTER_SHAPES)
+def test_filter_shapes_give_identical_results(ctx, shape):
+    """Every launch shape of the bound filter (rows per lane, B tile,
+    groups per CU) returns the oracle's pairs; join groups here are large
+    enough for the tiled kernel, with odd-length runs of equal B length."""
+    saved = ctx.filter_shape
+    ctx.filter_shape = shape
+    try:
+        assert ctx.filter_shape == shape
+        L, R = _adversarial_tables(21, 900, 700, b"abcdefgh", 40)
+        for name in ("eq_sim", "sim_only", "two_eq_two_sim", "multi_rule"):
+            plan = hb.HostPlan.from_yaml(PLANS[name])
+            got, audit, stats = H.run_gpu(ctx, plan, L, R)
+            want, want_audit, _ = H.run_oracle(plan, L, R)
+            H.assert_same_pairs(got, want, "%s %s" % (name, shape))
+            H.assert_same_audit(audit, want_audit, "%s %s" % (name, shape))
+    finally:
+        ctx.filter_shape = saved
+
+
+def test_filter_shape_validation(ctx):
+    for bad in [(3, 256, 8), (2, 100, 8), (2, 256, 0), (2, 256, 65), (1, 131072, 4)]:
+        with pytest.raises(capi.HBError) as e:
+            ctx.filter_shape = bad
+        assert e.value.status == capi.HB_ERR_INVALID_ARG
+
+
 def test_audit_band_entries(ctx):
     """Thresholds set to JW values of actual pairs put those pairs in the
     audit band; the device lists exactly the oracle's entries."""
