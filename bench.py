@@ -107,10 +107,13 @@ def main():
     L, R, dup, cfg = datagen.make(args.workload)
     plan = hb.HostPlan.from_yaml(cfg["rules"])
     ctx = hb.Context(0)
+    ctx.synchronize()
+    t = time.perf_counter()
     tl = ctx.upload(hb.ColumnTable(L))
     tr = ctx.upload(hb.ColumnTable(R))
-    dp = ctx.plan(plan)
     ctx.synchronize()
+    upload_s = time.perf_counter() - t
+    dp = ctx.plan(plan)
 
     def finish(job):
         pairs, audit, stats = job.results()
@@ -175,6 +178,8 @@ def main():
         "stage_ms": {k: float(np.mean([s[k] for s in stats]))
                      for k in ("device_ms", "join_ms", "filter_ms", "exact_ms")},
         "cold_first_job_ms": cold_s * 1e3,
+        "upload_ms": upload_s * 1e3,
+        "pcie_inclusive_pairs_per_s": n_pairs / (upload_s + ms_per_step * 1e-3),
         "inflight": args.inflight, "job_alone_ms": float(np.median(lat)),
         "job_alone_device_ms": st_alone["device_ms"],
         "filtered": stats[-1]["filtered"], "jw_exact": stats[-1]["jw_exact"],
