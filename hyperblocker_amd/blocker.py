@@ -225,6 +225,18 @@ class Context:
         check(lib().hb_job_submit(self._h, plan._h, l_table._h, r_table._h, jid))
         return Job(self, jid)
 
+    def submit_batch(self, jobs, job_ids=None):
+        """jobs: list of (plan, l_table, r_table).  One batch: a single launch
+        sequence covers every job.  Returns the Job handles."""
+        n = len(jobs)
+        ids = [next(_job_ids) for _ in range(n)] if job_ids is None else list(job_ids)
+        plans = (C.c_void_p * n)(*[p._h.value for p, _, _ in jobs])
+        ls = (C.c_void_p * n)(*[l._h.value for _, l, _ in jobs])
+        rs = (C.c_void_p * n)(*[r._h.value for _, _, r in jobs])
+        idarr = (C.c_uint64 * n)(*ids)
+        check(lib().hb_batch_submit(self._h, n, plans, ls, rs, idarr))
+        return [Job(self, i) for i in ids]
+
     def block(self, plan, l_table, r_table):
         """Synchronous blocking of l_table x r_table: (pairs uint32[n,2] in no
         particular order, audit structured array, stats dict)."""

@@ -31,7 +31,7 @@ EXPORTS = [
     "hb_table_upload", "hb_table_upload_serialized", "hb_table_upload_rows",
     "hb_table_info", "hb_table_release",
     "hb_plan_create", "hb_plan_release",
-    "hb_job_submit", "hb_job_query", "hb_job_wait", "hb_job_results",
+    "hb_job_submit", "hb_batch_submit", "hb_job_query", "hb_job_wait", "hb_job_results",
     "hb_job_stats_get", "hb_job_release", "hb_block", "hb_free",
     "hb_rules_load", "hb_rules_parse", "hb_host_plan_view", "hb_host_plan_free",
     "hb_csv_load", "hb_host_table_view", "hb_host_table_cell", "hb_host_table_free",
@@ -113,6 +113,8 @@ def lib():
         "hb_plan_create": ([VP, C.POINTER(SerializedPlan), PVP], st),
         "hb_plan_release": ([VP, VP], st),
         "hb_job_submit": ([VP, VP, VP, VP, C.c_uint64], st),
+        "hb_batch_submit": ([VP, C.c_uint32, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
+                             C.POINTER(C.c_uint64)], st),
         "hb_job_query": ([VP, C.c_uint64], st),
         "hb_job_wait": ([VP, C.c_uint64], st),
         "hb_job_results": ([VP, C.c_uint64, C.POINTER(C.POINTER(C.c_uint32)),

@@ -9,6 +9,11 @@
 //              by a similarity predicate
 //   code[c]  : uint32 dictionary code per row, built on first use by an
 //              equality predicate; equal codes <=> equal byte strings
+//
+// Work is submitted in batches of jobs (a plain job is a batch of one).  A
+// job-rule (JR) is one rule of one job's plan; every kernel of a batch
+// covers all of its JRs, so a batch costs the same number of launches
+// whatever its number of jobs.
 #pragma once
 #include <stdint.h>
 
@@ -17,6 +22,8 @@
 #else
 #include <hip/hip_runtime_api.h>
 #endif
+
+#include "../../include/hyperblocker_amd.h"
 
 namespace hb {
 
@@ -53,19 +60,79 @@ struct DevColumn {
   uint32_t nwc;
 };
 
-// Join groups.  Run g of equal key in the sorted left keys: left positions
-// [l_off, l_off + l_cnt) of perm_l, right positions [r_off, r_off + r_cnt)
-// of perm_r.  Arrays hold n_l + 1 entries; entries at and beyond *n_runs are
-// zero so the exclusive scans give totals at index n_l.
+// One side (0 = left, 1 = right) of a JR, in its join order: position i is
+// row perm[i].  Runs of equal first-equality code: run g covers positions
+// [run_off[g], run_off[g + 1]) and has code run_code[g] (codes ascending; a
+// rule without equalities has a single run of code 0).
+struct SideDesc {
+  const uint32_t *perm;
+  const SigRow *jsig;      // signatures of the rule's first similarity column, or null
+  const uint32_t *jcode;   // kMaxEq extra equality codes per position, or null
+  const uint32_t *run_code;
+  const uint32_t *run_off;  // n_runs + 1 entries
+  uint32_t n_runs;
+  uint32_t pad;
+};
+
+struct JRDesc {
+  SideDesc s[2];
+  float c0, c1, c2, c3;  // bound-filter coefficients of the first similarity
+  uint32_t job;
+  int32_t n_eq;          // extra equalities (beyond the join key)
+  int32_t n_sims;
+  uint32_t sim_off;      // first SimDesc of this JR
+  uint32_t multi;        // the job has several rules: its final pairs are deduplicated
+  uint32_t pad[3];
+};
+
+struct SimDesc {
+  DevColumn col_l, col_r;
+  float thr;
+  int32_t slot;
+};
+
+// Per-job counters, structure of arrays: counter k of job j lives at
+// job_cnt[k * cnt_stride + j], cnt_stride = jobs + 1 (the extra entry stays
+// zero, so an exclusive scan of a row gives its total at index jobs).
+enum : int {
+  kJobJoined = 0,    // candidate pairs (sharing the join key), over rules
+  kJobFiltered = 1,  // pairs the bound filter kept
+  kJobJw = 2,        // exact JW evaluations
+  kJobMatches = 3,   // distinct output pairs
+  kJobAudit = 4,     // audit entries
+  kJobCounters = 5,
+};
+
+// Per-job hash sets deduplicating the output of jobs with several rules:
+// job j owns slots [off[j], off[j] + cap[j]) (cap a power of two, 0 for jobs
+// with one rule); empty slots hold ~0.
+struct JobSets {
+  unsigned long long *slots;
+  const unsigned long long *off;
+  const unsigned long long *cap;
+};
+
+struct BatchDev {
+  const JRDesc *jrs;
+  const uint32_t *jr_group_off;  // n_jr + 1: first group of each JR
+  uint32_t n_jr;
+  uint32_t cnt_stride;
+  const SimDesc *sims;
+  unsigned long long *job_cnt;   // kJobCounters * cnt_stride
+  JobSets sets;
+  unsigned long long *overflow;  // set to 1 when any list or set is full
+};
+
+// Join groups of a batch: group gi pairs left positions [l_off, l_off + l_cnt)
+// with right positions [r_off, r_off + r_cnt) of its JR.  The arrays hold
+// n + 1 entries; entry n is zero so that exclusive scans give totals there.
 struct GroupArrays {
-  uint32_t *n_runs;        // [1]
+  uint32_t n;
   uint32_t *l_off, *l_cnt, *r_off, *r_cnt;
   uint64_t *small_pairs;   // pairs of groups with <= small_pairs_max pairs, else 0
   uint64_t *pair_off;      // exclusive scan of small_pairs
   uint64_t *items;         // work items of the other groups, else 0
   uint64_t *item_off;      // exclusive scan of items
-  unsigned long long *joined;  // += pairs sharing the join key
-  uint32_t cap;            // n_l: array entries - 1
   uint32_t tile_a;         // rows of the larger side per work item (256 * rows per lane)
   uint32_t tile_b;         // rows of the smaller side per work item
   uint32_t small_pairs_max;  // groups with at most this many pairs take the list path
@@ -78,21 +145,24 @@ struct FilterShape {
   unsigned grid_per_cu;
 };
 
-struct EqCheck {  // equality predicates checked per pair (beyond the join key)
-  int32_t n;
-  const uint32_t *code_l[kMaxEq];
-  const uint32_t *code_r[kMaxEq];
+// Pair list entry: rows l, r of JR jr (in the final list: of job jr).
+struct PairEntry {
+  uint32_t l, r, jr, pad;
 };
 
-struct StageOut {  // pair list output with capacity
-  uint32_t *pairs;                  // 2 * cap uint32
+struct PairList {
+  PairEntry *e;
   unsigned long long *count;
   unsigned long long cap;
-  unsigned long long *overflow;     // set to 1 when count exceeded cap
 };
 
-struct AuditOut {
-  void *entries;  // hb_audit_entry[cap]
+struct AuditRec {
+  hb_audit_entry e;
+  uint32_t job, pad;
+};
+
+struct AuditList {
+  AuditRec *e;
   unsigned long long *count;
   unsigned long long cap;
 };
@@ -103,59 +173,62 @@ void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
                          uint32_t *planes, hipStream_t stream);
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
-// Join order of a table for one rule: keys (code << 8) | min(length, 255)
-// (code or length may be absent: null pointer), sorted with the row ids.
+void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long long v,
+                     hipStream_t stream);
+
+// Join order of a table for one rule shape: keys (code << 8) | min(length,
+// 255) (code or length may be absent: null pointer), sorted with the row ids;
+// then the runs of equal code (key >> 8): flags, exclusive scan, run arrays.
 void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,
                         hipStream_t stream);
 void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
                        hipStream_t stream);
-// Runs of equal code (key >> 8) in the sorted left keys, matched with the
-// sorted right keys.
 void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
                       hipStream_t stream);
-void launch_run_info(const uint64_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
-                     uint32_t n_l, const uint64_t *sorted_r, uint32_t n_r, GroupArrays g,
-                     hipStream_t stream);
-void launch_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g, hipStream_t stream);
+// run_code / run_off of every run start (flags, run_id = exclusive scan of
+// flags) and run_off[n_runs] = n, with n_runs = run_id[n-1] + flags[n-1].
+void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
+                        const uint32_t *run_id, uint32_t n, uint32_t *run_code,
+                        uint32_t *run_off, uint32_t *n_runs, hipStream_t stream);
+// kMaxEq codes per join-order position from up to kMaxEq code columns (three
+// columns are stored as four, the third repeated).
+struct CodeCols {
+  int32_t n;
+  const uint32_t *col[kMaxEq];
+};
+void launch_gather_codes(const uint32_t *perm, uint32_t n, CodeCols cols, uint32_t *jcode,
+                         hipStream_t stream);
 
-// Signatures and extra equality codes copied into join order (element i of
-// the outputs belongs to row perm[i]; codes kMaxEq per row).  side 0 = left.
-void launch_gather_join(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
-                        EqCheck eq, int side, uint32_t *jcode, hipStream_t stream);
+// Groups of every JR of a batch (one per left run), with their work.
+void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream);
 
-// First stage over join groups: remaining equalities + bound filter of the
-// first similarity predicate (has_sim = 0: every equal pair is emitted).
-// Inputs are in join order (launch_gather_join).
-void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t *perm_r,
-                          const SigRow *jsig_l, const SigRow *jsig_r, const uint32_t *jcode_l,
-                          const uint32_t *jcode_r, int n_eq, int has_sim, const float coef[4],
-                          StageOut out, FilterShape shape, int n_cu, hipStream_t stream);
+// Stage 0 over groups [g_begin, g_end) (JRs of one variant): remaining
+// equalities + bound filter of the first similarity predicate; kept pairs
+// are appended to `out` tagged with their JR.
+void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, int n_eq,
+                   bool has_sim, PairList out, FilterShape shape, int n_cu, hipStream_t stream);
 
-// Exact Jaro-Winkler on a pair list (count read from the device): one pair
-// per lane, then one wave per pair for the pairs moved to `defer` (shorter
-// string longer than 256 bytes).
-void launch_exact(const uint32_t *pairs, const unsigned long long *n_pairs,
-                  unsigned long long in_cap, DevColumn col_l, DevColumn col_r, float threshold,
-                  int32_t slot, StageOut out, AuditOut audit, unsigned long long *jw_count,
-                  StageOut defer, int n_cu, hipStream_t stream);
+// Exact stage `stage` (1-based similarity index) on list `in`: JRs without
+// similarity predicates pass their pairs to the output directly; pairs
+// passing their last predicate go to `final_` (deduplicated per job where
+// needed), the others to `next`.  Pairs with a shorter string above 256
+// bytes run one wave per pair (second kernel, via `defer`).
+void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
+                  PairList defer, AuditList audit, int n_cu, hipStream_t stream);
 
-// Append a pair list to the job output, dropping pairs already present
-// (device hash set keyed by (l, r); empty slots hold ~0).
-void launch_merge_unique(const uint32_t *pairs, const unsigned long long *n_pairs,
-                         unsigned long long in_cap, unsigned long long *set,
-                         unsigned long long set_cap, StageOut out, int n_cu, hipStream_t stream);
-void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long long v,
-                     hipStream_t stream);
+// Copies the final list and the audit list into per-job contiguous ranges:
+// job j's pairs start at out_off[j] (exclusive scan of its match count).
+void launch_scatter(BatchDev b, PairList final_, const uint64_t *out_off,
+                    unsigned long long *fill, uint32_t *out_pairs, unsigned long long out_cap,
+                    AuditList audit, const uint64_t *audit_off, unsigned long long *audit_fill,
+                    hb_audit_entry *out_audit, unsigned long long audit_cap, int n_cu,
+                    hipStream_t stream);
 
 // Jaro-Winkler from counts, one thread per (m, S, T, p) tuple (tests).
 void launch_jw_from_counts(const int32_t *mstp, int64_t n, double *out, double *out_fused,
                            hipStream_t stream);
 
 // hb_sort.hip: rocPRIM wrappers.  temp may be null to query temp_bytes.
-hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in,
-                          uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
-                          uint32_t n, hipStream_t stream);
-// Stable radix sort of (key, value) pairs on key bits [0, end_bit).
 hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const uint64_t *keys_in,
                           uint64_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint32_t n, int end_bit, hipStream_t stream);

@@ -4,23 +4,26 @@
 // core/gpu/device_func.cuh:39-52 (equalities_kernel), :54-132
 // (lev_jaro_ratio), :210-231 (lev_jaro_winkler_ratio).
 //
-// Structure of one rule's evaluation on the device:
-//   1. join: the first equality's dictionary codes of both sides are sorted
-//      (hb_sort.hip); each run of one code on the left meets the run of the
-//      same code on the right (k_run_info).  A rule without equalities is one
-//      group covering L x R.
+// A batch of jobs runs as one sequence of launches over all of its job-rules
+// (JRs, one rule of one job):
+//   1. join (k_batch_join): every left run of equal first-equality code in a
+//      JR's cached layout meets the right run of the same code (binary
+//      search); a rule without equalities is one group covering L x R.
 //   2. k_filter_tiles / k_filter_small: pairs of every group are checked for
 //      the remaining equalities (integer code compares) and an upper bound of
 //      the first similarity predicate (byte-class histograms, v_sad_u8);
-//      survivors are compacted with wave ballots into a pair list.  Large
-//      groups run as tiles (256 rows in lanes x up to 1024 rows read as
-//      wave-uniform scalar loads, four per step), small groups as explicit
-//      pair lists.  Signatures are first gathered into join order.
+//      survivors are compacted with wave ballots into a shared pair list
+//      tagged with their JR.  Large groups run as tiles (512 rows in lanes x
+//      256 rows staged in LDS and read as broadcasts), small groups as
+//      explicit pair lists.
 //   3. k_exact_lanes: each surviving pair (one per lane) gets the exact greedy
 //      Jaro match count from bit-planes of its shorter string, then the
 //      reference's fp32/fp64 tail.  Shorter strings above 256 bytes go to
-//      k_exact (one wave per pair).  Later similarity predicates of the rule
-//      run as further exact stages on the shrinking list.
+//      k_exact (one wave per pair).  Later similarity predicates run as
+//      further stages on the shrinking list; pairs passing a JR's last
+//      predicate enter the final list once per job (hash set for jobs with
+//      several rules).
+//   4. k_scatter: the final list is copied into per-job ranges.
 //
 // Floating point: contraction is off for this file, fp32 '/' is correctly
 // rounded (hipcc default), so every operation of the Jaro/Winkler tail
@@ -50,8 +53,11 @@ __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, i
   return ((unsigned long long)hi << 32) | lo;
 }
 
-// Appends (l, r) of every lane with ok set; one atomic per wave.
-__device__ __forceinline__ void wave_append(bool ok, uint32_t l, uint32_t r, const StageOut &out) {
+// Appends the entry of every lane with ok set to a pair list; one atomic
+// per wave.  A full list raises the batch's overflow flag (the batch is then
+// run again with larger lists).
+__device__ __forceinline__ void list_append(bool ok, const PairEntry &e, const PairList &out,
+                                            unsigned long long *overflow) {
   const uint64_t mask = __ballot(ok);
   if (mask == 0) return;
   const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -60,28 +66,60 @@ __device__ __forceinline__ void wave_append(bool ok, uint32_t l, uint32_t r, con
   base = readlane64(base, leader);
   if (ok) {
     const unsigned long long pos = base + rank_in(mask);
-    if (pos < out.cap) {
-      out.pairs[2 * pos] = l;
-      out.pairs[2 * pos + 1] = r;
-    } else {
-      *out.overflow = 1ull;
-    }
+    if (pos < out.cap)
+      out.e[pos] = e;
+    else
+      *overflow = 1ull;
   }
+}
+
+// Adds one for every lane with ok set to counter k of the lane's job: one
+// atomic per distinct job in the wave (usually one).
+__device__ __forceinline__ void job_count(const BatchDev &b, int k, bool ok, uint32_t job) {
+  uint64_t mask = __ballot(ok);
+  while (mask) {
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t lj = (uint32_t)__builtin_amdgcn_readlane((int)job, leader);
+    const uint64_t same = __ballot(ok && job == lj);
+    if ((int)lane_id() == leader)
+      atomicAdd(b.job_cnt + (size_t)k * b.cnt_stride + lj, (unsigned long long)__popcll(same));
+    mask &= ~same;
+  }
+}
+
+// A slot of the lane's job among fill[job]: consecutive per job, one atomic
+// per distinct job in the wave.
+__device__ __forceinline__ unsigned long long job_slot(bool ok, uint32_t job,
+                                                       unsigned long long *fill) {
+  unsigned long long mine = 0;
+  uint64_t mask = __ballot(ok);
+  while (mask) {
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t lj = (uint32_t)__builtin_amdgcn_readlane((int)job, leader);
+    const uint64_t same = __ballot(ok && job == lj);
+    unsigned long long base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(fill + lj, (unsigned long long)__popcll(same));
+    base = readlane64(base, leader);
+    if (ok && job == lj) mine = base + rank_in(same);
+    mask &= ~same;
+  }
+  return mine;
 }
 
 // Survivor buffer of one work-group in LDS.  Waves append with one LDS
 // atomic each; the group moves the buffer to the global list with a single
 // global atomic (block_flush), so a busy list counter is not contended by
 // every wave.  Appends past the buffer go to the global list directly.
-constexpr uint32_t kBlockOut = 1024;
+constexpr uint32_t kBlockOut = 512;
 struct BlockOut {
-  uint32_t pairs[2 * kBlockOut];
+  PairEntry e[kBlockOut];
   uint32_t n;
   unsigned long long base;
 };
 
-__device__ __forceinline__ void block_append(bool ok, uint32_t l, uint32_t r, BlockOut &bo,
-                                             const StageOut &out) {
+__device__ __forceinline__ void block_append(bool ok, uint32_t l, uint32_t r, uint32_t jr,
+                                             BlockOut &bo, const PairList &out,
+                                             unsigned long long *overflow) {
   const uint64_t mask = __ballot(ok);
   if (mask == 0) return;
   const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -90,23 +128,22 @@ __device__ __forceinline__ void block_append(bool ok, uint32_t l, uint32_t r, Bl
   base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
   if (ok) {
     const uint32_t pos = base + rank_in(mask);
+    const PairEntry e{l, r, jr, 0u};
     if (pos < kBlockOut) {
-      bo.pairs[2 * pos] = l;
-      bo.pairs[2 * pos + 1] = r;
+      bo.e[pos] = e;
     } else {
       const unsigned long long g = atomicAdd(out.count, 1ull);
-      if (g < out.cap) {
-        out.pairs[2 * g] = l;
-        out.pairs[2 * g + 1] = r;
-      } else {
-        *out.overflow = 1ull;
-      }
+      if (g < out.cap)
+        out.e[g] = e;
+      else
+        *overflow = 1ull;
     }
   }
 }
 
 // Called by every thread of the group.  Leaves bo.n == 0.
-__device__ __forceinline__ void block_flush(BlockOut &bo, const StageOut &out) {
+__device__ __forceinline__ void block_flush(BlockOut &bo, const PairList &out,
+                                            unsigned long long *overflow) {
   __syncthreads();
   const uint32_t n = min(bo.n, kBlockOut);
   if (threadIdx.x == 0) bo.base = n ? atomicAdd(out.count, (unsigned long long)n) : 0ull;
@@ -114,12 +151,10 @@ __device__ __forceinline__ void block_flush(BlockOut &bo, const StageOut &out) {
   const unsigned long long base = bo.base;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     const unsigned long long pos = base + i;
-    if (pos < out.cap) {
-      out.pairs[2 * pos] = bo.pairs[2 * i];
-      out.pairs[2 * pos + 1] = bo.pairs[2 * i + 1];
-    } else {
-      *out.overflow = 1ull;
-    }
+    if (pos < out.cap)
+      out.e[pos] = bo.e[i];
+    else
+      *overflow = 1ull;
   }
   __syncthreads();
   if (threadIdx.x == 0) bo.n = 0;
@@ -293,7 +328,7 @@ __device__ __forceinline__ uint32_t lower_bound_code(const uint64_t *a, uint32_t
   return lo;
 }
 
-__device__ __forceinline__ void set_group_work(GroupArrays g, uint32_t gi, uint32_t l_cnt,
+__device__ __forceinline__ void set_group_work(const GroupArrays &g, uint32_t gi, uint32_t l_cnt,
                                                uint32_t r_cnt) {
   const uint64_t pairs = (uint64_t)l_cnt * r_cnt;
   uint64_t small = 0, items = 0;
@@ -307,47 +342,69 @@ __device__ __forceinline__ void set_group_work(GroupArrays g, uint32_t gi, uint3
   g.items[gi] = items;
 }
 
-__global__ void k_run_info(const uint64_t *__restrict__ sl, const uint32_t *__restrict__ flags,
-                           const uint32_t *__restrict__ run_id, uint32_t n_l,
-                           const uint64_t *__restrict__ sr, uint32_t n_r, GroupArrays g) {
+// Runs of equal code in a sorted layout: run g starts at run_off[g] with
+// code run_code[g]; run_off[n_runs] = n.
+__global__ void k_layout_runs(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ flags,
+                              const uint32_t *__restrict__ run_id, uint32_t n,
+                              uint32_t *__restrict__ run_code, uint32_t *__restrict__ run_off,
+                              uint32_t *__restrict__ n_runs) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n_l) return;
-  const uint32_t n_runs = n_l ? run_id[n_l - 1] + flags[n_l - 1] : 0u;
-  if (i == 0) *g.n_runs = n_runs;
-  if (i >= n_runs) {
-    g.l_off[i] = g.l_cnt[i] = g.r_off[i] = g.r_cnt[i] = 0;
-    g.small_pairs[i] = 0;
-    g.items[i] = 0;
+  if (i >= n) return;
+  if (flags[i]) {
+    run_code[run_id[i]] = (uint32_t)(keys[i] >> 8);
+    run_off[run_id[i]] = i;
   }
-  if (i < n_l && flags[i]) {
-    const uint32_t gi = run_id[i];
-    const uint64_t code = sl[i] >> 8;
-    const uint32_t l_end = upper_bound_code(sl, i, n_l, code);
-    const uint32_t r_lo = lower_bound_code(sr, 0, n_r, code);
-    const uint32_t r_hi = upper_bound_code(sr, r_lo, n_r, code);
-    g.l_off[gi] = i;
-    g.l_cnt[gi] = l_end - i;
-    g.r_off[gi] = r_lo;
-    g.r_cnt[gi] = r_hi - r_lo;
-    set_group_work(g, gi, l_end - i, r_hi - r_lo);
-    const unsigned long long jp = (unsigned long long)(l_end - i) * (r_hi - r_lo);
-    if (jp) atomicAdd(g.joined, jp);
+  if (i == n - 1) {
+    const uint32_t nr = run_id[i] + flags[i];
+    run_off[nr] = n;
+    *n_runs = nr;
   }
 }
 
-__global__ void k_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    *g.n_runs = 1;
-    g.l_off[0] = 0;
-    g.l_cnt[0] = n_l;
-    g.r_off[0] = 0;
-    g.r_cnt[0] = n_r;
-    set_group_work(g, 0, n_l, n_r);
-    atomicAdd(g.joined, (unsigned long long)n_l * n_r);
-    g.l_off[1] = g.l_cnt[1] = g.r_off[1] = g.r_cnt[1] = 0;
-    g.small_pairs[1] = 0;
-    g.items[1] = 0;
+// Largest j in [0, n) with off[j] <= x (off non-decreasing, off[0] <= x).
+__device__ __forceinline__ uint32_t find_le_u32(const uint32_t *off, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (off[mid] <= x) lo = mid; else hi = mid;
   }
+  return lo;
+}
+
+// One thread per group of the batch: left run `local` of JR jr meets the
+// right run of the same code (binary search over the right runs).
+__global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi > g.n) return;
+  if (gi == g.n) {
+    g.l_off[gi] = g.l_cnt[gi] = g.r_off[gi] = g.r_cnt[gi] = 0;
+    g.small_pairs[gi] = 0;
+    g.items[gi] = 0;
+    return;
+  }
+  const uint32_t jr = find_le_u32(b.jr_group_off, b.n_jr, gi);
+  const JRDesc &d = b.jrs[jr];
+  const uint32_t local = gi - b.jr_group_off[jr];
+  const SideDesc &L = d.s[0], &R = d.s[1];
+  const uint32_t code = L.run_code[local];
+  const uint32_t l_off = L.run_off[local], l_cnt = L.run_off[local + 1] - l_off;
+  uint32_t lo = 0, hi = R.n_runs;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (R.run_code[mid] < code) lo = mid + 1; else hi = mid;
+  }
+  uint32_t r_off = 0, r_cnt = 0;
+  if (lo < R.n_runs && R.run_code[lo] == code) {
+    r_off = R.run_off[lo];
+    r_cnt = R.run_off[lo + 1] - r_off;
+  }
+  g.l_off[gi] = l_off;
+  g.l_cnt[gi] = l_cnt;
+  g.r_off[gi] = r_off;
+  g.r_cnt[gi] = r_cnt;
+  set_group_work(g, gi, l_cnt, r_cnt);
+  const unsigned long long jp = (unsigned long long)l_cnt * r_cnt;
+  if (jp) atomicAdd(b.job_cnt + (size_t)kJobJoined * b.cnt_stride + d.job, jp);
 }
 
 // Largest g in [0, n) with off[g] <= x (off non-decreasing, off[0] = 0).
@@ -362,17 +419,9 @@ __device__ __forceinline__ uint32_t find_group(const uint64_t *off, uint32_t n, 
 
 // ---- stage 1: equalities + bound filter over join groups ------------------
 
-// Signatures and extra equality codes arrive in join order: element i of
-// jsig_l / jcode_l belongs to left row perm_l[i] (likewise on the right), so
-// the rows of a join group are contiguous.  jcode_* hold kMaxEq codes per
-// row, row-major.
-struct FilterArgs {
-  GroupArrays g;
-  const uint32_t *perm_l, *perm_r;
-  int32_t n_eq;
-  float c0, c1, c2, c3;
-  StageOut out;
-};
+// Signatures and extra equality codes arrive in join order (the JR's
+// layouts): element i of jsig / jcode belongs to row perm[i], so the rows of
+// a join group are contiguous.  jcode holds kMaxEq codes per row.
 
 // The bound test of may_pass in allowance form, for a lane-held A row and a
 // B row shared by the wave.  may_pass rejects when
@@ -447,13 +496,9 @@ struct __attribute__((aligned(8))) BRow {
 };
 
 template <int NEQ, bool SIM, int AR, int BU>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6 : 8))) void k_filter_tiles(
-    const FilterArgs a,
-                                                      const SigRow *__restrict__ jsig_l,
-                                                      const SigRow *__restrict__ jsig_r,
-                                                      const uint32_t *__restrict__ jcode_l,
-                                                      const uint32_t *__restrict__ jcode_r,
-                                                      uint32_t tile_b) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6 : 8))) void
+k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_t g_end,
+               PairList out) {
   constexpr uint32_t kChunk = 256;
   __shared__ BRow s_b[SIM ? kChunk + 1 : 1];
   __shared__ uint32_t s_code[NEQ > 0 ? NEQ : 1][kChunk + 1];
@@ -462,24 +507,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6
   __shared__ BlockOut bo;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) bo.n = 0;
-  const float k0 = 2.0f * a.c0, k3 = 2.0f * a.c3;
-  const uint32_t n_runs = *a.g.n_runs;
-  const uint64_t total = a.g.item_off[a.g.cap];
-  for (uint64_t item = blockIdx.x; item < total; item += gridDim.x) {
-    const uint32_t gi = find_group(a.g.item_off, n_runs, item);
-    const uint32_t l_off = a.g.l_off[gi], l_cnt = a.g.l_cnt[gi];
-    const uint32_t r_off = a.g.r_off[gi], r_cnt = a.g.r_cnt[gi];
+  const uint32_t tile_b = g.tile_b;
+  const uint64_t item_begin = g.item_off[g_begin], item_end = g.item_off[g_end];
+  for (uint64_t item = item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
+    const uint32_t gi = g_begin + find_group(g.item_off + g_begin, g_end - g_begin, item);
+    const uint32_t jr = find_le_u32(bd.jr_group_off, bd.n_jr, gi);
+    const JRDesc &d = bd.jrs[jr];
+    const float k0 = 2.0f * d.c0, k3 = 2.0f * d.c3;
+    const uint32_t l_off = g.l_off[gi], l_cnt = g.l_cnt[gi];
+    const uint32_t r_off = g.r_off[gi], r_cnt = g.r_cnt[gi];
     const bool a_is_l = l_cnt >= r_cnt;
     const uint32_t a_cnt = a_is_l ? l_cnt : r_cnt, b_cnt = a_is_l ? r_cnt : l_cnt;
     const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
-    const uint32_t *perm_a = a_is_l ? a.perm_l : a.perm_r;
-    const uint32_t *perm_b = a_is_l ? a.perm_r : a.perm_l;
-    const SigRow *sig_a = (a_is_l ? jsig_l : jsig_r) + a_off;
-    const SigRow *sig_b = (a_is_l ? jsig_r : jsig_l) + b_off;
-    const uint32_t *code_a = (a_is_l ? jcode_l : jcode_r) + (uint64_t)a_off * kMaxEq;
-    const uint32_t *code_b = (a_is_l ? jcode_r : jcode_l) + (uint64_t)b_off * kMaxEq;
+    const SideDesc &sa = d.s[a_is_l ? 0 : 1], &sb = d.s[a_is_l ? 1 : 0];
+    const uint32_t *perm_a = sa.perm;
+    const uint32_t *perm_b = sb.perm;
+    const SigRow *sig_a = sa.jsig + a_off;
+    const SigRow *sig_b = sb.jsig + b_off;
+    const uint32_t *code_a = sa.jcode + (uint64_t)a_off * kMaxEq;
+    const uint32_t *code_b = sb.jcode + (uint64_t)b_off * kMaxEq;
+    const uint64_t local = item - g.item_off[gi];
     const uint32_t n_bseg = (b_cnt + tile_b - 1) / tile_b;
-    const uint64_t local = item - a.g.item_off[gi];
     const uint32_t a_tile = (uint32_t)(local / n_bseg), b_seg = (uint32_t)(local % n_bseg);
     bool a_valid[AR];
     uint32_t a_idx[AR], ah[AR][8], apre[AR], la[AR], acode[AR][NEQ > 0 ? NEQ : 1];
@@ -512,7 +560,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6
     for (uint32_t cb = b_begin; cb < b_end; cb += kChunk) {
       const uint32_t n_b = min(kChunk, b_end - cb);
       __syncthreads();
-      if (bo.n >= kBlockOut / 2) block_flush(bo, a.out);  // bo.n is group-uniform here
+      if (bo.n >= kBlockOut / 2) block_flush(bo, out, bd.overflow);  // bo.n is group-uniform here
       uint32_t my_len = 0;
       if (tid < n_b) {
         if (SIM) {
@@ -590,8 +638,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6
                 if (__builtin_amdgcn_ballot_w64(ok[u][r]) == 0) continue;
                 const uint32_t b_row = perm_b[b_off + cb + min(jj + u, n_b - 1)];
                 const uint32_t a_row = ok[u][r] ? perm_a[a_off + a_idx[r]] : 0u;
-                block_append(ok[u][r], a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, bo,
-                             a.out);
+                block_append(ok[u][r], a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, jr, bo,
+                             out, bd.overflow);
               }
             }
           }
@@ -600,61 +648,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6
       }
     }
   }
-  block_flush(bo, a.out);
+  block_flush(bo, out, bd.overflow);
 }
 
 // Pairs of small groups, one pair per lane.
 template <int NEQ, bool SIM>
-__global__ __launch_bounds__(256) void k_filter_small(const FilterArgs a,
-                                                      const SigRow *__restrict__ jsig_l,
-                                                      const SigRow *__restrict__ jsig_r,
-                                                      const uint32_t *__restrict__ jcode_l,
-                                                      const uint32_t *__restrict__ jcode_r) {
+__global__ __launch_bounds__(256) void k_filter_small(const BatchDev bd, const GroupArrays g,
+                                                      uint32_t g_begin, uint32_t g_end,
+                                                      PairList out) {
   __shared__ BlockOut bo;
   if (threadIdx.x == 0) bo.n = 0;
   __syncthreads();
-  const uint32_t n_runs = *a.g.n_runs;
-  const uint64_t total = a.g.pair_off[a.g.cap];
+  const uint64_t begin = g.pair_off[g_begin], total = g.pair_off[g_end];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   // Trip count is uniform per wave: every lane runs the same number of
   // rounds so the ballots inside see all lanes.
-  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  const uint64_t base0 = begin + (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
   for (uint64_t q0 = base0; q0 < total; q0 += stride) {
     const uint64_t q = q0 + lane_id();
     bool ok = q < total;
-    uint32_t l_row = 0, r_row = 0, li = 0, ri = 0;
+    uint32_t l_row = 0, r_row = 0, li = 0, ri = 0, jr = 0;
+    const JRDesc *d = nullptr;
     if (ok) {
-      const uint32_t gi = find_group(a.g.pair_off, n_runs, q);
-      const uint64_t local = q - a.g.pair_off[gi];
-      const uint32_t r_cnt = a.g.r_cnt[gi];
-      li = a.g.l_off[gi] + (uint32_t)(local / r_cnt);
-      ri = a.g.r_off[gi] + (uint32_t)(local % r_cnt);
-      l_row = a.perm_l[li];
-      r_row = a.perm_r[ri];
+      const uint32_t gi = g_begin + find_group(g.pair_off + g_begin, g_end - g_begin, q);
+      jr = find_le_u32(bd.jr_group_off, bd.n_jr, gi);
+      d = bd.jrs + jr;
+      const uint64_t local = q - g.pair_off[gi];
+      const uint32_t r_cnt = g.r_cnt[gi];
+      li = g.l_off[gi] + (uint32_t)(local / r_cnt);
+      ri = g.r_off[gi] + (uint32_t)(local % r_cnt);
+      l_row = d->s[0].perm[li];
+      r_row = d->s[1].perm[ri];
 #pragma unroll
       for (int k = 0; k < NEQ; k++)
-        ok = ok && (jcode_l[(uint64_t)li * kMaxEq + k] == jcode_r[(uint64_t)ri * kMaxEq + k]);
+        ok = ok && (d->s[0].jcode[(uint64_t)li * kMaxEq + k] ==
+                    d->s[1].jcode[(uint64_t)ri * kMaxEq + k]);
     }
     if (SIM && ok) {
-      const SigRow sa = jsig_l[li], sb = jsig_r[ri];
-      ok = may_pass(sa.h, sa.pre, sa.len, sb.h, sb.pre, sb.len, a.c0, a.c1, a.c2, a.c3);
+      const SigRow sa = d->s[0].jsig[li], sb = d->s[1].jsig[ri];
+      ok = may_pass(sa.h, sa.pre, sa.len, sb.h, sb.pre, sb.len, d->c0, d->c1, d->c2, d->c3);
     }
-    block_append(ok, l_row, r_row, bo, a.out);
+    block_append(ok, l_row, r_row, jr, bo, out, bd.overflow);
   }
-  block_flush(bo, a.out);  // every wave of the group reaches this point
+  block_flush(bo, out, bd.overflow);  // every wave of the group reaches this point
 }
 
-// Copies per-row signatures and equality codes into join order.
-__global__ void k_gather_join(const uint32_t *__restrict__ perm, uint32_t n,
-                              const SigRow *__restrict__ sig, SigRow *__restrict__ jsig,
-                              EqCheck eq, int side, uint32_t *__restrict__ jcode) {
+// kMaxEq codes per join-order position from the rule's extra equality
+// columns; three columns are stored as four, the third repeated, so the
+// filter can check them as four.
+__global__ void k_gather_codes(const uint32_t *__restrict__ perm, uint32_t n, CodeCols cols,
+                               uint32_t *__restrict__ jcode) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t row = perm[i];
-  if (sig) jsig[i] = sig[row];
-  for (int k = 0; k < eq.n; k++)
-    jcode[(uint64_t)i * kMaxEq + k] = (side == 0 ? eq.code_l[k] : eq.code_r[k])[row];
-  if (eq.n == 3) jcode[(uint64_t)i * kMaxEq + 3] = jcode[(uint64_t)i * kMaxEq + 2];
+  for (int k = 0; k < cols.n; k++) jcode[(uint64_t)i * kMaxEq + k] = cols.col[k][row];
+  if (cols.n == 3) jcode[(uint64_t)i * kMaxEq + 3] = jcode[(uint64_t)i * kMaxEq + 2];
 }
 
 // ---- exact Jaro-Winkler -----------------------------------------------------
@@ -727,39 +775,81 @@ __device__ int common_prefix(const uint8_t *a, const uint8_t *b, int n) {
 }
 
 struct ExactArgs {
-  const uint32_t *pairs;
-  const unsigned long long *n_pairs;
-  unsigned long long in_cap;
-  DevColumn col_l, col_r;
-  float threshold;
-  int32_t slot;
-  StageOut out;
-  AuditOut audit;
-  unsigned long long *jw_count;
-  StageOut defer;  // pairs whose shorter string exceeds 256 bytes
+  BatchDev b;
+  PairList in;
+  int stage;        // 1-based index of the similarity predicate evaluated
+  PairList next;    // pairs with further predicates
+  PairList final_;  // pairs that passed their JR's last predicate (tagged with the job)
+  PairList defer;   // pairs whose shorter string exceeds 256 bytes
+  AuditList audit;
 };
 
-// Decision, output and audit of one evaluated pair (lane-local).
-__device__ __forceinline__ void emit_result(const ExactArgs &a, uint32_t l, uint32_t r, int m,
-                                            int S, int T, int p, bool ok) {
-  const double thr = (double)a.threshold;
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// Inserts (l, r) into the job's output set; true if it was not there yet.
+// Jobs with one rule have no set (each pair is evaluated once) and always
+// insert.  A full set raises the overflow flag.
+__device__ bool set_insert(const BatchDev &b, uint32_t job, uint32_t l, uint32_t r) {
+  const unsigned long long cap = b.sets.cap ? b.sets.cap[job] : 0ull;
+  if (cap == 0) return true;
+  unsigned long long *slots = b.sets.slots + b.sets.off[job];
+  const unsigned long long key = ((unsigned long long)l << 32) | r;
+  unsigned long long slot = mix64(key) & (cap - 1);
+  for (unsigned long long probe = 0; probe < cap; probe++) {
+    const unsigned long long prev = atomicCAS(slots + slot, kEmpty, key);
+    if (prev == kEmpty) return true;
+    if (prev == key) return false;
+    slot = (slot + 1) & (cap - 1);
+  }
+  *b.overflow = 1ull;
+  return false;
+}
+
+// A pair that satisfies a whole rule: appended to the final list once per
+// job (called by every lane of the wave).
+__device__ __forceinline__ void finalize(const ExactArgs &a, bool ok, uint32_t l, uint32_t r,
+                                         uint32_t job) {
+  const bool emit = ok && set_insert(a.b, job, l, r);
+  list_append(emit, PairEntry{l, r, job, 0u}, a.final_, a.b.overflow);
+  job_count(a.b, kJobMatches, emit, job);
+}
+
+// Decision, output and audit of one evaluated pair (lane-local values,
+// called by every lane of the wave).
+__device__ __forceinline__ void emit_result(const ExactArgs &a, const PairEntry &e, uint32_t job,
+                                            bool last, float thr_f, int32_t slot, int m, int S,
+                                            int T, int p, bool ok) {
+  const double thr = (double)thr_f;
   const float jaro = jaro_value(m, S, T);
   const double jw = winkler(jaro, p);
-  wave_append(ok && !(jw < thr), l, r, a.out);
-  if (ok && fabs(jw - thr) < 1e-6) {
+  const bool pass = ok && !(jw < thr);
+  finalize(a, pass && last, e.l, e.r, job);
+  list_append(pass && !last, e, a.next, a.b.overflow);
+  const bool band = ok && fabs(jw - thr) < 1e-6;
+  if (band) {
     const double jf = winkler_fused(jaro, p);
     const unsigned long long pos = atomicAdd(a.audit.count, 1ull);
     if (pos < a.audit.cap) {
-      hb_audit_entry *e = reinterpret_cast<hb_audit_entry *>(a.audit.entries) + pos;
-      e->l = l;
-      e->r = r;
-      e->slot = a.slot;
-      e->flags = (((jw < thr) != (jf < thr)) ? 1 : 0) | (S >= 257 ? 2 : 0);
-      e->jw = jw;
+      AuditRec &rec = a.audit.e[pos];
+      rec.e.l = e.l;
+      rec.e.r = e.r;
+      rec.e.slot = slot;
+      rec.e.flags = (((jw < thr) != (jf < thr)) ? 1 : 0) | (S >= 257 ? 2 : 0);
+      rec.e.jw = jw;
+      rec.job = job;
+      rec.pad = 0;
     } else {
-      *a.out.overflow = 1ull;
+      *a.b.overflow = 1ull;
     }
   }
+  job_count(a.b, kJobAudit, band, job);
 }
 
 // Common prefix length of two cells up to n bytes, one lane, 16 bytes at a
@@ -878,33 +968,45 @@ __device__ __forceinline__ int lane_matches(const uint32_t *row_planes, int nwc,
 // One pair per lane.  Pairs whose shorter string is longer than 256 bytes
 // are deferred to k_exact (one wave per pair, 257-slot semantics).
 __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
-  const unsigned long long n_in = *a.n_pairs;
-  const unsigned long long n_use = n_in < a.in_cap ? n_in : a.in_cap;
+  const unsigned long long n_in = *a.in.count;
+  const unsigned long long n_use = n_in < a.in.cap ? n_in : a.in.cap;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  unsigned long long evals = 0;
   for (uint64_t q0 = base0; q0 < n_use; q0 += stride) {
     const uint64_t idx = q0 + lane_id();
     bool act = idx < n_use;
-    uint32_t l = 0, r = 0;
+    PairEntry e{0u, 0u, 0u, 0u};
+    uint32_t job = 0, sim_off = 0;
+    int32_t n_sims = 0;
+    if (act) {
+      e = a.in.e[idx];
+      const JRDesc &d = a.b.jrs[e.jr];
+      job = d.job;
+      n_sims = d.n_sims;
+      sim_off = d.sim_off;
+    }
+    if (a.stage == 1) job_count(a.b, kJobFiltered, act, job);
+    // a rule without similarity predicates is complete after the filter
+    finalize(a, act && n_sims == 0, e.l, e.r, job);
+    act = act && n_sims >= a.stage;
+    SimDesc sd{};
     int la = 0, lb = 0;
     if (act) {
-      l = a.pairs[2 * idx];
-      r = a.pairs[2 * idx + 1];
-      la = a.col_l.len[l];
-      lb = a.col_r.len[r];
+      sd = a.b.sims[sim_off + a.stage - 1];
+      la = sd.col_l.len[e.l];
+      lb = sd.col_r.len[e.r];
     }
     const int S = min(la, lb), T = max(la, lb);
     const bool deferred = act && S > 256;
-    wave_append(deferred, l, r, a.defer);
+    list_append(deferred, e, a.defer, a.b.overflow);
     act = act && !deferred;
     const bool swap = la > lb;  // device_func.cuh:71-79: left stays inner on ties
-    const uint8_t *ca = a.col_l.bytes + (uint64_t)l * a.col_l.width;
-    const uint8_t *cb = a.col_r.bytes + (uint64_t)r * a.col_r.width;
+    const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
+    const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
     const uint8_t *outer = swap ? ca : cb;
-    const uint32_t *planes = swap ? a.col_r.planes + (uint64_t)r * a.col_r.nwc * 8
-                                  : a.col_l.planes + (uint64_t)l * a.col_l.nwc * 8;
-    const int nwc = (int)(swap ? a.col_r.nwc : a.col_l.nwc);
+    const uint32_t *planes = swap ? sd.col_r.planes + (uint64_t)e.r * sd.col_r.nwc * 8
+                                  : sd.col_l.planes + (uint64_t)e.l * sd.col_l.nwc * 8;
+    const int nwc = (int)(swap ? sd.col_r.nwc : sd.col_l.nwc);
     const int p = act ? lane_prefix(ca, cb, S) : 0;
     const bool live = act && S > 0;
     int m = 0;
@@ -915,34 +1017,32 @@ __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
     else if (!gt128) m = lane_matches<4>(planes, nwc, S, T, outer, live);
     else m = lane_matches<8>(planes, nwc, S, T, outer, live);
     if (!live) m = 0;
-    emit_result(a, l, r, m, S, T, p, act);
-    evals += act;
+    emit_result(a, e, job, n_sims == a.stage, sd.thr, sd.slot, m, S, T, p, act);
+    job_count(a.b, kJobJw, act, job);
   }
-  for (int off = 32; off > 0; off >>= 1) evals += __shfl_xor(evals, off);
-  if (lane_id() == 0 && evals) atomicAdd(a.jw_count, evals);
 }
 
 // One wave per pair (scalar greedy); used for pairs deferred by
 // k_exact_lanes, i.e. shorter strings of more than 256 bytes.
 __global__ __launch_bounds__(256) void k_exact(ExactArgs a) {
   const uint32_t lane = lane_id();
-  const unsigned long long n_in = *a.n_pairs;
-  const unsigned long long n_use = n_in < a.in_cap ? n_in : a.in_cap;
+  const unsigned long long n_in = *a.in.count;
+  const unsigned long long n_use = n_in < a.in.cap ? n_in : a.in.cap;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t n_waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  const double thr = (double)a.threshold;
-  unsigned long long evals = 0;
   for (uint64_t idx = wave; idx < n_use; idx += n_waves) {
-    const uint32_t l = a.pairs[2 * idx], r = a.pairs[2 * idx + 1];
-    const uint8_t *ca = a.col_l.bytes + (uint64_t)l * a.col_l.width;
-    const uint8_t *cb = a.col_r.bytes + (uint64_t)r * a.col_r.width;
-    const int la = a.col_l.len[l], lb = a.col_r.len[r];
+    const PairEntry e = a.in.e[idx];
+    const JRDesc &d = a.b.jrs[e.jr];
+    const SimDesc sd = a.b.sims[d.sim_off + a.stage - 1];
+    const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
+    const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
+    const int la = sd.col_l.len[e.l], lb = sd.col_r.len[e.r];
     const int S = min(la, lb), T = max(la, lb);
     // device_func.cuh:71-79: the left cell stays inner unless strictly longer.
     const bool swap = la > lb;
     const uint8_t *inner = swap ? cb : ca;
     const uint8_t *outer = swap ? ca : cb;
-    const uint32_t outer_width = swap ? a.col_l.width : a.col_r.width;
+    const uint32_t outer_width = swap ? sd.col_l.width : sd.col_r.width;
     const int p = common_prefix(ca, cb, S);
     int m = 0;
     if (S > 0) {
@@ -955,81 +1055,51 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs a) {
       else
         m = greedy_matches<5>(inner, S, odw, ndw, T);
     }
-    const float jaro = jaro_value(m, S, T);
-    const double jw = winkler(jaro, p);
-    evals++;
-    if (lane == 0) {
-      const bool pass = !(jw < thr);
-      if (pass) {
-        const unsigned long long pos = atomicAdd(a.out.count, 1ull);
-        if (pos < a.out.cap) {
-          a.out.pairs[2 * pos] = l;
-          a.out.pairs[2 * pos + 1] = r;
-        } else {
-          *a.out.overflow = 1ull;
-        }
-      }
-      if (fabs(jw - thr) < 1e-6) {
-        const double jf = winkler_fused(jaro, p);
-        const unsigned long long pos = atomicAdd(a.audit.count, 1ull);
-        if (pos < a.audit.cap) {
-          hb_audit_entry *e = reinterpret_cast<hb_audit_entry *>(a.audit.entries) + pos;
-          e->l = l;
-          e->r = r;
-          e->slot = a.slot;
-          e->flags = (((jw < thr) != (jf < thr)) ? 1 : 0) | (S >= 257 ? 2 : 0);
-          e->jw = jw;
-        } else {
-          *a.out.overflow = 1ull;
-        }
+    emit_result(a, e, d.job, d.n_sims == a.stage, sd.thr, sd.slot, m, S, T, p, lane == 0);
+    job_count(a.b, kJobJw, lane == 0, d.job);
+  }
+}
+
+// Copies the final pairs and the audit entries into per-job ranges.
+__global__ void k_scatter(const BatchDev b, const PairList fin, const uint64_t *__restrict__ out_off,
+                          unsigned long long *fill, uint32_t *__restrict__ out_pairs,
+                          unsigned long long out_cap, const AuditList au,
+                          const uint64_t *__restrict__ audit_off,
+                          unsigned long long *audit_fill, hb_audit_entry *__restrict__ out_audit,
+                          unsigned long long audit_cap) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  const unsigned long long n = min(*fin.count, fin.cap);
+  for (uint64_t q0 = base0; q0 < n; q0 += stride) {
+    const uint64_t idx = q0 + lane_id();
+    const bool act = idx < n;
+    PairEntry e{0u, 0u, 0u, 0u};
+    if (act) e = fin.e[idx];
+    const unsigned long long k = job_slot(act, e.jr, fill);
+    if (act) {
+      const unsigned long long pos = out_off[e.jr] + k;
+      if (pos < out_cap) {
+        out_pairs[2 * pos] = e.l;
+        out_pairs[2 * pos + 1] = e.r;
+      } else {
+        *b.overflow = 1ull;
       }
     }
   }
-  if (lane == 0 && evals) atomicAdd(a.jw_count, evals);
-}
-
-// ---- output merge -------------------------------------------------------------
-
-__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
-  x ^= x >> 33;
-  x *= 0xff51afd7ed558ccdull;
-  x ^= x >> 33;
-  x *= 0xc4ceb9fe1a85ec53ull;
-  x ^= x >> 33;
-  return x;
-}
-
-__global__ void k_merge_unique(const uint32_t *__restrict__ pairs,
-                               const unsigned long long *__restrict__ n_pairs,
-                               unsigned long long in_cap, unsigned long long *set,
-                               unsigned long long set_cap, StageOut out) {
-  const unsigned long long n = *n_pairs < in_cap ? *n_pairs : in_cap;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    const uint32_t l = pairs[2 * i], r = pairs[2 * i + 1];
-    const unsigned long long key = ((unsigned long long)l << 32) | r;
-    unsigned long long slot = mix64(key) & (set_cap - 1);
-    bool placed = false;
-    for (unsigned long long probe = 0; probe < set_cap; probe++) {
-      const unsigned long long prev = atomicCAS(set + slot, kEmpty, key);
-      if (prev == kEmpty) {
-        const unsigned long long pos = atomicAdd(out.count, 1ull);
-        if (pos < out.cap) {
-          out.pairs[2 * pos] = l;
-          out.pairs[2 * pos + 1] = r;
-        } else {
-          *out.overflow = 1ull;
-        }
-        placed = true;
-        break;
-      }
-      if (prev == key) {
-        placed = true;
-        break;
-      }
-      slot = (slot + 1) & (set_cap - 1);
+  const unsigned long long na = min(*au.count, au.cap);
+  for (uint64_t q0 = base0; q0 < na; q0 += stride) {
+    const uint64_t idx = q0 + lane_id();
+    const bool act = idx < na;
+    uint32_t job = 0;
+    if (act) job = au.e[idx].job;
+    const unsigned long long k = job_slot(act, job, audit_fill);
+    if (act) {
+      const unsigned long long pos = audit_off[job] + k;
+      if (pos < audit_cap)
+        out_audit[pos] = au.e[idx].e;
+      else
+        *b.overflow = 1ull;
     }
-    if (!placed) *out.overflow = 1ull;
   }
 }
 
@@ -1099,57 +1169,47 @@ void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
                      flags);
 }
 
-void launch_run_info(const uint64_t *sorted_l, const uint32_t *flags, const uint32_t *run_id,
-                     uint32_t n_l, const uint64_t *sorted_r, uint32_t n_r, GroupArrays g,
-                     hipStream_t stream) {
-  hipLaunchKernelGGL(k_run_info, dim3(grid_for((uint64_t)n_l + 1, 256)), dim3(256), 0, stream,
-                     sorted_l, flags, run_id, n_l, sorted_r, n_r, g);
+void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
+                        const uint32_t *run_id, uint32_t n, uint32_t *run_code,
+                        uint32_t *run_off, uint32_t *n_runs, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_layout_runs, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted_keys,
+                     flags, run_id, n, run_code, run_off, n_runs);
 }
 
-void launch_single_group(uint32_t n_l, uint32_t n_r, GroupArrays g, hipStream_t stream) {
-  hipLaunchKernelGGL(k_single_group, dim3(1), dim3(64), 0, stream, n_l, n_r, g);
+void launch_gather_codes(const uint32_t *perm, uint32_t n, CodeCols cols, uint32_t *jcode,
+                         hipStream_t stream) {
+  if (!n || cols.n == 0) return;
+  hipLaunchKernelGGL(k_gather_codes, dim3(grid_for(n, 256)), dim3(256), 0, stream, perm, n, cols,
+                     jcode);
 }
 
-void launch_gather_join(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
-                        EqCheck eq, int side, uint32_t *jcode, hipStream_t stream) {
-  if (!n || (!sig && eq.n == 0)) return;
-  hipLaunchKernelGGL(k_gather_join, dim3(grid_for(n, 256)), dim3(256), 0, stream, perm, n, sig,
-                     jsig, eq, side, jcode);
+void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch_join, dim3(grid_for((uint64_t)g.n + 1, 256)), dim3(256), 0, stream,
+                     b, g);
 }
 
-void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t *perm_r,
-                          const SigRow *jsig_l, const SigRow *jsig_r, const uint32_t *jcode_l,
-                          const uint32_t *jcode_r, int n_eq, int has_sim, const float coef[4],
-                          StageOut out, FilterShape shape, int n_cu, hipStream_t stream) {
-  FilterArgs a;
-  a.g = g;
-  a.perm_l = perm_l;
-  a.perm_r = perm_r;
-  a.n_eq = n_eq;
-  a.c0 = coef[0];
-  a.c1 = coef[1];
-  a.c2 = coef[2];
-  a.c3 = coef[3];
-  a.out = out;
+void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, int n_eq,
+                   bool has_sim, PairList out, FilterShape shape, int n_cu, hipStream_t stream) {
+  if (g_end <= g_begin) return;
   // The tile grid launches grid_per_cu work-groups per CU (16 by default,
   // two rounds of the 8 that fit at once), each taking work items in a
   // grid-stride loop; surplus groups find no item and exit.
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
   const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 4;
-  const uint32_t tile_b = shape.tile_b;
 #define HB_FILTER(NEQ, SIM)                                                                   \
   do {                                                                                        \
     if (shape.rows_per_lane == 4)                                                             \
       hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 4, 1>), dim3(grid), dim3(256), 0, stream,  \
-                         a, jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                        \
+                         b, g, g_begin, g_end, out);                                          \
     else if (shape.rows_per_lane == 2)                                                        \
       hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 2, 2>), dim3(grid), dim3(256), 0, stream,  \
-                         a, jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                        \
+                         b, g, g_begin, g_end, out);                                          \
     else                                                                                      \
       hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 1, 2>), dim3(grid), dim3(256), 0, stream,  \
-                         a, jsig_l, jsig_r, jcode_l, jcode_r, tile_b);                        \
-    hipLaunchKernelGGL((k_filter_small<NEQ, SIM>), dim3(grid_small), dim3(256), 0, stream, a, \
-                       jsig_l, jsig_r, jcode_l, jcode_r);                                     \
+                         b, g, g_begin, g_end, out);                                          \
+    hipLaunchKernelGGL((k_filter_small<NEQ, SIM>), dim3(grid_small), dim3(256), 0, stream, b, \
+                       g, g_begin, g_end, out);                                               \
   } while (0)
   // Three extra equalities run as four: the gathered codes repeat the third.
   const int neq = n_eq == 3 ? 4 : n_eq;
@@ -1171,37 +1231,24 @@ void launch_filter_groups(GroupArrays g, const uint32_t *perm_l, const uint32_t 
 #undef HB_FILTER
 }
 
-void launch_exact(const uint32_t *pairs, const unsigned long long *n_pairs,
-                  unsigned long long in_cap, DevColumn col_l, DevColumn col_r, float threshold,
-                  int32_t slot, StageOut out, AuditOut audit, unsigned long long *jw_count,
-                  StageOut defer, int n_cu, hipStream_t stream) {
-  ExactArgs a;
-  a.pairs = pairs;
-  a.n_pairs = n_pairs;
-  a.in_cap = in_cap;
-  a.defer = defer;
-  a.col_l = col_l;
-  a.col_r = col_r;
-  a.threshold = threshold;
-  a.slot = slot;
-  a.out = out;
-  a.audit = audit;
-  a.jw_count = jw_count;
+void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
+                  PairList defer, AuditList audit, int n_cu, hipStream_t stream) {
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
+  ExactArgs a{b, in, stage, next, final_, defer, audit};
   hipLaunchKernelGGL(k_exact_lanes, dim3(cus * 8), dim3(256), 0, stream, a);
-  ExactArgs b = a;
-  b.pairs = defer.pairs;
-  b.n_pairs = defer.count;
-  b.in_cap = defer.cap;
-  hipLaunchKernelGGL(k_exact, dim3(cus * 4), dim3(256), 0, stream, b);
+  ExactArgs d{b, defer, stage, next, final_, defer, audit};
+  d.defer.cap = 0;  // deferred pairs are not deferred again
+  hipLaunchKernelGGL(k_exact, dim3(cus * 4), dim3(256), 0, stream, d);
 }
 
-void launch_merge_unique(const uint32_t *pairs, const unsigned long long *n_pairs,
-                         unsigned long long in_cap, unsigned long long *set,
-                         unsigned long long set_cap, StageOut out, int n_cu, hipStream_t stream) {
-  const unsigned grid = (unsigned)(n_cu > 0 ? n_cu : 256) * 4;
-  hipLaunchKernelGGL(k_merge_unique, dim3(grid), dim3(256), 0, stream, pairs, n_pairs, in_cap,
-                     set, set_cap, out);
+void launch_scatter(BatchDev b, PairList final_, const uint64_t *out_off,
+                    unsigned long long *fill, uint32_t *out_pairs, unsigned long long out_cap,
+                    AuditList audit, const uint64_t *audit_off, unsigned long long *audit_fill,
+                    hb_audit_entry *out_audit, unsigned long long audit_cap, int n_cu,
+                    hipStream_t stream) {
+  const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
+  hipLaunchKernelGGL(k_scatter, dim3(cus * 2), dim3(256), 0, stream, b, final_, out_off, fill,
+                     out_pairs, out_cap, audit, audit_off, audit_fill, out_audit, audit_cap);
 }
 
 void launch_jw_from_counts(const int32_t *mstp, int64_t n, double *out, double *out_fused,

@@ -1,4 +1,4 @@
-// C ABI runtime: contexts, device tables, plans and blocking jobs.
+// C ABI runtime: contexts, device tables, plans, batches and jobs.
 //
 // Replaces the reference's submit/complete path: HostProducer::AsyncSubmit
 // (core/components/host_producer.h:109-215) and HostReducer::Run
@@ -6,9 +6,12 @@
 //   - device buffers come from a per-context cache, not a cudaMalloc per call
 //     that is never freed;
 //   - tables are copied host-to-device once (not twice) and may stay resident;
+//     per-table structures (signatures, codes, join layouts) are built once;
+//   - jobs run in batches: one launch sequence covers every job of a batch
+//     (a plain job is a batch of one);
 //   - results are index pairs with no 65,536 cap: a buffer that overflows is
-//     grown and the job re-run;
-//   - completion is an event per job, queried or waited on, not a busy poll.
+//     grown and the batch re-run;
+//   - completion is an event per batch, queried or waited on, not a busy poll.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -231,6 +234,7 @@ class Dictionary {
 using namespace hbrt;
 
 struct Job;
+struct Batch;
 
 struct hb_ctx {
   int device = 0;
@@ -243,7 +247,7 @@ struct hb_ctx {
   HostPool hpool;
   EventPool events;
   Dictionary dict;
-  std::mutex mu;  // jobs map
+  std::mutex mu;  // jobs map, shape
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
 };
 
@@ -273,12 +277,18 @@ struct Column {
 
 // The rows of a table in join order for one rule shape: sorted by the code
 // of its first equality column, then by the length of its first similarity
-// column (either may be absent, -1).  Kept for the table's lifetime.
+// column (either may be absent, -1), with the runs of equal code and, per
+// list of extra equality columns, their codes gathered into that order.
+// Kept for the table's lifetime.
 struct Layout {
   Lazy lz;
   uint64_t *d_keys = nullptr;     // sorted (code << 8) | min(length, 255)
   uint32_t *d_perm = nullptr;     // row id of each join-order position
   hb::SigRow *d_jsig = nullptr;   // similarity signatures in join order
+  uint32_t *d_run_code = nullptr;
+  uint32_t *d_run_off = nullptr;  // n_runs + 1
+  uint32_t n_runs = 0;
+  std::map<std::vector<int>, uint32_t *> jcode;  // extra equality columns -> codes
 };
 
 }  // namespace
@@ -291,7 +301,6 @@ struct hb_table {
   std::map<std::pair<int, int>, Layout> layouts;  // (equality column, similarity column)
   hipEvent_t ready = nullptr;
   std::mutex mu;  // guards lazy builds
-  std::atomic<int> users{0};
 };
 
 struct PlanRule {
@@ -314,104 +323,99 @@ struct hb_plan {
 
 namespace {
 
-enum class JobState { kRunning, kDone, kFailed };
+enum class BatchState { kRunning, kDone, kFailed };
 
-// Counter slots of a job (device, unsigned long long).
+// Layout of the batch's counter block (device, unsigned long long; copied to
+// pinned host memory at the end of the batch): fixed slots, then per stage
+// the list and defer counts, then the per-job counters and output offsets.
 enum : int {
   kCntOverflow = 0,
-  kCntJw = 1,
+  kCntFinal = 1,
   kCntAudit = 2,
-  kCntJoined = 3,
-  kCntOut = 4,
-  kCntFiltered = 5,
-  kCntStage0 = 8,
+  kCntFixed = 4,
 };
 
 }  // namespace
 
-struct Job {
-  uint64_t id = 0;
+// Jobs run in batches: one launch sequence over all job-rules of the batch.
+struct Batch {
   hb_ctx *ctx = nullptr;
   hipStream_t stream = nullptr;
-  const hb_plan *plan = nullptr;
-  hb_table *L = nullptr, *R = nullptr;
   std::mutex mu;
-  JobState state = JobState::kRunning;
+  BatchState state = BatchState::kRunning;
   hb_status error = HB_OK;
   std::string error_text;
-  // capacities (pairs / entries)
-  unsigned long long list_cap = 0, out_cap = 0, audit_cap = 0, set_cap = 0;
+  struct Item {
+    const hb_plan *plan;
+    hb_table *L, *R;
+  };
+  std::vector<Item> items;
+  hb::FilterShape shape{2, 256, 16};
+  int max_stage = 1;             // exact stages launched (>= 1)
+  // capacities (entries)
+  unsigned long long list_cap = 0, defer_cap = 0, final_cap = 0, audit_cap = 0;
+  std::vector<unsigned long long> set_cap;  // per job, 0 for jobs with one rule
   int retries = 0;
-  hb::FilterShape shape{2, 256, 16};  // the context's filter shape at submit
-  // device memory
+  // memory
   struct Alloc {
     void *p;
     size_t n;
   };
-  std::vector<Alloc> allocs;
-  std::vector<Alloc> host_allocs;       // pinned, from the context's host pool
-  unsigned long long *d_cnt = nullptr;
-  unsigned long long *h_cnt = nullptr;  // pinned copy of the counters
-  int n_counters = 0;
-  std::vector<int> filter_counters;  // per rule: counter of stage 0, or -1
+  std::vector<Alloc> allocs, host_allocs;
+  unsigned long long *h_cnt = nullptr;  // pinned copy of the counter block
+  size_t cnt_words = 0;
+  int cnt_stage_off = 0, cnt_job_off = 0, cnt_out_off = 0, cnt_audit_off = 0;
+  uint32_t *h_out = nullptr;           // pinned, job-contiguous (l, r) pairs
+  hb_audit_entry *h_audit = nullptr;   // pinned, job-contiguous audit entries
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
-  std::vector<hipEvent_t> marks;  // per rule: join, filter, exact stage boundaries
-  size_t n_marks = 0;
-  // outputs: pinned host memory written by the kernels through its device
-  // address (d_*), read by the caller at the host address (h_*)
-  uint32_t *d_out = nullptr, *h_out = nullptr;
-  hb_audit_entry *d_audit = nullptr, *h_audit = nullptr;
+  hipEvent_t marks[3] = {nullptr, nullptr, nullptr};  // after join, filter, exact
+  float join_ms = 0, filter_ms = 0, exact_ms = 0, device_ms = 0;
+
+  uint32_t n_jobs() const { return (uint32_t)items.size(); }
+  uint32_t stride() const { return n_jobs() + 1; }
+  unsigned long long job_counter(int k, uint32_t j) const {
+    return h_cnt[cnt_job_off + (size_t)k * stride() + j];
+  }
+  ~Batch();
+};
+
+struct Job {
+  uint64_t id = 0;
+  hb_ctx *ctx = nullptr;
+  std::shared_ptr<Batch> batch;
+  uint32_t index = 0;
   hb_job_stats stats{};
+  bool stats_done = false;
 };
 
 namespace {
 
-hb_status mark(Job *j, hipStream_t st) {
-  if (j->n_marks == j->marks.size()) {
-    hipEvent_t e;
-    HB_TRY(j->ctx->events.get(&e));
-    j->marks.push_back(e);
-  }
-  HIP_OK(hipEventRecord(j->marks[j->n_marks++], st));
-  return HB_OK;
-}
-
-hb_status job_alloc(Job *j, size_t bytes, void **out) {
-  HB_TRY(j->ctx->pool.alloc(bytes, out));
-  j->allocs.push_back({*out, bytes});
-  return HB_OK;
-}
-
-hb_status job_alloc_host(Job *j, size_t bytes, void **host, void **dev) {
-  HB_TRY(j->ctx->hpool.alloc(bytes, host, dev));
-  j->host_allocs.push_back({*host, bytes});
-  return HB_OK;
-}
-
-// Returns the job's device and pinned output memory to the context's pools
-// (the job's stream must be idle).
-void job_free_device(Job *j) {
-  for (auto &a : j->allocs) j->ctx->pool.free(a.p, a.n);
-  j->allocs.clear();
-  for (auto &a : j->host_allocs) j->ctx->hpool.free(a.p, a.n);
-  j->host_allocs.clear();
-}
-
-// Returns everything a finished job holds to the context's pools.
-void job_free_all(Job *j) {
-  job_free_device(j);
-  if (j->h_cnt) j->ctx->hpool.free(j->h_cnt, (size_t)j->n_counters * sizeof(unsigned long long));
-  j->h_cnt = nullptr;
-  j->ctx->events.put(j->ev_start);
-  j->ctx->events.put(j->ev_end);
-  for (hipEvent_t e : j->marks) j->ctx->events.put(e);
-  j->ev_start = j->ev_end = nullptr;
-  j->marks.clear();
-}
-
 hb_status set_device(hb_ctx *ctx) {
   HIP_OK(hipSetDevice(ctx->device));
   return HB_OK;
+}
+
+hb_status batch_alloc(Batch *b, size_t bytes, void **out) {
+  HB_TRY(b->ctx->pool.alloc(bytes, out));
+  b->allocs.push_back({*out, bytes});
+  return HB_OK;
+}
+
+hb_status batch_alloc_host(Batch *b, size_t bytes, void **host, void **dev) {
+  HB_TRY(b->ctx->hpool.alloc(bytes, host, dev));
+  b->host_allocs.push_back({*host, bytes});
+  return HB_OK;
+}
+
+// Returns the batch's memory to the context's pools (the stream must be idle).
+void batch_free_memory(Batch *b) {
+  for (auto &a : b->allocs) b->ctx->pool.free(a.p, a.n);
+  b->allocs.clear();
+  for (auto &a : b->host_allocs) b->ctx->hpool.free(a.p, a.n);
+  b->host_allocs.clear();
+  b->h_cnt = nullptr;
+  b->h_out = nullptr;
+  b->h_audit = nullptr;
 }
 
 // Waits (on `stream`) for a lazy structure built by another stream.
@@ -465,9 +469,9 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
 
 // Builds (once per table and rule shape) the join-order layout: keys from the
 // equality column's dictionary codes and the similarity column's lengths,
-// radix-sorted with the row ids, and the similarity signatures gathered into
-// that order.
-hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, const Layout **out) {
+// radix-sorted with the row ids, the runs of equal code, and the similarity
+// signatures gathered into that order.
+hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, Layout **out) {
   if (eq >= 0) HB_TRY(ensure_code(t, eq, stream));
   if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
   uint64_t n_codes = 1;
@@ -484,31 +488,70 @@ hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, const 
   const size_t n1 = std::max<size_t>(n, 1);
   HIP_OK(hipMalloc(&ly.d_keys, n1 * sizeof(uint64_t)));
   HIP_OK(hipMalloc(&ly.d_perm, n1 * sizeof(uint32_t)));
+  HIP_OK(hipMalloc(&ly.d_run_code, n1 * sizeof(uint32_t)));
+  HIP_OK(hipMalloc(&ly.d_run_off, (n1 + 1) * sizeof(uint32_t)));
   if (sim >= 0) HIP_OK(hipMalloc(&ly.d_jsig, n1 * sizeof(hb::SigRow)));
   if (n) {
     uint64_t *keys_in = nullptr;
-    uint32_t *iota = nullptr;
+    uint32_t *iota = nullptr, *flags = nullptr, *run_id = nullptr, *h_nruns = nullptr;
     HIP_OK(hipMalloc(&keys_in, n1 * sizeof(uint64_t)));
     HIP_OK(hipMalloc(&iota, n1 * sizeof(uint32_t)));
+    HIP_OK(hipMalloc(&flags, n1 * sizeof(uint32_t)));
+    HIP_OK(hipMalloc(&run_id, (n1 + 1) * sizeof(uint32_t)));
+    HIP_OK(hipHostMalloc((void **)&h_nruns, sizeof(uint32_t), hipHostMallocMapped));
     hb::launch_layout_keys(eq >= 0 ? t->cols[eq].d_code : nullptr,
                            sim >= 0 ? t->cols[sim].d_len : nullptr, n, keys_in, stream);
     hb::launch_iota(iota, n, stream);
     int end_bit = 8;
     while (end_bit < 64 && (1ull << (end_bit - 8)) < n_codes) end_bit++;
-    size_t tb = 0;
+    size_t tb = 0, ts = 0;
     HIP_OK(hb::sort_pairs_u64(nullptr, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit,
                               stream));
+    HIP_OK(hb::exscan_u32(nullptr, ts, flags, run_id, n, stream));
     void *tmp = nullptr;
-    HIP_OK(hipMalloc(&tmp, std::max<size_t>(tb, 16)));
+    HIP_OK(hipMalloc(&tmp, std::max<size_t>(std::max(tb, ts), 16)));
     HIP_OK(hb::sort_pairs_u64(tmp, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit, stream));
+    hb::launch_run_flags(ly.d_keys, n, flags, stream);
+    HIP_OK(hb::exscan_u32(tmp, ts, flags, run_id, n, stream));
+    uint32_t *d_nruns = nullptr;
+    HIP_OK(hipHostGetDevicePointer((void **)&d_nruns, h_nruns, 0));
+    hb::launch_layout_runs(ly.d_keys, flags, run_id, n, ly.d_run_code, ly.d_run_off, d_nruns,
+                           stream);
     if (sim >= 0) hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(stream));  // one-off per layout; frees below need it
+    HIP_OK(hipStreamSynchronize(stream));  // one-off per layout: run count, frees below
+    ly.n_runs = *h_nruns;
     HIP_OK(hipFree(tmp));
     HIP_OK(hipFree(keys_in));
     HIP_OK(hipFree(iota));
+    HIP_OK(hipFree(flags));
+    HIP_OK(hipFree(run_id));
+    HIP_OK(hipHostFree(h_nruns));
   }
   return record_lazy(ly.lz, stream);
+}
+
+// Extra equality codes of `cols` in the layout's order (built once).
+hb_status ensure_jcode(hb_table *t, Layout *ly, const std::vector<int> &cols, hipStream_t stream,
+                       const uint32_t **out) {
+  for (int c : cols) HB_TRY(ensure_code(t, c, stream));
+  std::lock_guard<std::mutex> lk(t->mu);
+  auto it = ly->jcode.find(cols);
+  if (it != ly->jcode.end()) {
+    *out = it->second;
+    return HB_OK;
+  }
+  const size_t n1 = (size_t)std::max<int64_t>(t->n_rows, 1);
+  uint32_t *p = nullptr;
+  HIP_OK(hipMalloc(&p, n1 * hb::kMaxEq * sizeof(uint32_t)));
+  hb::CodeCols cc{};
+  cc.n = (int32_t)cols.size();
+  for (size_t k = 0; k < cols.size(); k++) cc.col[k] = t->cols[cols[k]].d_code;
+  hb::launch_gather_codes(ly->d_perm, (uint32_t)t->n_rows, cc, p, stream);
+  HIP_OK(hipGetLastError());
+  ly->jcode.emplace(cols, p);
+  *out = p;
+  return HB_OK;
 }
 
 float filter_coef(double thr, int p) {
@@ -525,84 +568,177 @@ bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
          tile_b <= 65536 && tile_b % 256 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
 }
 
-hb_status enqueue_job(Job *j) {
-  hb_ctx *ctx = j->ctx;
-  const hb_plan *plan = j->plan;
-  hb_table *L = j->L, *R = j->R;
-  hipStream_t st = j->stream;
-  const uint32_t n_l = (uint32_t)L->n_rows, n_r = (uint32_t)R->n_rows;
+// Filter variant of a JR: the kernel template it needs.
+int filter_variant(const PlanRule &rule) {
+  const int n_eq = rule.eq.empty() ? 0 : (int)rule.eq.size() - 1;
+  const int neq = n_eq == 3 ? 4 : n_eq;
+  return (rule.sims.empty() ? 0 : 8) + neq;
+}
 
-  HIP_OK(hipStreamWaitEvent(st, L->ready, 0));
-  if (R != L) HIP_OK(hipStreamWaitEvent(st, R->ready, 0));
-  // Lazy per-column structures and join layouts needed by the plan.
-  std::vector<std::pair<const Layout *, const Layout *>> layouts;
-  for (const PlanRule &rule : plan->rules) {
+unsigned long long pow2_at_least(unsigned long long x) {
+  unsigned long long p = 1024;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+hb_status enqueue_batch(Batch *b) {
+  hb_ctx *ctx = b->ctx;
+  hipStream_t st = b->stream;
+  const uint32_t n_jobs = b->n_jobs();
+
+  // Tables ready, lazy structures and layouts; job-rule descriptors.
+  struct JR {
+    uint32_t job;
+    int rule;
+    int variant;
+    Layout *ly[2];
+  };
+  std::vector<JR> jrs;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    Batch::Item &it = b->items[j];
+    HIP_OK(hipStreamWaitEvent(st, it.L->ready, 0));
+    if (it.R != it.L) HIP_OK(hipStreamWaitEvent(st, it.R->ready, 0));
+    for (int k = 0; k < (int)it.plan->rules.size(); k++) {
+      const PlanRule &rule = it.plan->rules[k];
+      for (const auto &s : rule.sims) {
+        HB_TRY(ensure_sig(it.L, s.cl, st));
+        HB_TRY(ensure_sig(it.R, s.cr, st));
+      }
+      JR jr{j, k, filter_variant(rule), {nullptr, nullptr}};
+      const int el = rule.eq.empty() ? -1 : rule.eq[0].first;
+      const int er = rule.eq.empty() ? -1 : rule.eq[0].second;
+      const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
+      const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
+      HB_TRY(ensure_layout(it.L, el, sl, st, &jr.ly[0]));
+      HB_TRY(ensure_layout(it.R, er, sr, st, &jr.ly[1]));
+      jrs.push_back(jr);
+    }
+  }
+  // JRs of one filter variant are contiguous (one filter launch each).
+  std::stable_sort(jrs.begin(), jrs.end(),
+                   [](const JR &x, const JR &y) { return x.variant < y.variant; });
+  const uint32_t n_jr = (uint32_t)jrs.size();
+  std::vector<hb::JRDesc> hd(n_jr);
+  std::vector<uint32_t> jr_group_off(n_jr + 1, 0);
+  std::vector<hb::SimDesc> sims;
+  b->max_stage = 1;
+  for (uint32_t q = 0; q < n_jr; q++) {
+    const JR &jr = jrs[q];
+    Batch::Item &it = b->items[jr.job];
+    const PlanRule &rule = it.plan->rules[jr.rule];
+    hb::JRDesc &d = hd[q];
+    std::memset(&d, 0, sizeof(d));
+    hb_table *tab[2] = {it.L, it.R};
+    std::vector<int> extra[2];
     for (size_t k = 1; k < rule.eq.size(); k++) {
-      HB_TRY(ensure_code(L, rule.eq[k].first, st));
-      HB_TRY(ensure_code(R, rule.eq[k].second, st));
+      extra[0].push_back(rule.eq[k].first);
+      extra[1].push_back(rule.eq[k].second);
     }
+    for (int side = 0; side < 2; side++) {
+      Layout *ly = jr.ly[side];
+      hb::SideDesc &sd = d.s[side];
+      sd.perm = ly->d_perm;
+      sd.jsig = ly->d_jsig;
+      sd.run_code = ly->d_run_code;
+      sd.run_off = ly->d_run_off;
+      sd.n_runs = ly->n_runs;
+      sd.jcode = nullptr;
+      if (!extra[side].empty()) HB_TRY(ensure_jcode(tab[side], ly, extra[side], st, &sd.jcode));
+    }
+    if (!rule.sims.empty()) {
+      const double thr = (double)rule.sims[0].thr;
+      d.c0 = filter_coef(thr, 0);
+      d.c1 = filter_coef(thr, 1);
+      d.c2 = filter_coef(thr, 2);
+      d.c3 = filter_coef(thr, 3);
+    }
+    d.job = jr.job;
+    d.n_eq = rule.eq.empty() ? 0 : (int32_t)rule.eq.size() - 1;
+    d.n_sims = (int32_t)rule.sims.size();
+    d.sim_off = (uint32_t)sims.size();
+    d.multi = it.plan->rules.size() > 1 ? 1u : 0u;
     for (const auto &s : rule.sims) {
-      HB_TRY(ensure_sig(L, s.cl, st));
-      HB_TRY(ensure_sig(R, s.cr, st));
+      const Column &cl = it.L->cols[s.cl];
+      const Column &cr = it.R->cols[s.cr];
+      hb::SimDesc x{};
+      x.col_l = hb::DevColumn{cl.d_bytes, cl.d_len, cl.width, cl.d_planes, cl.nwc};
+      x.col_r = hb::DevColumn{cr.d_bytes, cr.d_len, cr.width, cr.d_planes, cr.nwc};
+      x.thr = s.thr;
+      x.slot = s.slot;
+      sims.push_back(x);
     }
-    const int el = rule.eq.empty() ? -1 : rule.eq[0].first;
-    const int er = rule.eq.empty() ? -1 : rule.eq[0].second;
-    const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
-    const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
-    const Layout *ll = nullptr, *lr = nullptr;
-    HB_TRY(ensure_layout(L, el, sl, st, &ll));
-    HB_TRY(ensure_layout(R, er, sr, st, &lr));
-    layouts.push_back({ll, lr});
+    b->max_stage = std::max(b->max_stage, (int)rule.sims.size());
+    jr_group_off[q + 1] = jr_group_off[q] + d.s[0].n_runs;
   }
+  const uint32_t n_groups = jr_group_off[n_jr];
 
-  // Job memory.
-  void *p = nullptr;
-  j->filter_counters.clear();
-  HB_TRY(job_alloc(j, (size_t)j->n_counters * sizeof(unsigned long long), &p));
-  j->d_cnt = (unsigned long long *)p;
-  uint32_t *list[2];
-  HB_TRY(job_alloc(j, j->list_cap * 8, &p));
-  list[0] = (uint32_t *)p;
-  HB_TRY(job_alloc(j, j->list_cap * 8, &p));
-  list[1] = (uint32_t *)p;
-  HB_TRY(job_alloc(j, j->list_cap * 8, &p));
-  uint32_t *d_defer = (uint32_t *)p;
-  void *hp = nullptr;
-  HB_TRY(job_alloc_host(j, j->out_cap * 8, &hp, &p));
-  j->h_out = (uint32_t *)hp;
-  j->d_out = (uint32_t *)p;
-  HB_TRY(job_alloc_host(j, j->audit_cap * sizeof(hb_audit_entry), &hp, &p));
-  j->h_audit = (hb_audit_entry *)hp;
-  j->d_audit = (hb_audit_entry *)p;
-  unsigned long long *d_set = nullptr;
-  const bool multi = plan->rules.size() > 1;
-  if (multi) {
-    HB_TRY(job_alloc(j, j->set_cap * 8, &p));
-    d_set = (unsigned long long *)p;
+  // Counter block: fixed slots, list counts per stage (0..max_stage), defer
+  // counts per stage, per-job counters, per-job fill counters, per-job
+  // output and audit offsets (scans).
+  const int n_stage_slots = b->max_stage + 1;
+  const uint32_t stride = b->stride();
+  b->cnt_stage_off = kCntFixed;
+  b->cnt_job_off = b->cnt_stage_off + 2 * n_stage_slots;
+  const int fill_off = b->cnt_job_off + (int)(hb::kJobCounters * stride);
+  const int afill_off = fill_off + (int)stride;
+  b->cnt_out_off = afill_off + (int)stride;
+  b->cnt_audit_off = b->cnt_out_off + (int)stride;
+  b->cnt_words = (size_t)b->cnt_audit_off + stride;
+
+  // Device memory.
+  void *p = nullptr, *hp = nullptr;
+  HB_TRY(batch_alloc(b, b->cnt_words * sizeof(unsigned long long), &p));
+  unsigned long long *d_cnt = (unsigned long long *)p;
+  HB_TRY(batch_alloc_host(b, b->cnt_words * sizeof(unsigned long long), &hp, nullptr));
+  b->h_cnt = (unsigned long long *)hp;
+  // descriptors: staged in pinned memory, copied on the batch stream
+  const size_t desc_bytes = n_jr * sizeof(hb::JRDesc) + (n_jr + 1) * sizeof(uint32_t) +
+                            std::max<size_t>(sims.size(), 1) * sizeof(hb::SimDesc) +
+                            2 * (size_t)n_jobs * sizeof(unsigned long long) + 128;
+  HB_TRY(batch_alloc_host(b, desc_bytes, &hp, nullptr));
+  HB_TRY(batch_alloc(b, desc_bytes, &p));
+  char *hdesc = (char *)hp, *ddesc = (char *)p;
+  size_t o = 0;
+  auto place = [&](const void *src, size_t bytes) {
+    o = (o + 15) & ~size_t(15);
+    std::memcpy(hdesc + o, src, bytes);
+    void *dst = ddesc + o;
+    o += bytes;
+    return dst;
+  };
+  hb::BatchDev bd{};
+  bd.jrs = (const hb::JRDesc *)place(hd.data(), n_jr * sizeof(hb::JRDesc));
+  bd.jr_group_off = (const uint32_t *)place(jr_group_off.data(), (n_jr + 1) * sizeof(uint32_t));
+  bd.n_jr = n_jr;
+  bd.cnt_stride = stride;
+  bd.sims = (const hb::SimDesc *)place(sims.data(), sims.size() * sizeof(hb::SimDesc));
+  bd.job_cnt = d_cnt + b->cnt_job_off;
+  bd.overflow = d_cnt + kCntOverflow;
+  // per-job output sets (jobs with several rules)
+  std::vector<unsigned long long> set_off(n_jobs, 0);
+  unsigned long long set_total = 0;
+  for (uint32_t j = 0; j < n_jobs; j++) {
+    set_off[j] = set_total;
+    set_total += b->set_cap[j];
   }
-  // Join scratch sized for the larger of the left tables' row counts.
-  const uint32_t cap = std::max<uint32_t>(n_l, 1);
-  uint32_t *flags, *run_id;
-  HB_TRY(job_alloc(j, (size_t)cap * 4, &p));
-  flags = (uint32_t *)p;
-  HB_TRY(job_alloc(j, (size_t)cap * 4, &p));
-  run_id = (uint32_t *)p;
-  // Extra equality codes in join order (the signatures come with the layout).
-  uint32_t *jcode_l, *jcode_r;
-  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_l, 1) * 4 * hb::kMaxEq, &p));
-  jcode_l = (uint32_t *)p;
-  HB_TRY(job_alloc(j, (size_t)std::max<uint32_t>(n_r, 1) * 4 * hb::kMaxEq, &p));
-  jcode_r = (uint32_t *)p;
-  hb::GroupArrays g;
-  g.cap = n_l;
-  g.tile_a = 256u * (uint32_t)j->shape.rows_per_lane;
-  g.tile_b = j->shape.tile_b;
+  bd.sets.slots = nullptr;
+  bd.sets.off = nullptr;
+  bd.sets.cap = nullptr;
+  if (set_total) {
+    HB_TRY(batch_alloc(b, set_total * sizeof(unsigned long long), &p));
+    bd.sets.slots = (unsigned long long *)p;
+    bd.sets.off = (const unsigned long long *)place(set_off.data(), n_jobs * sizeof(uint64_t));
+    bd.sets.cap = (const unsigned long long *)place(b->set_cap.data(), n_jobs * sizeof(uint64_t));
+  }
+  // groups
+  hb::GroupArrays g{};
+  g.n = n_groups;
+  g.tile_a = 256u * (uint32_t)b->shape.rows_per_lane;
+  g.tile_b = b->shape.tile_b;
   g.small_pairs_max = 16384;
-  HB_TRY(job_alloc(j, 4 * sizeof(uint32_t), &p));
-  g.n_runs = (uint32_t *)p;
   uint32_t *u32arr[4];
   for (auto &a : u32arr) {
-    HB_TRY(job_alloc(j, ((size_t)cap + 2) * 4, &p));
+    HB_TRY(batch_alloc(b, ((size_t)n_groups + 1) * 4, &p));
     a = (uint32_t *)p;
   }
   g.l_off = u32arr[0];
@@ -611,207 +747,182 @@ hb_status enqueue_job(Job *j) {
   g.r_cnt = u32arr[3];
   uint64_t *u64arr[4];
   for (auto &a : u64arr) {
-    HB_TRY(job_alloc(j, ((size_t)cap + 2) * 8, &p));
+    HB_TRY(batch_alloc(b, ((size_t)n_groups + 1) * 8, &p));
     a = (uint64_t *)p;
   }
   g.small_pairs = u64arr[0];
   g.pair_off = u64arr[1];
   g.items = u64arr[2];
   g.item_off = u64arr[3];
-  g.joined = j->d_cnt + kCntJoined;
-  size_t tb32 = 0, tb64 = 0;
-  HIP_OK(hb::exscan_u32(nullptr, tb32, flags, run_id, cap, st));
-  HIP_OK(hb::exscan_u64(nullptr, tb64, g.small_pairs, g.pair_off, std::max<uint32_t>(cap + 1, 2),
-                        st));
+  size_t tb64 = 0, tbj = 0;
+  HIP_OK(hb::exscan_u64(nullptr, tb64, g.small_pairs, g.pair_off, n_groups + 1, st));
+  HIP_OK(hb::exscan_u64(nullptr, tbj, g.small_pairs, g.pair_off, stride, st));
   void *tmp = nullptr;
-  const size_t tmp_bytes = std::max<size_t>(std::max(tb32, tb64), 16);
-  HB_TRY(job_alloc(j, tmp_bytes, &tmp));
-
-  j->n_marks = 0;
-  HIP_OK(hipEventRecord(j->ev_start, st));
-  hb::launch_fill_u64(j->d_cnt, (unsigned long long)j->n_counters, 0ull, st);
-  if (multi) hb::launch_fill_u64(d_set, j->set_cap, ~0ull, st);
-
-  hb::AuditOut audit{j->d_audit, j->d_cnt + kCntAudit, j->audit_cap};
-  hb::StageOut job_out{j->d_out, j->d_cnt + kCntOut, j->out_cap, j->d_cnt + kCntOverflow};
-  int stage = kCntStage0;
-
-  for (size_t ri = 0; ri < plan->rules.size(); ri++) {
-    const PlanRule &rule = plan->rules[ri];
-    const Layout &ll = *layouts[ri].first, &lr = *layouts[ri].second;
-    const uint32_t *perm_l = ll.d_perm, *perm_r = lr.d_perm;
-    hb::EqCheck eq{};
-    HB_TRY(mark(j, st));
-    if (!rule.eq.empty()) {
-      if (n_l) {
-        hb::launch_run_flags(ll.d_keys, n_l, flags, st);
-        size_t t = tmp_bytes;
-        HIP_OK(hb::exscan_u32(tmp, t, flags, run_id, n_l, st));
-      }
-      hb::launch_run_info(ll.d_keys, flags, run_id, n_l, lr.d_keys, n_r, g, st);
-      eq.n = (int32_t)rule.eq.size() - 1;
-      for (int k = 1; k < (int)rule.eq.size(); k++) {
-        eq.code_l[k - 1] = L->cols[rule.eq[k].first].d_code;
-        eq.code_r[k - 1] = R->cols[rule.eq[k].second].d_code;
-      }
-    } else {
-      hb::GroupArrays g1 = g;
-      g1.cap = 1;
-      hb::launch_single_group(n_l, n_r, g1, st);
-    }
-    hb::GroupArrays gu = g;
-    if (rule.eq.empty()) gu.cap = 1;
-    {
-      size_t t = tmp_bytes;
-      HIP_OK(hb::exscan_u64(tmp, t, gu.small_pairs, gu.pair_off, gu.cap + 1, st));
-      t = tmp_bytes;
-      HIP_OK(hb::exscan_u64(tmp, t, gu.items, gu.item_off, gu.cap + 1, st));
-    }
-    // Stage chain: stage 0 = equalities + bound filter, stage s >= 1 = exact
-    // JW of similarity s-1 on the previous stage's list.  Stages alternate
-    // between the two lists; the last stage of a single-rule plan writes the
-    // job output directly.
-    const int n_stages = 1 + (int)rule.sims.size();
-    float coef[4] = {0, 0, 0, 0};
-    if (!rule.sims.empty())
-      for (int q = 0; q < 4; q++) coef[q] = filter_coef((double)rule.sims[0].thr, q);
-    const hb::SigRow *jl = ll.d_jsig, *jr = lr.d_jsig;
-    if (eq.n > 0) {
-      hb::launch_gather_join(perm_l, n_l, nullptr, nullptr, eq, 0, jcode_l, st);
-      hb::launch_gather_join(perm_r, n_r, nullptr, nullptr, eq, 1, jcode_r, st);
-      HIP_OK(hipGetLastError());
-    }
-    hb::StageOut prev{};
-    HB_TRY(mark(j, st));
-    for (int s = 0; s < n_stages; s++) {
-      hb::StageOut o;
-      if (!multi && s == n_stages - 1) {
-        o = job_out;
-        if (s == 0) j->filter_counters.push_back(-1);
-      } else {
-        if (stage >= j->n_counters)
-          return fail(HB_ERR_INVALID_ARG, "internal: stage counters exhausted");
-        o = hb::StageOut{list[s & 1], j->d_cnt + stage, j->list_cap, j->d_cnt + kCntOverflow};
-        if (s == 0) j->filter_counters.push_back(stage);
-        stage++;
-      }
-      if (s == 0) {
-        hb::launch_filter_groups(gu, perm_l, perm_r, jl, jr, jcode_l, jcode_r, eq.n,
-                                 rule.sims.empty() ? 0 : 1, coef, o, j->shape, ctx->n_cu, st);
-      } else {
-        const auto &sim = rule.sims[s - 1];
-        const Column &cl = L->cols[sim.cl];
-        const Column &cr = R->cols[sim.cr];
-        hb::DevColumn dl{cl.d_bytes, cl.d_len, cl.width, cl.d_planes, cl.nwc};
-        hb::DevColumn dr{cr.d_bytes, cr.d_len, cr.width, cr.d_planes, cr.nwc};
-        if (stage >= j->n_counters)
-          return fail(HB_ERR_INVALID_ARG, "internal: stage counters exhausted");
-        hb::StageOut defer{d_defer, j->d_cnt + stage, j->list_cap, j->d_cnt + kCntOverflow};
-        stage++;
-        hb::launch_exact(prev.pairs, prev.count, prev.cap, dl, dr, sim.thr, sim.slot, o, audit,
-                         j->d_cnt + kCntJw, defer, ctx->n_cu, st);
-      }
-      HIP_OK(hipGetLastError());
-      prev = o;
-      if (s == 0) HB_TRY(mark(j, st));
-    }
-    HB_TRY(mark(j, st));
-    if (multi) {
-      hb::launch_merge_unique(prev.pairs, prev.count, prev.cap, d_set, j->set_cap, job_out,
-                              ctx->n_cu, st);
-      HIP_OK(hipGetLastError());
-    }
+  const size_t tmp_bytes = std::max<size_t>(std::max(tb64, tbj), 16);
+  HB_TRY(batch_alloc(b, tmp_bytes, &tmp));
+  // pair lists
+  hb::PairEntry *lists[2], *defer_buf, *final_buf;
+  for (auto &l : lists) {
+    HB_TRY(batch_alloc(b, b->list_cap * sizeof(hb::PairEntry), &p));
+    l = (hb::PairEntry *)p;
   }
-  HIP_OK(hipMemcpyAsync(j->h_cnt, j->d_cnt, (size_t)j->n_counters * sizeof(unsigned long long),
+  HB_TRY(batch_alloc(b, b->defer_cap * sizeof(hb::PairEntry), &p));
+  defer_buf = (hb::PairEntry *)p;
+  HB_TRY(batch_alloc(b, b->final_cap * sizeof(hb::PairEntry), &p));
+  final_buf = (hb::PairEntry *)p;
+  HB_TRY(batch_alloc(b, b->audit_cap * sizeof(hb::AuditRec), &p));
+  hb::AuditRec *audit_buf = (hb::AuditRec *)p;
+  // outputs in pinned memory, written by the scatter kernel
+  void *dout = nullptr, *daud = nullptr;
+  HB_TRY(batch_alloc_host(b, b->final_cap * 8, &hp, &dout));
+  b->h_out = (uint32_t *)hp;
+  HB_TRY(batch_alloc_host(b, std::max<unsigned long long>(b->audit_cap, 1) *
+                                 sizeof(hb_audit_entry), &hp, &daud));
+  b->h_audit = (hb_audit_entry *)hp;
+
+  // Launch sequence.
+  HIP_OK(hipEventRecord(b->ev_start, st));
+  HIP_OK(hipMemcpyAsync(ddesc, hdesc, o, hipMemcpyHostToDevice, st));
+  hb::launch_fill_u64(d_cnt, b->cnt_words, 0ull, st);
+  if (set_total) hb::launch_fill_u64(bd.sets.slots, set_total, ~0ull, st);
+  hb::launch_batch_join(bd, g, st);
+  {
+    size_t t = tmp_bytes;
+    HIP_OK(hb::exscan_u64(tmp, t, g.small_pairs, g.pair_off, n_groups + 1, st));
+    t = tmp_bytes;
+    HIP_OK(hb::exscan_u64(tmp, t, g.items, g.item_off, n_groups + 1, st));
+  }
+  HIP_OK(hipEventRecord(b->marks[0], st));
+  unsigned long long *list_cnt = d_cnt + b->cnt_stage_off;
+  unsigned long long *defer_cnt = list_cnt + n_stage_slots;
+  const hb::PairList list0{lists[0], list_cnt + 0, b->list_cap};
+  for (uint32_t q = 0; q < n_jr;) {
+    uint32_t e = q;
+    while (e < n_jr && jrs[e].variant == jrs[q].variant) e++;
+    const int variant = jrs[q].variant;
+    hb::launch_filter(bd, g, jr_group_off[q], jr_group_off[e], variant & 7, variant >= 8, list0,
+                      b->shape, ctx->n_cu, st);
+    q = e;
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(b->marks[1], st));
+  const hb::PairList fin{final_buf, d_cnt + kCntFinal, b->final_cap};
+  const hb::AuditList aud{audit_buf, d_cnt + kCntAudit, b->audit_cap};
+  for (int s = 1; s <= b->max_stage; s++) {
+    const hb::PairList in{lists[(s - 1) & 1], list_cnt + (s - 1), b->list_cap};
+    const hb::PairList next{lists[s & 1], list_cnt + s, b->list_cap};
+    const hb::PairList defer{defer_buf, defer_cnt + s, b->defer_cap};
+    hb::launch_exact(bd, in, s, next, fin, defer, aud, ctx->n_cu, st);
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(b->marks[2], st));
+  // per-job output ranges
+  uint64_t *out_off = (uint64_t *)(d_cnt + b->cnt_out_off);
+  uint64_t *audit_off = (uint64_t *)(d_cnt + b->cnt_audit_off);
+  {
+    size_t t = tmp_bytes;
+    HIP_OK(hb::exscan_u64(tmp, t, (const uint64_t *)(bd.job_cnt + (size_t)hb::kJobMatches * stride),
+                          out_off, stride, st));
+    t = tmp_bytes;
+    HIP_OK(hb::exscan_u64(tmp, t, (const uint64_t *)(bd.job_cnt + (size_t)hb::kJobAudit * stride),
+                          audit_off, stride, st));
+  }
+  hb::launch_scatter(bd, fin, out_off, d_cnt + fill_off, (uint32_t *)dout, b->final_cap, aud,
+                     audit_off, d_cnt + afill_off, (hb_audit_entry *)daud, b->audit_cap,
+                     ctx->n_cu, st);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(b->h_cnt, d_cnt, b->cnt_words * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, st));
-  HIP_OK(hipEventRecord(j->ev_end, st));
+  HIP_OK(hipEventRecord(b->ev_end, st));
   return HB_OK;
 }
 
-// Called with j->mu held once ev_end has completed.
-hb_status job_complete(Job *j) {
-  const unsigned long long *c = j->h_cnt;
+// Called with b->mu held once ev_end has completed.
+hb_status batch_complete(Batch *b) {
+  const unsigned long long *c = b->h_cnt;
   if (c[kCntOverflow]) {
-    unsigned long long need_list = 0;
-    for (int s = kCntStage0; s < j->n_counters; s++) need_list = std::max(need_list, c[s]);
+    const int n_stage_slots = b->max_stage + 1;
+    unsigned long long need_list = 0, need_defer = 0;
+    for (int s = 0; s < n_stage_slots; s++) {
+      need_list = std::max(need_list, c[b->cnt_stage_off + s]);
+      need_defer = std::max(need_defer, c[b->cnt_stage_off + n_stage_slots + s]);
+    }
     auto grow = [](unsigned long long cur, unsigned long long need) {
-      unsigned long long n = cur;
+      unsigned long long n = std::max<unsigned long long>(cur, 1);
       while (n < need + need / 4 + 1024) n *= 2;
       return n;
     };
-    j->list_cap = grow(j->list_cap, need_list);
-    j->out_cap = grow(j->out_cap, c[kCntOut]);
-    j->audit_cap = grow(j->audit_cap, c[kCntAudit]);
-    unsigned long long sc = 1024;
-    while (sc < 2 * j->out_cap) sc <<= 1;
-    j->set_cap = sc;
-    j->retries++;
-    if (j->retries > 8) return fail(HB_ERR_CAPACITY, "job %llu: output keeps overflowing",
-                                    (unsigned long long)j->id);
-    HIP_OK(hipStreamSynchronize(j->stream));
-    job_free_device(j);
-    HB_TRY(enqueue_job(j));
+    b->list_cap = grow(b->list_cap, need_list);
+    b->defer_cap = grow(b->defer_cap, need_defer);
+    // a list that overflowed hid pairs from the later stages: size the final
+    // list for the largest list seen as well
+    b->final_cap = grow(b->final_cap, std::max(c[kCntFinal], need_list));
+    b->audit_cap = grow(b->audit_cap, c[kCntAudit]);
+    for (uint32_t j = 0; j < b->n_jobs(); j++)
+      if (b->set_cap[j])
+        b->set_cap[j] = std::max(2 * b->set_cap[j],
+                                 pow2_at_least(4 * b->job_counter(hb::kJobMatches, j)));
+    b->retries++;
+    if (b->retries > 8) return fail(HB_ERR_CAPACITY, "batch: output keeps overflowing");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    batch_free_memory(b);
+    HB_TRY(enqueue_batch(b));
     return HB_NOT_READY;
   }
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, j->ev_start, j->ev_end);
-  hb_job_stats &s = j->stats;
-  s.pairs = (unsigned long long)j->L->n_rows * (unsigned long long)j->R->n_rows;
-  s.candidates = c[kCntJoined];
-  unsigned long long filt = 0;
-  for (size_t k = 0; k < j->filter_counters.size(); k++) {
-    // stage 0 of a rule with similarity predicates always writes a list
-    if (j->filter_counters[k] >= 0 && !j->plan->rules[k].sims.empty())
-      filt += c[j->filter_counters[k]];
-  }
-  s.filtered = filt;
-  s.jw_exact = c[kCntJw];
-  s.matches = c[kCntOut];
-  s.audit = c[kCntAudit];
-  s.device_ms = ms;
-  s.join_ms = s.filter_ms = s.exact_ms = 0;
-  for (size_t k = 0; k + 4 <= j->n_marks; k += 4) {  // 4 marks per rule
-    float a = 0, b = 0, c2 = 0;
-    (void)hipEventElapsedTime(&a, j->marks[k], j->marks[k + 1]);
-    (void)hipEventElapsedTime(&b, j->marks[k + 1], j->marks[k + 2]);
-    (void)hipEventElapsedTime(&c2, j->marks[k + 2], j->marks[k + 3]);
-    s.join_ms += a;
-    s.filter_ms += b;
-    s.exact_ms += c2;
-  }
-  s.retries = j->retries;
-  s.n_rules = (int32_t)j->plan->rules.size();
-  j->state = JobState::kDone;
+  (void)hipEventElapsedTime(&b->device_ms, b->ev_start, b->ev_end);
+  (void)hipEventElapsedTime(&b->join_ms, b->ev_start, b->marks[0]);
+  (void)hipEventElapsedTime(&b->filter_ms, b->marks[0], b->marks[1]);
+  (void)hipEventElapsedTime(&b->exact_ms, b->marks[1], b->marks[2]);
+  b->state = BatchState::kDone;
   return HB_OK;
 }
 
-hb_status job_poll(Job *j, bool block) {
-  std::lock_guard<std::mutex> lk(j->mu);
+hb_status batch_poll(Batch *b, bool block) {
+  std::lock_guard<std::mutex> lk(b->mu);
   for (;;) {
-    if (j->state == JobState::kDone) return HB_OK;
-    if (j->state == JobState::kFailed)
-      return fail(j->error, "%s", j->error_text.c_str());
-    HB_TRY(set_device(j->ctx));
-    hipError_t e = block ? hipEventSynchronize(j->ev_end) : hipEventQuery(j->ev_end);
+    if (b->state == BatchState::kDone) return HB_OK;
+    if (b->state == BatchState::kFailed) return fail(b->error, "%s", b->error_text.c_str());
+    HB_TRY(set_device(b->ctx));
+    hipError_t e = block ? hipEventSynchronize(b->ev_end) : hipEventQuery(b->ev_end);
     if (e == hipErrorNotReady) return HB_NOT_READY;
     if (e != hipSuccess) {
-      j->state = JobState::kFailed;
-      j->error = HB_ERR_HIP;
-      j->error_text = std::string("job failed on device: ") + hipGetErrorString(e);
-      return fail(j->error, "%s", j->error_text.c_str());
+      b->state = BatchState::kFailed;
+      b->error = HB_ERR_HIP;
+      b->error_text = std::string("batch failed on device: ") + hipGetErrorString(e);
+      return fail(b->error, "%s", b->error_text.c_str());
     }
-    hb_status st = job_complete(j);
+    hb_status st = batch_complete(b);
     if (st == HB_NOT_READY) {
       if (!block) return HB_NOT_READY;
       continue;
     }
     if (st != HB_OK) {
-      j->state = JobState::kFailed;
-      j->error = st;
-      j->error_text = g_last_error;
+      b->state = BatchState::kFailed;
+      b->error = st;
+      b->error_text = g_last_error;
       return st;
     }
   }
+}
+
+void fill_stats(Job *j) {
+  if (j->stats_done) return;
+  Batch *b = j->batch.get();
+  const Batch::Item &it = b->items[j->index];
+  hb_job_stats &s = j->stats;
+  s.pairs = (unsigned long long)it.L->n_rows * (unsigned long long)it.R->n_rows;
+  s.candidates = b->job_counter(hb::kJobJoined, j->index);
+  s.filtered = b->job_counter(hb::kJobFiltered, j->index);
+  s.jw_exact = b->job_counter(hb::kJobJw, j->index);
+  s.matches = b->job_counter(hb::kJobMatches, j->index);
+  s.audit = b->job_counter(hb::kJobAudit, j->index);
+  s.device_ms = b->device_ms;
+  s.join_ms = b->join_ms;
+  s.filter_ms = b->filter_ms;
+  s.exact_ms = b->exact_ms;
+  s.retries = b->retries;
+  s.n_rules = (int32_t)it.plan->rules.size();
+  j->stats_done = true;
 }
 
 Job *find_job(hb_ctx *ctx, uint64_t id) {
@@ -819,6 +930,89 @@ Job *find_job(hb_ctx *ctx, uint64_t id) {
   auto it = ctx->jobs.find(id);
   return it == ctx->jobs.end() ? nullptr : it->second.get();
 }
+
+hb_status check_job_args(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, const hb_table *r) {
+  if (!ctx || !plan || !l || !r) return fail(HB_ERR_INVALID_ARG, "job submit: null argument");
+  if (plan->max_col_l >= l->n_cols || plan->max_col_r >= r->n_cols)
+    return fail(HB_ERR_PLAN, "plan references column %d/%d but tables have %d/%d columns",
+                plan->max_col_l, plan->max_col_r, l->n_cols, r->n_cols);
+  if (l->ctx != ctx || r->ctx != ctx)
+    return fail(HB_ERR_INVALID_ARG, "job submit: table from another context");
+  return HB_OK;
+}
+
+// Creates the jobs of one batch and enqueues it.
+hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
+                       const hb_table *const *ls, const hb_table *const *rs,
+                       const uint64_t *ids) {
+  if (!ctx || n == 0 || !plans || !ls || !rs || !ids)
+    return fail(HB_ERR_INVALID_ARG, "batch submit: no jobs or null arrays");
+  for (uint32_t j = 0; j < n; j++) HB_TRY(check_job_args(ctx, plans[j], ls[j], rs[j]));
+  HB_TRY(set_device(ctx));
+  auto b = std::make_shared<Batch>();
+  b->ctx = ctx;
+  b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
+  unsigned long long rows = 0;
+  b->set_cap.assign(n, 0);
+  for (uint32_t j = 0; j < n; j++) {
+    Batch::Item it{plans[j], const_cast<hb_table *>(ls[j]), const_cast<hb_table *>(rs[j])};
+    b->items.push_back(it);
+    const unsigned long long jr = (unsigned long long)(it.L->n_rows + it.R->n_rows);
+    rows += jr;
+    if (plans[j]->rules.size() > 1) b->set_cap[j] = pow2_at_least(2 * jr);
+  }
+  b->list_cap = std::max<unsigned long long>(1ull << 17, rows);
+  b->defer_cap = std::max<unsigned long long>(1ull << 12, rows / 64);
+  b->final_cap = std::max<unsigned long long>(1ull << 16, rows / 2);
+  b->audit_cap = std::max<unsigned long long>(1ull << 12, rows / 64);
+  HB_TRY(ctx->events.get(&b->ev_start));
+  HB_TRY(ctx->events.get(&b->ev_end));
+  for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));
+  std::vector<std::unique_ptr<Job>> jobs;
+  for (uint32_t j = 0; j < n; j++) {
+    auto job = std::make_unique<Job>();
+    job->id = ids[j];
+    job->ctx = ctx;
+    job->batch = b;
+    job->index = j;
+    jobs.push_back(std::move(job));
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (uint32_t j = 0; j < n; j++) {
+      if (ctx->jobs.count(ids[j]))
+        return fail(HB_ERR_INVALID_ARG, "job id %llu already in use", (unsigned long long)ids[j]);
+      for (uint32_t k = 0; k < j; k++)
+        if (ids[k] == ids[j])
+          return fail(HB_ERR_INVALID_ARG, "job id %llu repeated in the batch",
+                      (unsigned long long)ids[j]);
+    }
+    b->shape = ctx->shape;
+    for (auto &job : jobs) ctx->jobs.emplace(job->id, std::move(job));
+  }
+  std::lock_guard<std::mutex> lk(b->mu);
+  hb_status st = enqueue_batch(b.get());
+  if (st != HB_OK) {
+    b->state = BatchState::kFailed;
+    b->error = st;
+    b->error_text = g_last_error;
+  }
+  return st;
+}
+
+}  // namespace
+
+Batch::~Batch() {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (stream) (void)hipStreamSynchronize(stream);
+  batch_free_memory(this);
+  ctx->events.put(ev_start);
+  ctx->events.put(ev_end);
+  for (hipEvent_t m : marks) ctx->events.put(m);
+}
+
+namespace {
 
 hb_status check_cell_bytes(const hb_table_view *v) {
   if (!v || v->n_rows < 0 || v->n_cols <= 0 || !v->col_base || !v->col_stride)
@@ -838,7 +1032,7 @@ extern "C" {
 
 const char *hb_last_error(void) { return g_last_error.c_str(); }
 
-const char *hb_version(void) { return "hyperblocker_amd 0.1 gfx950"; }
+const char *hb_version(void) { return "hyperblocker_amd 0.2 gfx950"; }
 
 hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
   if (!out) return fail(HB_ERR_INVALID_ARG, "hb_ctx_create: out is null");
@@ -904,8 +1098,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   (void)hipDeviceSynchronize();
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    for (auto &kv : ctx->jobs) job_free_all(kv.second.get());
-    ctx->jobs.clear();
+    ctx->jobs.clear();  // each batch returns its memory to the pools with its last job
   }
   ctx->pool.release_all();
   ctx->hpool.release_all();
@@ -1020,10 +1213,14 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
       if (lz->ev) (void)hipEventDestroy(lz->ev);
   }
   for (auto &kv : t->layouts) {
-    (void)hipFree(kv.second.d_keys);
-    (void)hipFree(kv.second.d_perm);
-    (void)hipFree(kv.second.d_jsig);
-    if (kv.second.lz.ev) (void)hipEventDestroy(kv.second.lz.ev);
+    Layout &ly = kv.second;
+    (void)hipFree(ly.d_keys);
+    (void)hipFree(ly.d_perm);
+    (void)hipFree(ly.d_jsig);
+    (void)hipFree(ly.d_run_code);
+    (void)hipFree(ly.d_run_off);
+    for (auto &jc : ly.jcode) (void)hipFree(jc.second);
+    if (ly.lz.ev) (void)hipEventDestroy(ly.lz.ev);
   }
   if (t->ready) (void)hipEventDestroy(t->ready);
   delete t;
@@ -1091,87 +1288,52 @@ hb_status hb_plan_release(hb_ctx *ctx, hb_plan *plan) {
 
 hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, const hb_table *r,
                         uint64_t job_id) {
-  if (!ctx || !plan || !l || !r) return fail(HB_ERR_INVALID_ARG, "hb_job_submit: null argument");
-  if (plan->max_col_l >= l->n_cols || plan->max_col_r >= r->n_cols)
-    return fail(HB_ERR_PLAN, "plan references column %d/%d but tables have %d/%d columns",
-                plan->max_col_l, plan->max_col_r, l->n_cols, r->n_cols);
-  if (l->ctx != ctx || r->ctx != ctx)
-    return fail(HB_ERR_INVALID_ARG, "hb_job_submit: table from another context");
-  HB_TRY(set_device(ctx));
-  auto job = std::make_unique<Job>();
-  Job *j = job.get();
-  j->id = job_id;
-  j->ctx = ctx;
-  j->plan = plan;
-  j->L = const_cast<hb_table *>(l);
-  j->R = const_cast<hb_table *>(r);
-  j->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
-  const unsigned long long base =
-      std::max<unsigned long long>(1 << 16, (unsigned long long)(l->n_rows + r->n_rows) * 2);
-  j->list_cap = base * 4;
-  j->out_cap = base;
-  j->audit_cap = 1 << 14;
-  unsigned long long sc = 1024;
-  while (sc < 2 * j->out_cap) sc <<= 1;
-  j->set_cap = sc;
-  j->n_counters = kCntStage0;
-  for (const PlanRule &rule : plan->rules) j->n_counters += 1 + 2 * (int)rule.sims.size();
-  {
-    void *hc = nullptr;
-    HB_TRY(ctx->hpool.alloc((size_t)j->n_counters * sizeof(unsigned long long), &hc, nullptr));
-    j->h_cnt = (unsigned long long *)hc;
-  }
-  HB_TRY(ctx->events.get(&j->ev_start));
-  HB_TRY(ctx->events.get(&j->ev_end));
-  {
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    if (ctx->jobs.count(job_id))
-      return fail(HB_ERR_INVALID_ARG, "job id %llu already in use", (unsigned long long)job_id);
-    j->shape = ctx->shape;
-    ctx->jobs.emplace(job_id, std::move(job));
-  }
-  std::lock_guard<std::mutex> lk(j->mu);
-  hb_status st = enqueue_job(j);
-  if (st != HB_OK) {
-    j->state = JobState::kFailed;
-    j->error = st;
-    j->error_text = g_last_error;
-  }
-  return st;
+  return submit_batch(ctx, 1, &plan, &l, &r, &job_id);
+}
+
+hb_status hb_batch_submit(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
+                          const hb_table *const *l, const hb_table *const *r,
+                          const uint64_t *job_ids) {
+  return submit_batch(ctx, n_jobs, plans, l, r, job_ids);
 }
 
 hb_status hb_job_query(hb_ctx *ctx, uint64_t job_id) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  return job_poll(j, false);
+  return batch_poll(j->batch.get(), false);
 }
 
 hb_status hb_job_wait(hb_ctx *ctx, uint64_t job_id) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  return job_poll(j, true);
+  return batch_poll(j->batch.get(), true);
 }
 
 hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs, uint64_t *n_pairs,
                          const hb_audit_entry **audit, uint64_t *n_audit) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  HB_TRY(job_poll(j, true));
-  std::lock_guard<std::mutex> lk(j->mu);
-  // The kernels wrote the outputs into pinned memory; the job's completion
-  // event orders those writes before this read.
-  if (pairs) *pairs = j->h_out;
-  if (n_pairs) *n_pairs = j->h_cnt[kCntOut];
-  if (audit) *audit = j->h_audit;
-  if (n_audit) *n_audit = j->h_cnt[kCntAudit];
+  Batch *b = j->batch.get();
+  HB_TRY(batch_poll(b, true));
+  std::lock_guard<std::mutex> lk(b->mu);
+  // The scatter kernel wrote the outputs into pinned memory; the batch's
+  // completion event orders those writes before this read.
+  const unsigned long long off = b->h_cnt[b->cnt_out_off + j->index];
+  const unsigned long long aoff = b->h_cnt[b->cnt_audit_off + j->index];
+  if (pairs) *pairs = b->h_out + 2 * off;
+  if (n_pairs) *n_pairs = b->job_counter(hb::kJobMatches, j->index);
+  if (audit) *audit = b->h_audit + aoff;
+  if (n_audit) *n_audit = b->job_counter(hb::kJobAudit, j->index);
   return HB_OK;
 }
 
 hb_status hb_job_stats_get(hb_ctx *ctx, uint64_t job_id, hb_job_stats *out) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  HB_TRY(job_poll(j, true));
-  std::lock_guard<std::mutex> lk(j->mu);
+  Batch *b = j->batch.get();
+  HB_TRY(batch_poll(b, true));
+  std::lock_guard<std::mutex> lk(b->mu);
+  fill_stats(j);
   if (out) *out = j->stats;
   return HB_OK;
 }
@@ -1188,8 +1350,7 @@ hb_status hb_job_release(hb_ctx *ctx, uint64_t job_id) {
     ctx->jobs.erase(it);
   }
   HB_TRY(set_device(ctx));
-  HIP_OK(hipStreamSynchronize(job->stream));
-  job_free_all(job.get());
+  job.reset();  // the batch's memory returns to the pools with its last job
   return HB_OK;
 }
 

@@ -10,13 +10,6 @@
 
 namespace hb {
 
-hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *keys_in,
-                          uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
-                          uint32_t n, hipStream_t stream) {
-  return rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out,
-                                   (size_t)n, 0, 32, stream);
-}
-
 hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const uint64_t *keys_in,
                           uint64_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
                           uint32_t n, int end_bit, hipStream_t stream) {

@@ -136,7 +136,7 @@ hb_status hb_ctx_get_filter_shape(const hb_ctx *ctx, int32_t *rows_per_lane, int
                                   int32_t *groups_per_cu);
 /* Text of the last error on this thread (never NULL). */
 const char *hb_last_error(void);
-/* Library build id, e.g. "hyperblocker_amd 0.1 gfx950". */
+/* Library build id, e.g. "hyperblocker_amd 0.2 gfx950". */
 const char *hb_version(void);
 
 /* ---- tables --------------------------------------------------------------
@@ -171,6 +171,15 @@ hb_status hb_plan_release(hb_ctx *ctx, hb_plan *plan);
  * until hb_job_release. */
 hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
                         const hb_table *r, uint64_t job_id);
+/* Batched form: n_jobs jobs (plans[i], l[i], r[i], id job_ids[i]) run as one
+ * batch, i.e. one sequence of kernel launches covering all of their rules
+ * (no counterpart in the reference, which launches one kernel per
+ * partition, host_producer.h:195-197).  Each job is then queried, read and
+ * released by its id like a job from hb_job_submit; results equal those of
+ * the same job submitted alone.  All jobs of a batch complete together. */
+hb_status hb_batch_submit(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
+                          const hb_table *const *l, const hb_table *const *r,
+                          const uint64_t *job_ids);
 hb_status hb_job_query(hb_ctx *ctx, uint64_t job_id); /* HB_OK or HB_NOT_READY */
 hb_status hb_job_wait(hb_ctx *ctx, uint64_t job_id);
 /* (l_row, r_row) pairs as 2*n_pairs uint32, each matching pair exactly once,

@@ -1,0 +1,129 @@
+"""Batched jobs on the HIP path (needs an MI355X).
+
+A batch runs many blocking jobs of different table sizes and plans as one
+sequence of kernel launches.  Each job's result must equal the same job run
+alone and the CPU oracle, pair for pair and audit entry for audit entry.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import hyperblocker_amd as hb
+from hyperblocker_amd import datagen
+from tests import helpers as H
+from tests.test_gpu_parity import PLANS, _adversarial_tables
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hb.Context(0)
+    yield c
+    c.close()
+
+
+def _results(job):
+    pairs, audit, stats = job.results()
+    job.release()
+    return hb.sort_pairs(pairs), H.sort_audit(audit), stats
+
+
+def _jobs(ctx, n, seed):
+    """n jobs with tables of 1..900 rows a side and assorted plans."""
+    rng = random.Random(seed)
+    names = sorted(PLANS)
+    out = []
+    for k in range(n):
+        n_l = rng.choice([0, 1, 7, 60, 300, 900])
+        n_r = rng.choice([1, 5, 80, 400, 700])
+        alpha = rng.choice([b"ab", b"abcd", b"abcdefgh"])
+        L, R = _adversarial_tables(1000 + k, max(n_l, 1), n_r, alpha, rng.choice([8, 30, 60]))
+        if n_l == 0:
+            L = [c[:0] for c in L]
+        name = names[k % len(names)]
+        out.append((name, L, R))
+    return out
+
+
+def test_batch_equals_alone_and_oracle(ctx):
+    specs = _jobs(ctx, 30, 5)
+    plans = {n: ctx.plan(hb.HostPlan.from_yaml(PLANS[n])) for n in PLANS}
+    tables = [(ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))) for _, L, R in specs]
+    try:
+        alone = [_results(ctx.submit(plans[n], tl, tr)) for (n, _, _), (tl, tr) in zip(specs, tables)]
+        batch = ctx.submit_batch([(plans[n], tl, tr) for (n, _, _), (tl, tr) in zip(specs, tables)])
+        together = [_results(j) for j in batch]
+        for k, ((name, L, R), a, t) in enumerate(zip(specs, alone, together)):
+            what = "job %d (%s, %d x %d)" % (k, name, len(L[0]), len(R[0]))
+            H.assert_same_pairs(t[0], a[0], what + " batch vs alone")
+            H.assert_same_audit(t[1], a[1], what + " batch vs alone")
+            assert t[2]["pairs"] == a[2]["pairs"] == len(L[0]) * len(R[0])
+            assert t[2]["matches"] == len(a[0])
+            if k % 3 == 0:
+                plan = hb.HostPlan.from_yaml(PLANS[name])
+                want, want_audit, _ = H.run_oracle(plan, L, R)
+                H.assert_same_pairs(t[0], want, what + " vs oracle")
+                H.assert_same_audit(t[1], want_audit, what + " vs oracle")
+    finally:
+        for tl, tr in tables:
+            tl.release()
+            tr.release()
+        for p in plans.values():
+            p.release()
+
+
+def test_batch_sharing_a_resident_right_table(ctx):
+    """Many left jobs against one resident right table (the served shape),
+    with multi-rule plans: per-job results equal the oracle's."""
+    L_all, R = datagen.make("c1")[:2]
+    rules = datagen.RULES_C3[:1] + [datagen.rule_yaml([3], [(1, 0.8)])]
+    plan = hb.HostPlan.from_yaml(rules)
+    dp = ctx.plan(plan)
+    tr = ctx.upload(hb.ColumnTable(R))
+    rng = random.Random(3)
+    lefts = []
+    for k in range(24):
+        rows = np.array(sorted(rng.sample(range(2600), rng.choice([1, 13, 200, 700]))))
+        lefts.append([c[rows] for c in L_all])
+    tls = [ctx.upload(hb.ColumnTable(L)) for L in lefts]
+    try:
+        jobs = ctx.submit_batch([(dp, tl, tr) for tl in tls])
+        for k, (job, L) in enumerate(zip(jobs, lefts)):
+            got, audit, stats = _results(job)
+            want, want_audit, _ = H.run_oracle(plan, L, R)
+            H.assert_same_pairs(got, want, "left job %d" % k)
+            H.assert_same_audit(audit, want_audit, "left job %d" % k)
+    finally:
+        for tl in tls:
+            tl.release()
+        tr.release()
+        dp.release()
+
+
+def test_batch_regrowth_keeps_every_job_exact(ctx):
+    """One job of the batch emits 2.1M pairs (far beyond the first buffers);
+    the batch regrows and every job still matches its lone result."""
+    rows_l = [(str(i).encode(), b"x", b"y", b"k", b"k") for i in range(1500)]
+    rows_r = [(str(i).encode(), b"x", b"y", b"k", b"k") for i in range(1400)]
+    big = (H.columns_from_rows(rows_l), H.columns_from_rows(rows_r))
+    small = [_adversarial_tables(50 + k, 100, 90, b"abc", 12) for k in range(5)]
+    p_eq = ctx.plan(hb.HostPlan.from_yaml([datagen.rule_yaml([3], [])]))
+    p_sim = ctx.plan(hb.HostPlan.from_yaml(PLANS["two_eq_two_sim"]))
+    specs = [(p_eq, big)] + [(p_sim, s) for s in small]
+    tables = [(ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))) for _, (L, R) in specs]
+    try:
+        jobs = ctx.submit_batch([(p, tl, tr) for (p, _), (tl, tr) in zip(specs, tables)])
+        res = [_results(j) for j in jobs]
+        assert len(res[0][0]) == 1500 * 1400 and res[0][2]["retries"] >= 1
+        for (p, _), (tl, tr), r in zip(specs[1:], tables[1:], res[1:]):
+            alone = _results(ctx.submit(p, tl, tr))
+            H.assert_same_pairs(r[0], alone[0], "small job")
+            H.assert_same_audit(r[1], alone[1], "small job")
+    finally:
+        for tl, tr in tables:
+            tl.release()
+            tr.release()
+        p_eq.release()
+        p_sim.release()
