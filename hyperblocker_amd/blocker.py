@@ -237,6 +237,26 @@ class Context:
         check(lib().hb_batch_submit(self._h, n, plans, ls, rs, idarr))
         return [Job(self, i) for i in ids]
 
+    def autotune(self, jobs, candidates=None):
+        """Picks the bound filter's launch shape for these jobs (list of
+        (plan, l_table, r_table)) and makes it the context's; returns
+        (shape, filter ms)."""
+        n = len(jobs)
+        plans = (C.c_void_p * n)(*[p._h.value for p, _, _ in jobs])
+        ls = (C.c_void_p * n)(*[l._h.value for _, l, _ in jobs])
+        rs = (C.c_void_p * n)(*[r._h.value for _, _, r in jobs])
+        cand = None
+        nc = 0
+        if candidates:
+            flat = [int(x) for c in candidates for x in c]
+            cand = (C.c_int32 * len(flat))(*flat)
+            nc = len(candidates)
+        best = (C.c_int32 * 3)()
+        ms = C.c_double()
+        check(lib().hb_autotune_filter_shape(self._h, n, plans, ls, rs, cand, nc, best,
+                                             C.byref(ms)))
+        return (best[0], best[1], best[2]), ms.value
+
     def block(self, plan, l_table, r_table):
         """Synchronous blocking of l_table x r_table: (pairs uint32[n,2] in no
         particular order, audit structured array, stats dict)."""
@@ -247,10 +267,47 @@ class Context:
             job.release()
 
 
+class Server:
+    """Serving loop on a context (hb_server_*): jobs are queued and admitted
+    into batches as batch slots and working memory free up."""
+
+    def __init__(self, ctx, max_batches_in_flight=2, max_jobs_per_batch=256, memory_budget=0):
+        self.ctx = ctx
+        cfg = capi.ServerConfig(max_batches_in_flight, max_jobs_per_batch, memory_budget)
+        h = C.c_void_p()
+        check(lib().hb_server_create(ctx._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def submit(self, plan, l_table, r_table, job_id=None):
+        jid = next(_job_ids) if job_id is None else job_id
+        check(lib().hb_server_submit(self._h, plan._h, l_table._h, r_table._h, jid))
+        return Job(self.ctx, jid)
+
+    def info(self):
+        v = [C.c_uint64() for _ in range(3)]
+        check(lib().hb_server_info(self._h, *[C.byref(x) for x in v]))
+        return {"batches": v[0].value, "jobs": v[1].value, "queued": v[2].value}
+
+    def close(self):
+        if self._h and self._h.value:
+            check(lib().hb_server_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class DeviceTable:
-    def __init__(self, ctx, table, rows=None):
+    def __init__(self, ctx, table, rows=None, _handle=None, _n_rows=None):
         self.ctx = ctx
         self.host = table  # keeps the source buffers alive for the caller
+        if _handle is not None:
+            self._h = _handle
+            self.n_rows = _n_rows
+            return
         h = C.c_void_p()
         if rows is None:
             check(lib().hb_table_upload(ctx._h, C.byref(table.view), C.byref(h)))
@@ -262,6 +319,12 @@ class DeviceTable:
                                              len(rows), C.byref(h)))
             self.n_rows = len(rows)
         self._h = h
+
+    def slice(self, begin, end):
+        """Rows [begin, end) as a table sharing this one's device data."""
+        h = C.c_void_p()
+        check(lib().hb_table_slice(self.ctx._h, self._h, int(begin), int(end), C.byref(h)))
+        return DeviceTable(self.ctx, self.host, _handle=h, _n_rows=int(end) - int(begin))
 
     def release(self):
         if self._h and self._h.value and self.ctx._h:

@@ -29,10 +29,11 @@ EXPORTS = [
     "hb_ctx_create", "hb_ctx_destroy", "hb_ctx_synchronize", "hb_ctx_set_filter_shape",
     "hb_ctx_get_filter_shape", "hb_last_error", "hb_version",
     "hb_table_upload", "hb_table_upload_serialized", "hb_table_upload_rows",
-    "hb_table_info", "hb_table_release",
+    "hb_table_slice", "hb_table_info", "hb_table_release",
     "hb_plan_create", "hb_plan_release",
     "hb_job_submit", "hb_batch_submit", "hb_job_query", "hb_job_wait", "hb_job_results",
-    "hb_job_stats_get", "hb_job_release", "hb_block", "hb_free",
+    "hb_job_stats_get", "hb_job_release", "hb_block", "hb_free", "hb_autotune_filter_shape",
+    "hb_server_create", "hb_server_submit", "hb_server_info", "hb_server_destroy",
     "hb_rules_load", "hb_rules_parse", "hb_host_plan_view", "hb_host_plan_free",
     "hb_csv_load", "hb_host_table_view", "hb_host_table_cell", "hb_host_table_free",
     "hb_partition", "hb_partitions_count", "hb_partitions_get", "hb_partitions_free",
@@ -72,11 +73,17 @@ AUDIT_DTYPE = np.dtype([("l", "<u4"), ("r", "<u4"), ("slot", "<i4"), ("flags", "
                         ("jw", "<f8")])
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("max_batches_in_flight", C.c_int32), ("max_jobs_per_batch", C.c_int32),
+                ("memory_budget", C.c_uint64)]
+
+
 class JobStats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("candidates", C.c_uint64), ("filtered", C.c_uint64),
                 ("jw_exact", C.c_uint64), ("matches", C.c_uint64), ("audit", C.c_uint64),
                 ("device_ms", C.c_double), ("join_ms", C.c_double), ("filter_ms", C.c_double),
-                ("exact_ms", C.c_double), ("retries", C.c_int32), ("n_rules", C.c_int32)]
+                ("exact_ms", C.c_double), ("retries", C.c_int32), ("n_rules", C.c_int32),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,6 +115,7 @@ def lib():
         "hb_table_upload_serialized": ([VP, C.POINTER(SerializedTable), PVP], st),
         "hb_table_upload_rows": ([VP, C.POINTER(TableView), C.POINTER(C.c_int64), C.c_int64,
                                   PVP], st),
+        "hb_table_slice": ([VP, VP, C.c_int64, C.c_int64, PVP], st),
         "hb_table_info": ([VP, C.POINTER(C.c_int64), C.POINTER(C.c_int32)], st),
         "hb_table_release": ([VP, VP], st),
         "hb_plan_create": ([VP, C.POINTER(SerializedPlan), PVP], st),
@@ -125,6 +133,13 @@ def lib():
         "hb_block": ([VP, VP, VP, VP, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint64),
                       C.POINTER(C.POINTER(AuditEntry)), C.POINTER(C.c_uint64),
                       C.POINTER(JobStats)], st),
+        "hb_autotune_filter_shape": ([VP, C.c_uint32, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
+                                      C.POINTER(C.c_int32), C.c_uint32, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_double)], st),
+        "hb_server_create": ([VP, C.POINTER(ServerConfig), PVP], st),
+        "hb_server_submit": ([VP, VP, VP, VP, C.c_uint64], st),
+        "hb_server_info": ([VP] + [C.POINTER(C.c_uint64)] * 3, st),
+        "hb_server_destroy": ([VP], st),
         "hb_free": ([VP], None),
         "hb_rules_load": ([C.c_char_p, PVP], st),
         "hb_rules_parse": ([C.c_int32, C.POINTER(C.c_char_p), PVP], st),

@@ -20,8 +20,12 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <thread>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -241,6 +245,7 @@ struct hb_ctx {
   int n_cu = 256;
   hb::FilterShape shape{2, 256, 16};  // filter launch shape; HB_FILTER_SHAPE overrides
   hipStream_t copy_stream = nullptr;
+  hipStream_t build_stream = nullptr;  // one-off per-table builds that the host waits for
   std::vector<hipStream_t> streams;
   std::atomic<uint64_t> next_stream{0};
   DevicePool pool;
@@ -301,6 +306,11 @@ struct hb_table {
   std::map<std::pair<int, int>, Layout> layouts;  // (equality column, similarity column)
   hipEvent_t ready = nullptr;
   std::mutex mu;  // guards lazy builds
+  // A slice borrows rows [row_off, row_off + n_rows) of its parent's column
+  // data (bytes, lengths, signatures, planes, codes); its layouts are its own.
+  hb_table *parent = nullptr;
+  int64_t row_off = 0;
+  std::atomic<int> slices{0};  // live slices of this table
 };
 
 struct PlanRule {
@@ -370,6 +380,8 @@ struct Batch {
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
   hipEvent_t marks[3] = {nullptr, nullptr, nullptr};  // after join, filter, exact
   float join_ms = 0, filter_ms = 0, exact_ms = 0, device_ms = 0;
+  std::chrono::steady_clock::time_point t_done;  // completion first observed
+  size_t mem_estimate = 0;                       // working memory admitted for it
 
   uint32_t n_jobs() const { return (uint32_t)items.size(); }
   uint32_t stride() const { return n_jobs() + 1; }
@@ -379,10 +391,17 @@ struct Batch {
   ~Batch();
 };
 
+using Clock = std::chrono::steady_clock;
+
 struct Job {
   uint64_t id = 0;
   hb_ctx *ctx = nullptr;
-  std::shared_ptr<Batch> batch;
+  const hb_plan *plan = nullptr;
+  hb_table *L = nullptr, *R = nullptr;
+  Clock::time_point t_submit, t_admit;
+  std::mutex mu;                  // batch assignment (a served job waits in a queue first)
+  std::condition_variable cv;
+  std::shared_ptr<Batch> batch;   // null while queued
   uint32_t index = 0;
   hb_job_stats stats{};
   bool stats_done = false;
@@ -433,6 +452,20 @@ hb_status record_lazy(Lazy &lz, hipStream_t stream) {
 
 hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   Column &col = t->cols[c];
+  if (t->parent) {  // borrow the parent's signatures at the slice's offset
+    HB_TRY(ensure_sig(t->parent, c, stream));
+    const Column &pc = t->parent->cols[c];
+    std::lock_guard<std::mutex> lk(t->mu);
+    if (!col.sig.built) {
+      const size_t off = (size_t)t->row_off;
+      col.d_len = pc.d_len + off;
+      col.d_sig = pc.d_sig + off;
+      col.nwc = pc.nwc;
+      col.d_planes = pc.d_planes + off * pc.nwc * 8;
+      col.sig.built = true;
+    }
+    return HB_OK;
+  }
   std::lock_guard<std::mutex> lk(t->mu);
   if (col.sig.built) return wait_lazy(col.sig, stream);
   const size_t n = (size_t)std::max<int64_t>(t->n_rows, 1);
@@ -448,6 +481,17 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
 
 hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   Column &col = t->cols[c];
+  if (t->parent) {  // borrow the parent's codes at the slice's offset
+    HB_TRY(ensure_code(t->parent, c, stream));
+    const Column &pc = t->parent->cols[c];
+    std::lock_guard<std::mutex> lk(t->mu);
+    if (!col.code.built) {
+      col.d_code = pc.d_code + t->row_off;
+      col.h_code = pc.h_code + t->row_off;
+      col.code.built = true;
+    }
+    return HB_OK;
+  }
   std::lock_guard<std::mutex> lk(t->mu);
   if (col.code.built) return wait_lazy(col.code, stream);
   const size_t n = (size_t)t->n_rows;
@@ -471,7 +515,21 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
 // equality column's dictionary codes and the similarity column's lengths,
 // radix-sorted with the row ids, the runs of equal code, and the similarity
 // signatures gathered into that order.
-hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, Layout **out) {
+//
+// The build runs on the context's build stream, which the host waits for
+// (the run count sizes the batch's groups); jobs' streams wait on the
+// layout's event, so a build never waits for other batches' kernels.
+hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t caller, Layout **out) {
+  {
+    std::lock_guard<std::mutex> lk(t->mu);
+    auto it = t->layouts.find({eq, sim});
+    if (it != t->layouts.end() && it->second.lz.built) {
+      *out = &it->second;
+      return wait_lazy(it->second.lz, caller);
+    }
+  }
+  const hipStream_t stream = t->ctx->build_stream;
+  HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
   if (eq >= 0) HB_TRY(ensure_code(t, eq, stream));
   if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
   uint64_t n_codes = 1;
@@ -483,7 +541,7 @@ hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, Layout
   std::lock_guard<std::mutex> lk(t->mu);
   Layout &ly = t->layouts[{eq, sim}];
   *out = &ly;
-  if (ly.lz.built) return wait_lazy(ly.lz, stream);
+  if (ly.lz.built) return wait_lazy(ly.lz, caller);
   const uint32_t n = (uint32_t)t->n_rows;
   const size_t n1 = std::max<size_t>(n, 1);
   HIP_OK(hipMalloc(&ly.d_keys, n1 * sizeof(uint64_t)));
@@ -528,7 +586,8 @@ hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t stream, Layout
     HIP_OK(hipFree(run_id));
     HIP_OK(hipHostFree(h_nruns));
   }
-  return record_lazy(ly.lz, stream);
+  HB_TRY(record_lazy(ly.lz, stream));
+  return wait_lazy(ly.lz, caller);
 }
 
 // Extra equality codes of `cols` in the layout's order (built once).
@@ -869,6 +928,7 @@ hb_status batch_complete(Batch *b) {
     HB_TRY(enqueue_batch(b));
     return HB_NOT_READY;
   }
+  b->t_done = std::chrono::steady_clock::now();
   (void)hipEventElapsedTime(&b->device_ms, b->ev_start, b->ev_end);
   (void)hipEventElapsedTime(&b->join_ms, b->ev_start, b->marks[0]);
   (void)hipEventElapsedTime(&b->filter_ms, b->marks[0], b->marks[1]);
@@ -922,6 +982,8 @@ void fill_stats(Job *j) {
   s.exact_ms = b->exact_ms;
   s.retries = b->retries;
   s.n_rules = (int32_t)it.plan->rules.size();
+  s.queue_ms = std::chrono::duration<double, std::milli>(j->t_admit - j->t_submit).count();
+  s.latency_ms = std::chrono::duration<double, std::milli>(b->t_done - j->t_submit).count();
   j->stats_done = true;
 }
 
@@ -941,25 +1003,65 @@ hb_status check_job_args(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, co
   return HB_OK;
 }
 
-// Creates the jobs of one batch and enqueues it.
-hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
-                       const hb_table *const *ls, const hb_table *const *rs,
-                       const uint64_t *ids) {
+// Registers jobs under their ids (not yet in a batch).
+hb_status register_jobs(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
+                        const hb_table *const *ls, const hb_table *const *rs,
+                        const uint64_t *ids, std::vector<Job *> &out) {
   if (!ctx || n == 0 || !plans || !ls || !rs || !ids)
-    return fail(HB_ERR_INVALID_ARG, "batch submit: no jobs or null arrays");
+    return fail(HB_ERR_INVALID_ARG, "job submit: no jobs or null arrays");
   for (uint32_t j = 0; j < n; j++) HB_TRY(check_job_args(ctx, plans[j], ls[j], rs[j]));
+  const auto now = Clock::now();
+  std::vector<std::unique_ptr<Job>> jobs;
+  for (uint32_t j = 0; j < n; j++) {
+    auto job = std::make_unique<Job>();
+    job->id = ids[j];
+    job->ctx = ctx;
+    job->plan = plans[j];
+    job->L = const_cast<hb_table *>(ls[j]);
+    job->R = const_cast<hb_table *>(rs[j]);
+    job->t_submit = now;
+    jobs.push_back(std::move(job));
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  for (uint32_t j = 0; j < n; j++) {
+    if (ctx->jobs.count(ids[j]))
+      return fail(HB_ERR_INVALID_ARG, "job id %llu already in use", (unsigned long long)ids[j]);
+    for (uint32_t k = 0; k < j; k++)
+      if (ids[k] == ids[j])
+        return fail(HB_ERR_INVALID_ARG, "job id %llu repeated in the batch",
+                    (unsigned long long)ids[j]);
+  }
+  out.clear();
+  for (auto &job : jobs) {
+    out.push_back(job.get());
+    ctx->jobs.emplace(job->id, std::move(job));
+  }
+  return HB_OK;
+}
+
+// Working memory of a job in a batch, for admission: pair lists, groups
+// and dedupe sets scale with its rows (about 160 bytes per row).
+size_t job_memory_estimate(const Job *j) {
+  return (size_t)(j->L->n_rows + j->R->n_rows) * 160 + 4096;
+}
+
+// Runs registered jobs as one batch and assigns it to them.
+hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
+                       std::shared_ptr<Batch> *out = nullptr) {
   HB_TRY(set_device(ctx));
   auto b = std::make_shared<Batch>();
   b->ctx = ctx;
   b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
+  const uint32_t n = (uint32_t)jobs.size();
   unsigned long long rows = 0;
   b->set_cap.assign(n, 0);
   for (uint32_t j = 0; j < n; j++) {
-    Batch::Item it{plans[j], const_cast<hb_table *>(ls[j]), const_cast<hb_table *>(rs[j])};
+    Batch::Item it{jobs[j]->plan, jobs[j]->L, jobs[j]->R};
     b->items.push_back(it);
     const unsigned long long jr = (unsigned long long)(it.L->n_rows + it.R->n_rows);
     rows += jr;
-    if (plans[j]->rules.size() > 1) b->set_cap[j] = pow2_at_least(2 * jr);
+    if (it.plan->rules.size() > 1) b->set_cap[j] = pow2_at_least(2 * jr);
+    b->mem_estimate += job_memory_estimate(jobs[j]);
   }
   b->list_cap = std::max<unsigned long long>(1ull << 17, rows);
   b->defer_cap = std::max<unsigned long long>(1ull << 12, rows / 64);
@@ -968,36 +1070,45 @@ hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
   HB_TRY(ctx->events.get(&b->ev_start));
   HB_TRY(ctx->events.get(&b->ev_end));
   for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));
-  std::vector<std::unique_ptr<Job>> jobs;
-  for (uint32_t j = 0; j < n; j++) {
-    auto job = std::make_unique<Job>();
-    job->id = ids[j];
-    job->ctx = ctx;
-    job->batch = b;
-    job->index = j;
-    jobs.push_back(std::move(job));
-  }
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
-    for (uint32_t j = 0; j < n; j++) {
-      if (ctx->jobs.count(ids[j]))
-        return fail(HB_ERR_INVALID_ARG, "job id %llu already in use", (unsigned long long)ids[j]);
-      for (uint32_t k = 0; k < j; k++)
-        if (ids[k] == ids[j])
-          return fail(HB_ERR_INVALID_ARG, "job id %llu repeated in the batch",
-                      (unsigned long long)ids[j]);
-    }
     b->shape = ctx->shape;
-    for (auto &job : jobs) ctx->jobs.emplace(job->id, std::move(job));
   }
-  std::lock_guard<std::mutex> lk(b->mu);
-  hb_status st = enqueue_batch(b.get());
-  if (st != HB_OK) {
-    b->state = BatchState::kFailed;
-    b->error = st;
-    b->error_text = g_last_error;
+  hb_status st;
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    st = enqueue_batch(b.get());
+    if (st != HB_OK) {
+      b->state = BatchState::kFailed;
+      b->error = st;
+      b->error_text = g_last_error;
+    }
   }
+  const auto now = Clock::now();
+  for (uint32_t j = 0; j < n; j++) {
+    std::lock_guard<std::mutex> lk(jobs[j]->mu);
+    jobs[j]->batch = b;
+    jobs[j]->index = j;
+    jobs[j]->t_admit = now;
+    jobs[j]->cv.notify_all();
+  }
+  if (out) *out = b;
   return st;
+}
+
+hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
+                       const hb_table *const *ls, const hb_table *const *rs,
+                       const uint64_t *ids) {
+  std::vector<Job *> jobs;
+  HB_TRY(register_jobs(ctx, n, plans, ls, rs, ids, jobs));
+  return launch_batch(ctx, jobs);
+}
+
+// The job's batch; waits for a queued job to be admitted when `block`.
+std::shared_ptr<Batch> job_batch(Job *j, bool block) {
+  std::unique_lock<std::mutex> lk(j->mu);
+  if (!j->batch && block) j->cv.wait(lk, [&] { return j->batch != nullptr; });
+  return j->batch;
 }
 
 }  // namespace
@@ -1054,6 +1165,7 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
       ctx->shape = hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc};
   }
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&ctx->build_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; i++) {
     hipStream_t s;
     HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -1104,6 +1216,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   ctx->hpool.release_all();
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+  if (ctx->build_stream) (void)hipStreamDestroy(ctx->build_stream);
   delete ctx;
   return HB_OK;
 }
@@ -1194,13 +1307,66 @@ hb_status hb_table_info(const hb_table *t, int64_t *n_rows, int32_t *n_cols) {
   return HB_OK;
 }
 
+hb_status hb_table_slice(hb_ctx *ctx, hb_table *parent, int64_t row_begin, int64_t row_end,
+                         hb_table **out) {
+  if (!ctx || !parent || !out) return fail(HB_ERR_INVALID_ARG, "hb_table_slice: null argument");
+  *out = nullptr;
+  if (parent->ctx != ctx) return fail(HB_ERR_INVALID_ARG, "hb_table_slice: table from another context");
+  if (row_begin < 0 || row_end < row_begin || row_end > parent->n_rows)
+    return fail(HB_ERR_INVALID_ARG, "hb_table_slice: rows [%lld, %lld) outside [0, %lld)",
+                (long long)row_begin, (long long)row_end, (long long)parent->n_rows);
+  hb_table *root = parent;
+  int64_t off = row_begin;
+  while (root->parent) {  // slices of slices borrow from the root table
+    off += root->row_off;
+    root = root->parent;
+  }
+  auto t = std::make_unique<hb_table>();
+  t->ctx = ctx;
+  t->n_rows = row_end - row_begin;
+  t->n_cols = root->n_cols;
+  t->parent = root;
+  t->row_off = off;
+  t->ready = root->ready;
+  t->cols.resize(root->n_cols);
+  for (int c = 0; c < root->n_cols; c++) {
+    const Column &pc = root->cols[c];
+    Column &col = t->cols[c];
+    col.width = pc.width;
+    col.d_bytes = pc.d_bytes + (size_t)off * pc.width;
+    col.h_bytes = pc.h_bytes + (size_t)off * pc.width;
+  }
+  root->slices++;
+  *out = t.release();
+  return HB_OK;
+}
+
 hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
   if (!ctx || !t) return fail(HB_ERR_INVALID_ARG, "hb_table_release: null argument");
+  if (t->slices.load() > 0)
+    return fail(HB_ERR_INVALID_ARG, "hb_table_release: %d slices of this table are alive",
+                t->slices.load());
   HB_TRY(set_device(ctx));
   // Jobs hold no table memory of their own, so every stream that may still
   // read this table must drain first.
   for (hipStream_t s : ctx->streams) HIP_OK(hipStreamSynchronize(s));
   HIP_OK(hipStreamSynchronize(ctx->copy_stream));
+  HIP_OK(hipStreamSynchronize(ctx->build_stream));
+  if (t->parent) {  // only the slice's own layouts are freed
+    for (auto &kv : t->layouts) {
+      Layout &ly = kv.second;
+      (void)hipFree(ly.d_keys);
+      (void)hipFree(ly.d_perm);
+      (void)hipFree(ly.d_jsig);
+      (void)hipFree(ly.d_run_code);
+      (void)hipFree(ly.d_run_off);
+      for (auto &jc : ly.jcode) (void)hipFree(jc.second);
+      if (ly.lz.ev) (void)hipEventDestroy(ly.lz.ev);
+    }
+    t->parent->slices--;
+    delete t;
+    return HB_OK;
+  }
   for (Column &c : t->cols) {
     (void)hipFree(c.d_bytes);
     if (c.h_bytes) (void)hipHostFree(c.h_bytes);
@@ -1300,20 +1466,22 @@ hb_status hb_batch_submit(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *pl
 hb_status hb_job_query(hb_ctx *ctx, uint64_t job_id) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  return batch_poll(j->batch.get(), false);
+  std::shared_ptr<Batch> b = job_batch(j, false);
+  if (!b) return HB_NOT_READY;  // still queued
+  return batch_poll(b.get(), false);
 }
 
 hb_status hb_job_wait(hb_ctx *ctx, uint64_t job_id) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  return batch_poll(j->batch.get(), true);
+  return batch_poll(job_batch(j, true).get(), true);
 }
 
 hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs, uint64_t *n_pairs,
                          const hb_audit_entry **audit, uint64_t *n_audit) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  Batch *b = j->batch.get();
+  Batch *b = job_batch(j, true).get();
   HB_TRY(batch_poll(b, true));
   std::lock_guard<std::mutex> lk(b->mu);
   // The scatter kernel wrote the outputs into pinned memory; the batch's
@@ -1330,7 +1498,7 @@ hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs, u
 hb_status hb_job_stats_get(hb_ctx *ctx, uint64_t job_id, hb_job_stats *out) {
   Job *j = ctx ? find_job(ctx, job_id) : nullptr;
   if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
-  Batch *b = j->batch.get();
+  Batch *b = job_batch(j, true).get();
   HB_TRY(batch_poll(b, true));
   std::lock_guard<std::mutex> lk(b->mu);
   fill_stats(j);
@@ -1342,6 +1510,8 @@ hb_status hb_job_release(hb_ctx *ctx, uint64_t job_id) {
   std::unique_ptr<Job> job;
   {
     if (!ctx) return fail(HB_ERR_INVALID_ARG, "hb_job_release: null context");
+    Job *queued = find_job(ctx, job_id);
+    if (queued) (void)job_batch(queued, true);  // a served job leaves the queue first
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->jobs.find(job_id);
     if (it == ctx->jobs.end())
@@ -1384,6 +1554,209 @@ hb_status hb_block(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, const hb
 }
 
 void hb_free(void *p) { std::free(p); }
+
+}  // extern "C"
+
+// Serving loop: a queue of jobs admitted into batches as batch slots and
+// working memory free up, so that jobs arriving while others run overlap
+// with them on the device (continuous batching).
+struct hb_server {
+  hb_ctx *ctx = nullptr;
+  hb_server_config cfg{};
+  size_t budget = 0;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job *> queue;
+  std::vector<std::shared_ptr<Batch>> inflight;
+  size_t mem_in_use = 0;
+  bool stop = false;
+  uint64_t batches = 0, jobs_run = 0;
+};
+
+namespace {
+
+void server_loop(hb_server *s) {
+  (void)hipSetDevice(s->ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    // retire finished batches (failed ones report through their jobs)
+    for (auto it = s->inflight.begin(); it != s->inflight.end();) {
+      if (batch_poll(it->get(), false) == HB_NOT_READY) {
+        ++it;
+        continue;
+      }
+      s->mem_in_use -= (*it)->mem_estimate;
+      it = s->inflight.erase(it);
+    }
+    if (s->stop && s->queue.empty() && s->inflight.empty()) break;
+    // admit queued jobs while a batch slot and memory are free
+    bool admitted = false;
+    while (!s->queue.empty() && (int)s->inflight.size() < s->cfg.max_batches_in_flight) {
+      std::vector<Job *> take;
+      size_t mem = 0;
+      while (!s->queue.empty() && (int)take.size() < s->cfg.max_jobs_per_batch) {
+        const size_t m = job_memory_estimate(s->queue.front());
+        const bool idle = take.empty() && s->inflight.empty();  // always admit one job
+        if (!idle && s->mem_in_use + mem + m > s->budget) break;
+        take.push_back(s->queue.front());
+        s->queue.pop_front();
+        mem += m;
+      }
+      if (take.empty()) break;
+      lk.unlock();
+      std::shared_ptr<Batch> b;
+      (void)launch_batch(s->ctx, take, &b);  // a failure is reported through the jobs
+      lk.lock();
+      s->mem_in_use += b->mem_estimate;
+      s->inflight.push_back(std::move(b));
+      s->batches++;
+      s->jobs_run += take.size();
+      admitted = true;
+    }
+    if (admitted) continue;
+    if (s->inflight.empty()) {
+      s->cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+    } else {
+      // poll for completions; arrivals wake the loop early
+      s->cv.wait_for(lk, std::chrono::microseconds(50));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+hb_status hb_server_create(hb_ctx *ctx, const hb_server_config *cfg, hb_server **out) {
+  if (!ctx || !out) return fail(HB_ERR_INVALID_ARG, "hb_server_create: null argument");
+  *out = nullptr;
+  auto s = std::make_unique<hb_server>();
+  s->ctx = ctx;
+  s->cfg.max_batches_in_flight = 2;
+  s->cfg.max_jobs_per_batch = 256;
+  s->cfg.memory_budget = 0;
+  if (cfg) s->cfg = *cfg;
+  if (s->cfg.max_batches_in_flight < 1 || s->cfg.max_jobs_per_batch < 1)
+    return fail(HB_ERR_INVALID_ARG, "hb_server_create: batches in flight and jobs per batch "
+                "must be at least 1");
+  HB_TRY(set_device(ctx));
+  if (s->cfg.memory_budget) {
+    s->budget = (size_t)s->cfg.memory_budget;
+  } else {
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    s->budget = free_b / 4 * 3;
+  }
+  hb_server *sp = s.get();
+  s->worker = std::thread(server_loop, sp);
+  *out = s.release();
+  return HB_OK;
+}
+
+hb_status hb_server_submit(hb_server *s, const hb_plan *plan, const hb_table *l,
+                           const hb_table *r, uint64_t job_id) {
+  if (!s) return fail(HB_ERR_INVALID_ARG, "hb_server_submit: null server");
+  std::vector<Job *> jobs;
+  HB_TRY(register_jobs(s->ctx, 1, &plan, &l, &r, &job_id, jobs));
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->queue.push_back(jobs[0]);
+  }
+  s->cv.notify_one();
+  return HB_OK;
+}
+
+hb_status hb_server_info(hb_server *s, uint64_t *batches, uint64_t *jobs, uint64_t *queued) {
+  if (!s) return fail(HB_ERR_INVALID_ARG, "hb_server_info: null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (batches) *batches = s->batches;
+  if (jobs) *jobs = s->jobs_run;
+  if (queued) *queued = s->queue.size();
+  return HB_OK;
+}
+
+hb_status hb_server_destroy(hb_server *s) {
+  if (!s) return HB_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_one();
+  if (s->worker.joinable()) s->worker.join();
+  delete s;
+  return HB_OK;
+}
+
+hb_status hb_autotune_filter_shape(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
+                                   const hb_table *const *l, const hb_table *const *r,
+                                   const int32_t *candidates, uint32_t n_candidates,
+                                   int32_t best[3], double *best_ms) {
+  if (!ctx || n_jobs == 0) return fail(HB_ERR_INVALID_ARG, "hb_autotune_filter_shape: no jobs");
+  std::vector<hb::FilterShape> cands;
+  if (candidates && n_candidates) {
+    for (uint32_t k = 0; k < n_candidates; k++) {
+      const int32_t *c = candidates + 3 * k;
+      if (!valid_filter_shape(c[0], c[1], c[2]))
+        return fail(HB_ERR_INVALID_ARG, "candidate %u (%d, %d, %d) is not a valid shape", k, c[0],
+                    c[1], c[2]);
+      cands.push_back(hb::FilterShape{c[0], (uint32_t)c[1], (unsigned)c[2]});
+    }
+  } else {
+    for (int rpl : {1, 2, 4})
+      for (int tb : {256, 512})
+        for (int gpc : {8, 16, 32}) cands.push_back(hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc});
+  }
+  static std::atomic<uint64_t> seq{0};
+  std::vector<uint64_t> ids(n_jobs);
+  // One run: the batch's filter-stage time with the context's current shape.
+  auto run = [&](double *ms) -> hb_status {
+    for (auto &id : ids) id = 0xC000000000000000ull | seq++;
+    HB_TRY(submit_batch(ctx, n_jobs, plans, l, r, ids.data()));
+    hb_status st = hb_job_wait(ctx, ids[0]);
+    hb_job_stats js{};
+    if (st == HB_OK) st = hb_job_stats_get(ctx, ids[0], &js);
+    for (uint64_t id : ids) (void)hb_job_release(ctx, id);
+    if (ms) *ms = js.filter_ms;
+    return st;
+  };
+  const hb::FilterShape saved = ctx->shape;
+  HB_TRY(run(nullptr));  // cold structures (layouts) out of the way
+  double best_t = INFINITY;
+  hb::FilterShape best_s = saved;
+  for (const hb::FilterShape &c : cands) {
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->shape = c;
+    }
+    double t = INFINITY;
+    for (int rep = 0; rep < 2; rep++) {
+      double ms = 0;
+      hb_status st = run(&ms);
+      if (st != HB_OK) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->shape = saved;
+        return st;
+      }
+      t = std::min(t, ms);
+    }
+    if (t < best_t) {
+      best_t = t;
+      best_s = c;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->shape = best_s;
+  }
+  if (best) {
+    best[0] = best_s.rows_per_lane;
+    best[1] = (int32_t)best_s.tile_b;
+    best[2] = (int32_t)best_s.grid_per_cu;
+  }
+  if (best_ms) *best_ms = best_t;
+  return HB_OK;
+}
 
 hb_status hb_jw_from_counts(hb_ctx *ctx, const int32_t *mstp, int64_t n, double *out,
                             double *out_fused) {

@@ -1,0 +1,243 @@
+"""Batched and served blocking workloads (the job-set forms of the path).
+
+Batched (c4): 4,096 independent jobs, per-side table sizes log-uniform on
+[100, 50k] rows, each job using 1-4 rules drawn from four templates (the
+three rules of the streamed 1M x 1M workload plus venue equality with
+JW(city) >= 0.9), 8-column tables of the streamed workload's schema.  Job
+tables are slices of two resident 1M-row pools.  The batch runs as one
+launch sequence (hb_batch_submit) and is compared with the same 4,096 jobs
+submitted one by one (hb_job_submit, a few in flight).
+
+Served (c5): a 2M-row right table resident in HBM; left jobs of 1-10k rows
+(slices of a resident 200k-row pool) carrying 1-3 rules arrive as a Poisson
+stream at 70% of the measured capacity of plain sequential jobs.  They are
+served by the continuous-batching server (hb_server_*) and, on the same
+arrival trace, by plain one-at-a-time launches; both report p50/p99 latency
+(arrival to results ready) and throughput.
+
+Nothing here computes a predicate: every job runs through the C ABI.
+"""
+import math
+import random
+import threading
+import time
+
+import numpy as np
+
+from . import blocker as hbk
+from . import datagen
+
+TEMPLATES = datagen.RULES_C3 + [datagen.rule_yaml([3], [(5, 0.9)])]
+
+
+def c4_specs(n_jobs=4096, seed=4, pool=1_000_000, lo=100, hi=50_000):
+    """[(off_l, n_l, off_r, n_r, rule indices)] of the batched workload."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n_jobs):
+        n_l = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        n_r = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        k = rng.randint(1, 4)
+        rules = tuple(sorted(rng.sample(range(len(TEMPLATES)), k)))
+        out.append((rng.randrange(0, pool - n_l), n_l, rng.randrange(0, pool - n_r), n_r, rules))
+    return out
+
+
+def c5_specs(n_jobs, seed=5, pool=200_000, lo=1, hi=10_000):
+    """[(off_l, n_l, rule indices)] of the served workload (1-3 rules)."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n_jobs):
+        n_l = rng.randint(lo, hi)
+        k = rng.randint(1, 3)
+        rules = tuple(sorted(rng.sample(range(len(TEMPLATES)), k)))
+        out.append((rng.randrange(0, pool - n_l), n_l, rules))
+    return out
+
+
+class _Plans:
+    """Device plans per rule subset, created on first use."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.plans = {}
+
+    def __getitem__(self, rules):
+        if rules not in self.plans:
+            self.plans[rules] = self.ctx.plan(
+                hbk.HostPlan.from_yaml([TEMPLATES[r] for r in rules]))
+        return self.plans[rules]
+
+    def release(self):
+        for p in self.plans.values():
+            p.release()
+
+
+def _drain(jobs):
+    """Results of finished jobs copied out, jobs released; returns pairs."""
+    n = 0
+    for j in jobs:
+        pairs, _, _ = j.results()
+        n += len(pairs)
+        j.release()
+    return n
+
+
+def run_c4(ctx, n_jobs=4096, reps=2, inflight=4, autotune=True, log=print):
+    t0 = time.perf_counter()
+    L, R, _ = datagen.generate(4, 1_000_000, 1_000_000, 850_000, datagen.SCHEMA8)
+    tl = ctx.upload(hbk.ColumnTable(L))
+    tr = ctx.upload(hbk.ColumnTable(R))
+    ctx.synchronize()
+    log("c4: pools generated and uploaded in %.1f s" % (time.perf_counter() - t0))
+    specs = c4_specs(n_jobs)
+    plans = _Plans(ctx)
+    slices = []
+    jobs = []
+    for off_l, n_l, off_r, n_r, rules in specs:
+        sl, sr = tl.slice(off_l, off_l + n_l), tr.slice(off_r, off_r + n_r)
+        slices += [sl, sr]
+        jobs.append((plans[rules], sl, sr))
+    pairs_total = sum(n_l * n_r for _, n_l, _, n_r, _ in specs)
+    try:
+        t = time.perf_counter()
+        _drain(ctx.submit_batch(jobs))  # per-table structures (layouts) built here
+        cold_s = time.perf_counter() - t
+        log("c4: cold batch %.2f s" % cold_s)
+        tuned = None
+        if autotune:
+            t = time.perf_counter()
+            sample = jobs[:: max(1, len(jobs) // 512)]
+            cands = [(1, 256, 16), (2, 256, 16), (2, 256, 32), (2, 512, 16), (4, 256, 12),
+                     (4, 512, 8)]
+            tuned = ctx.autotune(sample, cands)
+            log("c4: autotuned shape %s (filter %.3f ms on %d jobs) in %.1f s"
+                % (tuned[0], tuned[1], len(sample), time.perf_counter() - t))
+        # batched: one launch sequence for all jobs
+        batched = []
+        stats = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            handles = ctx.submit_batch(jobs)
+            stats = handles[0].stats()
+            matches = _drain(handles)
+            batched.append(time.perf_counter() - t)
+        # plain: one job per submit, a few in flight
+        plain = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            pending = []
+            for p, sl, sr in jobs:
+                pending.append(ctx.submit(p, sl, sr))
+                if len(pending) >= inflight:
+                    _drain([pending.pop(0)])
+            _drain(pending)
+            plain.append(time.perf_counter() - t)
+        bt, pt = min(batched), min(plain)
+        return {
+            "jobs": n_jobs, "pairs": pairs_total, "matches": matches,
+            "batched_s": bt, "plain_s": pt,
+            "batched_pairs_per_s": pairs_total / bt, "plain_pairs_per_s": pairs_total / pt,
+            "speedup": pt / bt, "cold_s": cold_s,
+            "batch_device_ms": stats["device_ms"], "batch_filter_ms": stats["filter_ms"],
+            "batch_exact_ms": stats["exact_ms"], "batch_join_ms": stats["join_ms"],
+            "filter_shape": list(tuned[0]) if tuned else list(ctx.filter_shape),
+        }
+    finally:
+        for s in slices:
+            s.release()
+        tl.release()
+        tr.release()
+        plans.release()
+
+
+def _percentiles(xs):
+    a = np.sort(np.asarray(xs))
+    return float(np.percentile(a, 50)), float(np.percentile(a, 99))
+
+
+def _serve_trace(submit, specs, arrivals, plans, tables, n_l_pairs):
+    """Submits specs[k] at arrivals[k] seconds (busy-wait clock) and collects
+    each job's stats.  Returns (latencies ms, wall s)."""
+    handles = [None] * len(specs)
+    lat = []
+    done = threading.Event()
+
+    def consume():
+        for k in range(len(specs)):
+            while handles[k] is None:
+                time.sleep(0.0001)
+            j = handles[k]
+            j.wait()
+            s = j.stats()
+            lat.append(s["latency_ms"])
+            j.release()
+        done.set()
+
+    th = threading.Thread(target=consume)
+    th.start()
+    t0 = time.perf_counter()
+    for k, ((off_l, n_l, rules), at) in enumerate(zip(specs, arrivals)):
+        while time.perf_counter() - t0 < at:
+            pass
+        handles[k] = submit(plans[rules], tables[k])
+    done.wait()
+    th.join()
+    return lat, time.perf_counter() - t0
+
+
+def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
+    t0 = time.perf_counter()
+    L, R, _ = datagen.generate(5, 200_000, 2_000_000, 170_000, datagen.SCHEMA8)
+    tl = ctx.upload(hbk.ColumnTable(L))
+    tr = ctx.upload(hbk.ColumnTable(R))
+    ctx.synchronize()
+    log("c5: tables generated and uploaded in %.1f s" % (time.perf_counter() - t0))
+    specs = c5_specs(n_jobs)
+    plans = _Plans(ctx)
+    slices = [tl.slice(off, off + n) for off, n, _ in specs]
+    pairs_total = sum(n * 2_000_000 for _, n, _ in specs)
+    try:
+        # warm: per-table structures of every slice, and plain capacity
+        t = time.perf_counter()
+        _drain(ctx.submit_batch([(plans[r], sl, tr) for (_, _, r), sl in zip(specs, slices)]))
+        log("c5: cold batch %.2f s" % (time.perf_counter() - t))
+        n_cap = min(200, n_jobs)
+        t = time.perf_counter()
+        for (_, _, r), sl in zip(specs[:n_cap], slices[:n_cap]):
+            _drain([ctx.submit(plans[r], sl, tr)])
+        service = (time.perf_counter() - t) / n_cap
+        lam = load / service
+        rng = random.Random(55)
+        arrivals = []
+        at = 0.0
+        for _ in range(n_jobs):
+            at += rng.expovariate(lam)
+            arrivals.append(at)
+        log("c5: plain service %.3f ms/job, arrival rate %.0f jobs/s" % (service * 1e3, lam))
+        # served: continuous batching
+        with hbk.Server(ctx, max_batches_in_flight=2, max_jobs_per_batch=64) as srv:
+            lat_s, wall_s = _serve_trace(lambda p, sl: srv.submit(p, sl, tr), specs, arrivals,
+                                         plans, slices, pairs_total)
+            info = srv.info()
+        # plain: the same trace, one job at a time
+        with hbk.Server(ctx, max_batches_in_flight=1, max_jobs_per_batch=1) as srv:
+            lat_p, wall_p = _serve_trace(lambda p, sl: srv.submit(p, sl, tr), specs, arrivals,
+                                         plans, slices, pairs_total)
+        p50s, p99s = _percentiles(lat_s)
+        p50p, p99p = _percentiles(lat_p)
+        return {
+            "jobs": n_jobs, "right_rows": 2_000_000, "pairs": pairs_total,
+            "arrival_rate_per_s": lam, "offered_load": load,
+            "plain_service_ms": service * 1e3,
+            "served_pairs_per_s": pairs_total / wall_s, "served_p50_ms": p50s,
+            "served_p99_ms": p99s, "served_batches": info["batches"],
+            "plain_pairs_per_s": pairs_total / wall_p, "plain_p50_ms": p50p,
+            "plain_p99_ms": p99p,
+        }
+    finally:
+        for s in slices:
+            s.release()
+        tl.release()
+        tr.release()
+        plans.release()

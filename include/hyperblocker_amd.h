@@ -54,6 +54,7 @@ typedef struct hb_plan hb_plan;
 typedef struct hb_host_table hb_host_table;
 typedef struct hb_host_plan hb_host_plan;
 typedef struct hb_partitions hb_partitions;
+typedef struct hb_server hb_server;
 
 /* Column-addressed view of a host table.  Cell (r, c) is the byte string at
  * col_base[c] + r * col_stride[c], ending at the first NUL or after
@@ -115,6 +116,8 @@ typedef struct {
   double exact_ms;     /* exact Jaro-Winkler stages, summed over rules */
   int32_t retries;     /* capacity regrowth reruns */
   int32_t n_rules;
+  double queue_ms;     /* submit to admission into a batch (served jobs) */
+  double latency_ms;   /* submit to completion of its batch */
 } hb_job_stats;
 
 /* ---- context -------------------------------------------------------------
@@ -154,6 +157,14 @@ hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *t,
  * AsyncSubmit, host_producer.h:137-152). */
 hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *view, const int64_t *rows,
                                int64_t n_rows, hb_table **out);
+/* A table made of rows [row_begin, row_end) of `parent`, sharing its device
+ * data and per-column structures (no copy; join layouts are built per
+ * slice).  Used for many small jobs over a resident table (batched and
+ * served forms); no counterpart in the reference, which copies every
+ * partition (data_mngr.h:209-215).  The parent cannot be released while
+ * slices of it are alive. */
+hb_status hb_table_slice(hb_ctx *ctx, hb_table *parent, int64_t row_begin, int64_t row_end,
+                         hb_table **out);
 hb_status hb_table_info(const hb_table *t, int64_t *n_rows, int32_t *n_cols);
 hb_status hb_table_release(hb_ctx *ctx, hb_table *t);
 
@@ -197,6 +208,41 @@ hb_status hb_block(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
                    const hb_table *r, uint32_t **pairs, uint64_t *n_pairs,
                    hb_audit_entry **audit, uint64_t *n_audit, hb_job_stats *stats);
 void hb_free(void *p);
+
+/* ---- serving loop ----------------------------------------------------------
+ * No counterpart in the reference.  A server owns a worker thread that
+ * admits queued jobs into batches (at most max_jobs_per_batch jobs, at most
+ * max_batches_in_flight batches running) while their estimated working
+ * memory fits memory_budget bytes (0: three quarters of the device's free
+ * memory at creation); a job that arrives while others run joins the next
+ * batch and overlaps with them on the device.  Each job is a regular job of
+ * the context: query, wait, read and release it by id (waiting covers the
+ * time it is queued).  A job's result is the same alone and in any batch. */
+typedef struct {
+  int32_t max_batches_in_flight;
+  int32_t max_jobs_per_batch;
+  uint64_t memory_budget;
+} hb_server_config;
+
+hb_status hb_server_create(hb_ctx *ctx, const hb_server_config *cfg, hb_server **out);
+hb_status hb_server_submit(hb_server *s, const hb_plan *plan, const hb_table *l,
+                           const hb_table *r, uint64_t job_id);
+/* Batches launched, jobs admitted so far, jobs waiting in the queue. */
+hb_status hb_server_info(hb_server *s, uint64_t *batches, uint64_t *jobs, uint64_t *queued);
+/* Runs the queue to the end, waits for running batches, stops the worker.
+ * Jobs stay readable through the context until released. */
+hb_status hb_server_destroy(hb_server *s);
+
+/* Autotuner of the bound filter's launch shape: runs the given jobs once to
+ * build their per-table structures, then twice per candidate shape, and
+ * sets the shape with the smallest filter-stage time as the context's.
+ * candidates holds n_candidates triples (rows per lane, tile_b, groups per
+ * CU); NULL tries 18 built-in shapes.  Writes the chosen triple and its
+ * time.  No counterpart in the reference (its launch is fixed). */
+hb_status hb_autotune_filter_shape(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
+                                   const hb_table *const *l, const hb_table *const *r,
+                                   const int32_t *candidates, uint32_t n_candidates,
+                                   int32_t best[3], double *best_ms);
 
 /* ---- host front end (no device needed) ---------------------------------- */
 
