@@ -129,6 +129,7 @@ struct BatchDev {
 struct GroupArrays {
   uint32_t n;
   uint32_t *l_off, *l_cnt, *r_off, *r_cnt;
+  uint32_t *jr;            // the group's JR
   uint64_t *small_pairs;   // pairs of groups with <= small_pairs_max pairs, else 0
   uint64_t *pair_off;      // exclusive scan of small_pairs
   uint64_t *items;         // work items of the other groups, else 0

@@ -109,7 +109,8 @@ __device__ __forceinline__ unsigned long long job_slot(bool ok, uint32_t job,
 // Survivor buffer of one work-group in LDS.  Waves append with one LDS
 // atomic each; the group moves the buffer to the global list with a single
 // global atomic (block_flush), so a busy list counter is not contended by
-// every wave.  Appends past the buffer go to the global list directly.
+// every wave.  Appends past the buffer go to the global list directly, one
+// atomic per wave.
 constexpr uint32_t kBlockOut = 512;
 struct BlockOut {
   PairEntry e[kBlockOut];
@@ -126,19 +127,10 @@ __device__ __forceinline__ void block_append(bool ok, uint32_t l, uint32_t r, ui
   uint32_t base = 0;
   if ((int)lane_id() == leader) base = atomicAdd(&bo.n, (uint32_t)__popcll(mask));
   base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-  if (ok) {
-    const uint32_t pos = base + rank_in(mask);
-    const PairEntry e{l, r, jr, 0u};
-    if (pos < kBlockOut) {
-      bo.e[pos] = e;
-    } else {
-      const unsigned long long g = atomicAdd(out.count, 1ull);
-      if (g < out.cap)
-        out.e[g] = e;
-      else
-        *overflow = 1ull;
-    }
-  }
+  const uint32_t pos = base + rank_in(mask);
+  const PairEntry e{l, r, jr, 0u};
+  if (ok && pos < kBlockOut) bo.e[pos] = e;
+  list_append(ok && pos >= kBlockOut, e, out, overflow);
 }
 
 // Called by every thread of the group.  Leaves bo.n == 0.
@@ -158,6 +150,70 @@ __device__ __forceinline__ void block_flush(BlockOut &bo, const PairList &out,
   }
   __syncthreads();
   if (threadIdx.x == 0) bo.n = 0;
+  __syncthreads();
+}
+
+// Survivor buffer of a filter work item: 4-byte entries a | b << 10, with a
+// the A row's index in the item's A tile (< 1024) and b the B row's index
+// from the start of the item's B segment (< 65536).  The group flushes it
+// at the end of every work item (and at a chunk boundary when half full),
+// translating entries to table rows.  Entries past the buffer go to the
+// global list directly, one atomic per wave.
+constexpr uint32_t kTileOut = 2048;
+struct TileOut {
+  uint32_t e[kTileOut];
+  uint32_t n;
+  unsigned long long base;
+};
+
+// What a TileOut entry needs to become a pair list entry.
+struct TileCtx {
+  const uint32_t *perm_a;  // join-order permutation of the A side, from the group's first row
+  const uint32_t *perm_b;  // same for the B side, from the item's first B row
+  uint32_t a_base;         // A tile's first index in the group
+  uint32_t jr;
+  bool a_is_l;
+};
+
+__device__ __forceinline__ PairEntry tile_entry(const TileCtx &c, uint32_t a, uint32_t b) {
+  const uint32_t ar = c.perm_a[c.a_base + a], br = c.perm_b[b];
+  return PairEntry{c.a_is_l ? ar : br, c.a_is_l ? br : ar, c.jr, 0u};
+}
+
+__device__ __forceinline__ void tile_append(bool ok, uint32_t a, uint32_t b, const TileCtx &c,
+                                            TileOut &to, const PairList &out,
+                                            unsigned long long *overflow) {
+  const uint64_t mask = __ballot(ok);
+  if (mask == 0) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(&to.n, (uint32_t)__popcll(mask));
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+  const uint32_t pos = base + rank_in(mask);
+  if (ok && pos < kTileOut) to.e[pos] = a | (b << 10);
+  const bool spill = ok && pos >= kTileOut;
+  if (__builtin_amdgcn_ballot_w64(spill))
+    list_append(spill, spill ? tile_entry(c, a, b) : PairEntry{}, out, overflow);
+}
+
+// Called by every thread of the group.  Leaves to.n == 0.
+__device__ __forceinline__ void tile_flush(TileOut &to, const TileCtx &c, const PairList &out,
+                                           unsigned long long *overflow) {
+  __syncthreads();
+  const uint32_t n = min(to.n, kTileOut);
+  if (threadIdx.x == 0) to.base = n ? atomicAdd(out.count, (unsigned long long)n) : 0ull;
+  __syncthreads();
+  const unsigned long long base = to.base;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned long long pos = base + i;
+    const uint32_t v = to.e[i];
+    if (pos < out.cap)
+      out.e[pos] = tile_entry(c, v & 1023u, v >> 10);
+    else
+      *overflow = 1ull;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) to.n = 0;
   __syncthreads();
 }
 
@@ -377,7 +433,7 @@ __global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
   const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
   if (gi > g.n) return;
   if (gi == g.n) {
-    g.l_off[gi] = g.l_cnt[gi] = g.r_off[gi] = g.r_cnt[gi] = 0;
+    g.l_off[gi] = g.l_cnt[gi] = g.r_off[gi] = g.r_cnt[gi] = g.jr[gi] = 0;
     g.small_pairs[gi] = 0;
     g.items[gi] = 0;
     return;
@@ -402,6 +458,7 @@ __global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
   g.l_cnt[gi] = l_cnt;
   g.r_off[gi] = r_off;
   g.r_cnt[gi] = r_cnt;
+  g.jr[gi] = jr;
   set_group_work(g, gi, l_cnt, r_cnt);
   const unsigned long long jp = (unsigned long long)l_cnt * r_cnt;
   if (jp) atomicAdd(b.job_cnt + (size_t)kJobJoined * b.cnt_stride + d.job, jp);
@@ -504,14 +561,14 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
   __shared__ uint32_t s_code[NEQ > 0 ? NEQ : 1][kChunk + 1];
   __shared__ unsigned long long s_starts[kChunk / 64];  // bit i: B row i starts a run
   __shared__ float s_rlen[SIM ? kChunk + 1 : 1];
-  __shared__ BlockOut bo;
+  __shared__ TileOut to;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) bo.n = 0;
+  if (tid == 0) to.n = 0;
   const uint32_t tile_b = g.tile_b;
   const uint64_t item_begin = g.item_off[g_begin], item_end = g.item_off[g_end];
   for (uint64_t item = item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
     const uint32_t gi = g_begin + find_group(g.item_off + g_begin, g_end - g_begin, item);
-    const uint32_t jr = find_le_u32(bd.jr_group_off, bd.n_jr, gi);
+    const uint32_t jr = g.jr[gi];
     const JRDesc &d = bd.jrs[jr];
     const float k0 = 2.0f * d.c0, k3 = 2.0f * d.c3;
     const uint32_t l_off = g.l_off[gi], l_cnt = g.l_cnt[gi];
@@ -557,10 +614,11 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     }
     const uint32_t b_begin = b_seg * tile_b;
     const uint32_t b_end = min(b_begin + tile_b, b_cnt);
+    const TileCtx tc{perm_a + a_off, perm_b + b_off + b_begin, a_tile * 256 * AR, jr, a_is_l};
     for (uint32_t cb = b_begin; cb < b_end; cb += kChunk) {
       const uint32_t n_b = min(kChunk, b_end - cb);
       __syncthreads();
-      if (bo.n >= kBlockOut / 2) block_flush(bo, out, bd.overflow);  // bo.n is group-uniform here
+      if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);  // to.n is group-uniform here
       uint32_t my_len = 0;
       if (tid < n_b) {
         if (SIM) {
@@ -632,26 +690,29 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
             for (int r = 0; r < AR; r++) any |= ok[u][r];
           if (__builtin_amdgcn_ballot_w64(any)) {  // rare: a survivor in this step
 #pragma unroll
-            for (int u = 0; u < BU; u++) {
+            for (int u = 0; u < BU; u++)
 #pragma unroll
-              for (int r = 0; r < AR; r++) {
-                if (__builtin_amdgcn_ballot_w64(ok[u][r]) == 0) continue;
-                const uint32_t b_row = perm_b[b_off + cb + min(jj + u, n_b - 1)];
-                const uint32_t a_row = ok[u][r] ? perm_a[a_off + a_idx[r]] : 0u;
-                block_append(ok[u][r], a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, jr, bo,
-                             out, bd.overflow);
-              }
-            }
+              for (int r = 0; r < AR; r++)
+                tile_append(ok[u][r], a_idx[r] - tc.a_base, cb - b_begin + jj + u, tc, to, out,
+                            bd.overflow);
           }
         }
         run = run_end;
       }
     }
+    __syncthreads();
+    if (to.n) tile_flush(to, tc, out, bd.overflow);  // group-uniform after the barrier
   }
-  block_flush(bo, out, bd.overflow);
 }
 
-// Pairs of small groups, one pair per lane.
+// Pairs of small groups (at most small_pairs_max pairs each), packed across
+// groups.  A wave takes kSmallPerLane x 64 consecutive pairs; two
+// wave-uniform binary searches bound the groups they span, and each lane
+// finds its pair's group from the previous one (a short walk, or a binary
+// search when the span is wide).  Trip counts are uniform per wave, so the
+// ballots inside see all lanes.
+constexpr uint32_t kSmallPerLane = 8;
+
 template <int NEQ, bool SIM>
 __global__ __launch_bounds__(256) void k_filter_small(const BatchDev bd, const GroupArrays g,
                                                       uint32_t g_begin, uint32_t g_end,
@@ -660,35 +721,45 @@ __global__ __launch_bounds__(256) void k_filter_small(const BatchDev bd, const G
   if (threadIdx.x == 0) bo.n = 0;
   __syncthreads();
   const uint64_t begin = g.pair_off[g_begin], total = g.pair_off[g_end];
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  // Trip count is uniform per wave: every lane runs the same number of
-  // rounds so the ballots inside see all lanes.
-  const uint64_t base0 = begin + (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint64_t q0 = base0; q0 < total; q0 += stride) {
-    const uint64_t q = q0 + lane_id();
-    bool ok = q < total;
-    uint32_t l_row = 0, r_row = 0, li = 0, ri = 0, jr = 0;
-    const JRDesc *d = nullptr;
-    if (ok) {
-      const uint32_t gi = g_begin + find_group(g.pair_off + g_begin, g_end - g_begin, q);
-      jr = find_le_u32(bd.jr_group_off, bd.n_jr, gi);
-      d = bd.jrs + jr;
-      const uint64_t local = q - g.pair_off[gi];
-      const uint32_t r_cnt = g.r_cnt[gi];
-      li = g.l_off[gi] + (uint32_t)(local / r_cnt);
-      ri = g.r_off[gi] + (uint32_t)(local % r_cnt);
-      l_row = d->s[0].perm[li];
-      r_row = d->s[1].perm[ri];
+  const uint32_t n_groups = g_end - g_begin;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint64_t n_waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  constexpr uint64_t kSpan = 64 * kSmallPerLane;
+  for (uint64_t q0 = begin + (uint64_t)wave * kSpan; q0 < total; q0 += n_waves * kSpan) {
+    const uint64_t q_last = min(q0 + kSpan, total) - 1;
+    const uint32_t g_lo = g_begin + find_group(g.pair_off + g_begin, n_groups, q0);
+    const uint32_t g_hi = g_begin + find_group(g.pair_off + g_begin, n_groups, q_last);
+    uint32_t gi = g_lo;  // pair_off[gi] <= every q of this lane below
+    for (uint32_t k = 0; k < kSmallPerLane; k++) {
+      const uint64_t q = q0 + k * 64 + lane_id();
+      bool ok = q < total;
+      uint32_t l_row = 0, r_row = 0, li = 0, ri = 0, jr = 0;
+      const JRDesc *d = nullptr;
+      if (ok) {
+        if (g_hi - g_lo <= 8) {
+          while (g.pair_off[gi + 1] <= q) gi++;
+        } else {
+          gi += find_group(g.pair_off + gi, g_hi + 1 - gi, q);
+        }
+        jr = g.jr[gi];
+        d = bd.jrs + jr;
+        const uint32_t local = (uint32_t)(q - g.pair_off[gi]);  // < small_pairs_max
+        const uint32_t r_cnt = g.r_cnt[gi];
+        li = g.l_off[gi] + local / r_cnt;
+        ri = g.r_off[gi] + local % r_cnt;
+        l_row = d->s[0].perm[li];
+        r_row = d->s[1].perm[ri];
 #pragma unroll
-      for (int k = 0; k < NEQ; k++)
-        ok = ok && (d->s[0].jcode[(uint64_t)li * kMaxEq + k] ==
-                    d->s[1].jcode[(uint64_t)ri * kMaxEq + k]);
+        for (int e = 0; e < NEQ; e++)
+          ok = ok && (d->s[0].jcode[(uint64_t)li * kMaxEq + e] ==
+                      d->s[1].jcode[(uint64_t)ri * kMaxEq + e]);
+      }
+      if (SIM && ok) {
+        const SigRow sa = d->s[0].jsig[li], sb = d->s[1].jsig[ri];
+        ok = may_pass(sa.h, sa.pre, sa.len, sb.h, sb.pre, sb.len, d->c0, d->c1, d->c2, d->c3);
+      }
+      block_append(ok, l_row, r_row, jr, bo, out, bd.overflow);
     }
-    if (SIM && ok) {
-      const SigRow sa = d->s[0].jsig[li], sb = d->s[1].jsig[ri];
-      ok = may_pass(sa.h, sa.pre, sa.len, sb.h, sb.pre, sb.len, d->c0, d->c1, d->c2, d->c3);
-    }
-    block_append(ok, l_row, r_row, jr, bo, out, bd.overflow);
   }
   block_flush(bo, out, bd.overflow);  // every wave of the group reaches this point
 }

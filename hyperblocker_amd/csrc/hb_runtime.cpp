@@ -322,6 +322,14 @@ struct PlanRule {
   std::vector<Sim> sims;
 };
 
+// List sizes per table row (left + right rows of a job) that a batch
+// reserves for a plan's jobs: the defaults, raised to what the plan's
+// batches needed (so a plan whose filter keeps many pairs makes its next
+// batches reserve more instead of running twice).
+struct SizeHint {
+  double list = 1.0, defer = 1.0 / 64, final_ = 0.5, audit = 1.0 / 64;
+};
+
 struct hb_plan {
   int32_t n_rules = 0;
   std::vector<int32_t> pred_index;
@@ -329,6 +337,20 @@ struct hb_plan {
   std::vector<float> pred_threshold;
   std::vector<PlanRule> rules;
   int32_t max_col_l = -1, max_col_r = -1;
+  mutable std::mutex hint_mu;
+  mutable SizeHint hint;
+
+  SizeHint get_hint() const {
+    std::lock_guard<std::mutex> lk(hint_mu);
+    return hint;
+  }
+  void raise_hint(const SizeHint &h) const {
+    std::lock_guard<std::mutex> lk(hint_mu);
+    hint.list = std::max(hint.list, h.list);
+    hint.defer = std::max(hint.defer, h.defer);
+    hint.final_ = std::max(hint.final_, h.final_);
+    hint.audit = std::max(hint.audit, h.audit);
+  }
 };
 
 namespace {
@@ -795,7 +817,7 @@ hb_status enqueue_batch(Batch *b) {
   g.tile_a = 256u * (uint32_t)b->shape.rows_per_lane;
   g.tile_b = b->shape.tile_b;
   g.small_pairs_max = 16384;
-  uint32_t *u32arr[4];
+  uint32_t *u32arr[5];
   for (auto &a : u32arr) {
     HB_TRY(batch_alloc(b, ((size_t)n_groups + 1) * 4, &p));
     a = (uint32_t *)p;
@@ -804,6 +826,7 @@ hb_status enqueue_batch(Batch *b) {
   g.l_cnt = u32arr[1];
   g.r_off = u32arr[2];
   g.r_cnt = u32arr[3];
+  g.jr = u32arr[4];
   uint64_t *u64arr[4];
   for (auto &a : u64arr) {
     HB_TRY(batch_alloc(b, ((size_t)n_groups + 1) * 8, &p));
@@ -899,13 +922,28 @@ hb_status enqueue_batch(Batch *b) {
 // Called with b->mu held once ev_end has completed.
 hb_status batch_complete(Batch *b) {
   const unsigned long long *c = b->h_cnt;
-  if (c[kCntOverflow]) {
-    const int n_stage_slots = b->max_stage + 1;
-    unsigned long long need_list = 0, need_defer = 0;
-    for (int s = 0; s < n_stage_slots; s++) {
-      need_list = std::max(need_list, c[b->cnt_stage_off + s]);
-      need_defer = std::max(need_defer, c[b->cnt_stage_off + n_stage_slots + s]);
+  const int n_stage_slots = b->max_stage + 1;
+  unsigned long long need_list = 0, need_defer = 0;
+  for (int s = 0; s < n_stage_slots; s++) {
+    need_list = std::max(need_list, c[b->cnt_stage_off + s]);
+    need_defer = std::max(need_defer, c[b->cnt_stage_off + n_stage_slots + s]);
+  }
+  {  // what the batch needed, per row, for the plans it ran (list counters
+     // count every append, also those past a full list)
+    double rows = 0;
+    for (const auto &it : b->items) rows += (double)(it.L->n_rows + it.R->n_rows);
+    if (rows > 0) {
+      SizeHint h;
+      h.list = 1.25 * (double)need_list / rows;
+      h.defer = 1.25 * (double)need_defer / rows;
+      h.final_ = 1.25 * (double)std::max(c[kCntFinal], c[kCntOverflow] ? need_list : 0ull) / rows;
+      h.audit = 1.25 * (double)c[kCntAudit] / rows;
+      const hb_plan *last = nullptr;
+      for (const auto &it : b->items)
+        if (it.plan != last) (last = it.plan)->raise_hint(h);
     }
+  }
+  if (c[kCntOverflow]) {
     auto grow = [](unsigned long long cur, unsigned long long need) {
       unsigned long long n = std::max<unsigned long long>(cur, 1);
       while (n < need + need / 4 + 1024) n *= 2;
@@ -1053,20 +1091,24 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
   b->ctx = ctx;
   b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
   const uint32_t n = (uint32_t)jobs.size();
-  unsigned long long rows = 0;
+  SizeHint want{0, 0, 0, 0};
   b->set_cap.assign(n, 0);
   for (uint32_t j = 0; j < n; j++) {
     Batch::Item it{jobs[j]->plan, jobs[j]->L, jobs[j]->R};
     b->items.push_back(it);
     const unsigned long long jr = (unsigned long long)(it.L->n_rows + it.R->n_rows);
-    rows += jr;
+    const SizeHint h = it.plan->get_hint();
+    want.list += h.list * jr;
+    want.defer += h.defer * jr;
+    want.final_ += h.final_ * jr;
+    want.audit += h.audit * jr;
     if (it.plan->rules.size() > 1) b->set_cap[j] = pow2_at_least(2 * jr);
     b->mem_estimate += job_memory_estimate(jobs[j]);
   }
-  b->list_cap = std::max<unsigned long long>(1ull << 17, rows);
-  b->defer_cap = std::max<unsigned long long>(1ull << 12, rows / 64);
-  b->final_cap = std::max<unsigned long long>(1ull << 16, rows / 2);
-  b->audit_cap = std::max<unsigned long long>(1ull << 12, rows / 64);
+  b->list_cap = std::max<unsigned long long>(1ull << 17, (unsigned long long)want.list);
+  b->defer_cap = std::max<unsigned long long>(1ull << 12, (unsigned long long)want.defer);
+  b->final_cap = std::max<unsigned long long>(1ull << 16, (unsigned long long)want.final_);
+  b->audit_cap = std::max<unsigned long long>(1ull << 12, (unsigned long long)want.audit);
   HB_TRY(ctx->events.get(&b->ev_start));
   HB_TRY(ctx->events.get(&b->ev_end));
   for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));

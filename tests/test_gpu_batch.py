@@ -127,3 +127,68 @@ def test_batch_regrowth_keeps_every_job_exact(ctx):
             tr.release()
         p_eq.release()
         p_sim.release()
+
+
+@pytest.mark.parametrize("shape", [(1, 256, 4), (4, 1024, 6)])
+def test_filter_keeping_most_pairs(ctx, shape):
+    """Join groups of ~650 x 600 rows where the bound filter keeps most pairs
+    (two-letter strings, JW >= 0.5): work-groups fill their survivor buffers
+    many times over and spill to the global list; the pairs still equal the
+    oracle's, and a second batch of the same plan sizes its lists from the
+    first (no rerun)."""
+    L, R = _adversarial_tables(77, 1300, 1200, b"ab", 30, n_keys=2)
+    plan = hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.5)])])
+    old = ctx.filter_shape
+    ctx.filter_shape = shape
+    dp = ctx.plan(plan)
+    tl, tr = ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))
+    try:
+        want, want_audit, _ = H.run_oracle(plan, L, R)
+        first = _results(ctx.submit(dp, tl, tr))
+        second = _results(ctx.submit(dp, tl, tr))
+        assert first[2]["filtered"] > 0.5 * first[2]["candidates"]
+        assert second[2]["retries"] == 0
+        for got in (first, second):
+            H.assert_same_pairs(got[0], want, "JW >= 0.5 on two letters")
+            H.assert_same_audit(got[1], want_audit, "JW >= 0.5 on two letters")
+    finally:
+        ctx.filter_shape = old
+        tl.release()
+        tr.release()
+        dp.release()
+
+
+@pytest.mark.parametrize("rules", [
+    [datagen.rule_yaml([3], [(1, 0.6)])],
+    [datagen.rule_yaml([3, 4], [])],
+    [datagen.rule_yaml([3], [(2, 0.5), (1, 0.55)])],
+])
+def test_many_tiny_join_groups(ctx, rules):
+    """Join keys with ~2,000 values over 3,000 x 2,500 rows: thousands of
+    groups of one or two rows a side (many left keys without a right
+    partner), so the pairs of consecutive groups pack densely into the
+    small-group path, in one job and across the jobs of a batch."""
+    L, R = _adversarial_tables(91, 3000, 2500, b"abc", 16, n_keys=2000)
+    plan = hb.HostPlan.from_yaml(rules)
+    dp = ctx.plan(plan)
+    tl, tr = ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))
+    parts = [(0, 700), (700, 701), (701, 3000)]
+    slices = [tl.slice(b, e) for b, e in parts]
+    try:
+        want, want_audit, _ = H.run_oracle(plan, L, R)
+        got = _results(ctx.submit(dp, tl, tr))
+        H.assert_same_pairs(got[0], want, "tiny groups, one job")
+        H.assert_same_audit(got[1], want_audit, "tiny groups, one job")
+        jobs = ctx.submit_batch([(dp, sl, tr) for sl in slices] + [(dp, tl, tr)])
+        res = [_results(j) for j in jobs]
+        H.assert_same_pairs(res[-1][0], want, "tiny groups, in a batch")
+        for (b, e), r in zip(parts, res):
+            w, wa, _ = H.run_oracle(plan, [c[b:e] for c in L], R)
+            H.assert_same_pairs(r[0], w, "tiny groups, slice [%d, %d)" % (b, e))
+            H.assert_same_audit(r[1], wa, "tiny groups, slice [%d, %d)" % (b, e))
+    finally:
+        for sl in slices:
+            sl.release()
+        tl.release()
+        tr.release()
+        dp.release()
