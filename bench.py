@@ -80,8 +80,8 @@ def copy_calibration(n_bytes):
 
 
 def job_set_bench(args):
-    """Batched (c4) or served (c5) job sets: one JSON line with both the
-    job-set form and plain per-job launches on the same jobs."""
+    """Streamed partitions (c3), batched (c4) or served (c5) job sets: one
+    JSON line; c4 and c5 also time plain per-job launches on the same jobs."""
     import hyperblocker_amd as hb
     from hyperblocker_amd import workloads
 
@@ -89,7 +89,13 @@ def job_set_bench(args):
         print(msg, file=sys.stderr, flush=True)
 
     with hb.Context(0) as ctx:
-        if args.workload == "c4":
+        if args.workload == "c3":
+            r = workloads.run_c3(ctx, log=log)
+            value, desc = r["streamed_pairs_per_s"], (
+                "c3: 1M x 1M rows, 8 columns, 3 rules, %d hash partitions streamed "
+                "(uploads overlapped with compute)" % r["partitions"])
+            ms = r["streamed_s"] * 1e3
+        elif args.workload == "c4":
             r = workloads.run_c4(ctx, n_jobs=args.jobs or 4096, log=log)
             value, desc = r["batched_pairs_per_s"], (
                 "c4: %d jobs, rows per side log-uniform [100, 50k], 1-4 rules each, "
@@ -136,7 +142,7 @@ def main():
     import hyperblocker_amd as hb
     from hyperblocker_amd import datagen
 
-    if args.workload in ("c4", "c5"):
+    if args.workload in ("c3", "c4", "c5"):
         job_set_bench(args)
         return
     L, R, dup, cfg = datagen.make(args.workload)

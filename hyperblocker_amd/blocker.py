@@ -140,11 +140,16 @@ class ColumnTable:
 
 
 def partition(l_table, r_table, plan, n_partitions, prefix_idx=capi.CHECK_POINT):
-    """DataPartitioning (core/components/data_mngr.h:81-222).  Returns a list of
-    (l_rows, r_rows) int64 arrays, empty-side partitions dropped."""
+    """DataPartitioning (core/components/data_mngr.h:81-222) of two HostTables
+    or two ColumnTables.  Returns a list of (l_rows, r_rows) int64 arrays,
+    empty-side partitions dropped."""
     h = C.c_void_p()
-    check(lib().hb_partition(l_table._h, r_table._h, C.byref(plan.view), n_partitions,
-                             prefix_idx, C.byref(h)))
+    if isinstance(l_table, ColumnTable) and isinstance(r_table, ColumnTable):
+        check(lib().hb_partition_views(C.byref(l_table.view), C.byref(r_table.view),
+                                       C.byref(plan.view), n_partitions, prefix_idx, C.byref(h)))
+    else:
+        check(lib().hb_partition(l_table._h, r_table._h, C.byref(plan.view), n_partitions,
+                                 prefix_idx, C.byref(h)))
     try:
         out = []
         for k in range(lib().hb_partitions_count(h)):
@@ -195,6 +200,12 @@ class Context:
 
     def synchronize(self):
         check(lib().hb_ctx_synchronize(self._h))
+
+    def memory(self):
+        """Device memory state (hb_ctx_memory_info), bytes."""
+        m = capi.MemoryInfo()
+        check(lib().hb_ctx_memory_info(self._h, C.byref(m)))
+        return {k: getattr(m, k) for k, _ in m._fields_}
 
     @property
     def filter_shape(self):

@@ -26,7 +26,8 @@ CHECK_POINT = 2147483647
 
 # Every symbol include/hyperblocker_amd.h declares.
 EXPORTS = [
-    "hb_ctx_create", "hb_ctx_destroy", "hb_ctx_synchronize", "hb_ctx_set_filter_shape",
+    "hb_ctx_create", "hb_ctx_destroy", "hb_ctx_synchronize", "hb_ctx_memory_info",
+    "hb_ctx_set_filter_shape",
     "hb_ctx_get_filter_shape", "hb_last_error", "hb_version",
     "hb_table_upload", "hb_table_upload_serialized", "hb_table_upload_rows",
     "hb_table_slice", "hb_table_info", "hb_table_release",
@@ -36,7 +37,7 @@ EXPORTS = [
     "hb_server_create", "hb_server_submit", "hb_server_info", "hb_server_destroy",
     "hb_rules_load", "hb_rules_parse", "hb_host_plan_view", "hb_host_plan_free",
     "hb_csv_load", "hb_host_table_view", "hb_host_table_cell", "hb_host_table_free",
-    "hb_partition", "hb_partitions_count", "hb_partitions_get", "hb_partitions_free",
+    "hb_partition", "hb_partition_views", "hb_partitions_count", "hb_partitions_get", "hb_partitions_free",
     "hb_write_pairs", "hb_filter_coefs", "hb_jw_from_counts",
 ]
 
@@ -78,6 +79,12 @@ class ServerConfig(C.Structure):
                 ("memory_budget", C.c_uint64)]
 
 
+class MemoryInfo(C.Structure):
+    _fields_ = [("device_free", C.c_uint64), ("device_total", C.c_uint64),
+                ("pool_live", C.c_uint64), ("pool_cached", C.c_uint64),
+                ("pinned_cached", C.c_uint64)]
+
+
 class JobStats(C.Structure):
     _fields_ = [("pairs", C.c_uint64), ("candidates", C.c_uint64), ("filtered", C.c_uint64),
                 ("jw_exact", C.c_uint64), ("matches", C.c_uint64), ("audit", C.c_uint64),
@@ -107,6 +114,7 @@ def lib():
         "hb_ctx_create": ([C.c_int32, PVP], st),
         "hb_ctx_destroy": ([VP], st),
         "hb_ctx_synchronize": ([VP], st),
+        "hb_ctx_memory_info": ([VP, C.POINTER(MemoryInfo)], st),
         "hb_ctx_set_filter_shape": ([VP, C.c_int32, C.c_int32, C.c_int32], st),
         "hb_ctx_get_filter_shape": ([VP] + [C.POINTER(C.c_int32)] * 3, st),
         "hb_last_error": ([], C.c_char_p),
@@ -150,6 +158,8 @@ def lib():
         "hb_host_table_cell": ([VP, C.c_int64, C.c_int32], C.c_char_p),
         "hb_host_table_free": ([VP], None),
         "hb_partition": ([VP, VP, C.POINTER(SerializedPlan), C.c_uint64, C.c_int64, PVP], st),
+        "hb_partition_views": ([C.POINTER(TableView), C.POINTER(TableView),
+                                C.POINTER(SerializedPlan), C.c_uint64, C.c_int64, PVP], st),
         "hb_partitions_count": ([VP], C.c_int64),
         "hb_partitions_get": ([VP, C.c_int64, C.POINTER(C.POINTER(C.c_int64)),
                                C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_int64)),

@@ -24,6 +24,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <string_view>
 #include <sys/stat.h>
 #include <unordered_map>
 #include <vector>
@@ -601,6 +602,60 @@ void pack_view(hb_host_table *t) {
 
 }  // namespace
 
+namespace {
+
+// DataPartitioning (data_mngr.h:81-222) over any cell accessor
+// cell(side, row, col) -> std::string_view.  std::hash<std::string_view>
+// equals std::hash<std::string> on the same bytes.
+template <class Cell>
+hb_status partition_rows(int64_t n_l, int32_t cols_l, int64_t n_r, int32_t cols_r,
+                         const hb_serialized_plan *plan, uint64_t n_partitions,
+                         int64_t prefix_idx, Cell cell, hb_partitions **out) {
+  if (!plan || !out) return fail(HB_ERR_INVALID_ARG, "hb_partition: null argument");
+  *out = nullptr;
+  if (n_partitions == 0) return fail(HB_ERR_INVALID_ARG, "n_partitions must be >= 1");
+  const bool use_prefix = prefix_idx != (int64_t)INT_MAX;
+  if (use_prefix && (prefix_idx < 0 || prefix_idx >= cols_l || prefix_idx >= cols_r))
+    return fail(HB_ERR_INVALID_ARG, "prefix_hash_pred_idx %lld out of range",
+                (long long)prefix_idx);
+  // data_mngr.h:152-160: only plan slot 0 is hashed (MAX_HASH_TIMES = 1),
+  // and its (left) column index is used on both sides.
+  const bool hash_slot0 = plan->length > 0 && plan->pred_type[0] == HB_EQUALITIES;
+  const int32_t c0 = hash_slot0 ? plan->pred_index[0] : 0;
+  if (hash_slot0 && (c0 < 0 || c0 >= cols_l || c0 >= cols_r))
+    return fail(HB_ERR_PLAN, "plan slot 0 column %d out of range", c0);
+  const size_t P = (size_t)n_partitions;
+  std::hash<std::string_view> h;
+  auto bucket_of = [&](int side, int64_t row) -> size_t {
+    int bucket_id = 0;
+    if (hash_slot0) {
+      const size_t local = h(cell(side, row, c0)) % P;
+      bucket_id = (int)(((size_t)bucket_id + local) % P);
+    }
+    if (use_prefix) {
+      const std::string_view s = cell(side, row, (int32_t)prefix_idx);
+      if (s.length() > 2) {
+        const size_t two_d = (h(s.substr(0, 1)) + std::hash<int>{}(bucket_id)) % P;
+        bucket_id = (int)(((size_t)bucket_id + two_d) % P);
+      }
+    }
+    return (size_t)bucket_id;
+  };
+  std::vector<std::vector<int64_t>> bl(P), br(P);
+  for (int64_t i = 0; i < n_l; i++) bl[bucket_of(0, i)].push_back(i);
+  for (int64_t i = 0; i < n_r; i++) br[bucket_of(1, i)].push_back(i);
+  auto parts = std::make_unique<hb_partitions>();
+  for (size_t k = 0; k < P; k++) {
+    if (bl[k].empty() || br[k].empty()) continue;  // data_mngr.h:210-211
+    parts->l_rows.push_back(std::move(bl[k]));
+    parts->r_rows.push_back(std::move(br[k]));
+  }
+  *out = parts.release();
+  return HB_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 hb_status hb_rules_parse(int32_t n_docs, const char *const *docs, hb_host_plan **out) {
@@ -695,50 +750,33 @@ const char *hb_host_table_cell(const hb_host_table *t, int64_t row, int32_t col)
 
 void hb_host_table_free(hb_host_table *t) { delete t; }
 
+
 hb_status hb_partition(const hb_host_table *l, const hb_host_table *r,
                        const hb_serialized_plan *plan, uint64_t n_partitions, int64_t prefix_idx,
                        hb_partitions **out) {
-  if (!l || !r || !plan || !out) return fail(HB_ERR_INVALID_ARG, "hb_partition: null argument");
-  *out = nullptr;
-  if (n_partitions == 0) return fail(HB_ERR_INVALID_ARG, "n_partitions must be >= 1");
-  const bool use_prefix = prefix_idx != (int64_t)INT_MAX;
-  if (use_prefix && (prefix_idx < 0 || prefix_idx >= l->n_cols || prefix_idx >= r->n_cols))
-    return fail(HB_ERR_INVALID_ARG, "prefix_hash_pred_idx %lld out of range",
-                (long long)prefix_idx);
-  // data_mngr.h:152-160: only plan slot 0 is hashed (MAX_HASH_TIMES = 1),
-  // and its (left) column index is used on both sides.
-  const bool hash_slot0 = plan->length > 0 && plan->pred_type[0] == HB_EQUALITIES;
-  const int32_t c0 = hash_slot0 ? plan->pred_index[0] : 0;
-  if (hash_slot0 && (c0 >= l->n_cols || c0 >= r->n_cols))
-    return fail(HB_ERR_PLAN, "plan slot 0 column %d out of range", c0);
-  const size_t P = (size_t)n_partitions;
-  std::hash<std::string> h;
-  auto bucket_of = [&](const hb_host_table *t, int64_t row) -> size_t {
-    int bucket_id = 0;
-    if (hash_slot0) {
-      const size_t local = h(t->cells[(size_t)row * t->n_cols + c0]) % P;
-      bucket_id = (int)(((size_t)bucket_id + local) % P);
-    }
-    if (use_prefix) {
-      const std::string &s = t->cells[(size_t)row * t->n_cols + prefix_idx];
-      if (s.length() > 2) {
-        const size_t two_d = (h(s.substr(0, 1)) + std::hash<int>{}(bucket_id)) % P;
-        bucket_id = (int)(((size_t)bucket_id + two_d) % P);
-      }
-    }
-    return (size_t)bucket_id;
+  if (!l || !r) return fail(HB_ERR_INVALID_ARG, "hb_partition: null argument");
+  auto cell = [&](int side, int64_t row, int32_t col) -> std::string_view {
+    const hb_host_table *t = side ? r : l;
+    return t->cells[(size_t)row * t->n_cols + col];
   };
-  std::vector<std::vector<int64_t>> bl(P), br(P);
-  for (int64_t i = 0; i < l->n_rows; i++) bl[bucket_of(l, i)].push_back(i);
-  for (int64_t i = 0; i < r->n_rows; i++) br[bucket_of(r, i)].push_back(i);
-  auto parts = std::make_unique<hb_partitions>();
-  for (size_t k = 0; k < P; k++) {
-    if (bl[k].empty() || br[k].empty()) continue;  // data_mngr.h:210-211
-    parts->l_rows.push_back(std::move(bl[k]));
-    parts->r_rows.push_back(std::move(br[k]));
-  }
-  *out = parts.release();
-  return HB_OK;
+  return partition_rows(l->n_rows, l->n_cols, r->n_rows, r->n_cols, plan, n_partitions,
+                        prefix_idx, cell, out);
+}
+
+hb_status hb_partition_views(const hb_table_view *l, const hb_table_view *r,
+                             const hb_serialized_plan *plan, uint64_t n_partitions,
+                             int64_t prefix_idx, hb_partitions **out) {
+  if (!l || !r) return fail(HB_ERR_INVALID_ARG, "hb_partition_views: null argument");
+  for (const hb_table_view *v : {l, r})
+    if (v->n_rows < 0 || v->n_cols <= 0 || !v->col_base || !v->col_stride)
+      return fail(HB_ERR_INVALID_ARG, "hb_partition_views: bad table view");
+  auto cell = [&](int side, int64_t row, int32_t col) -> std::string_view {
+    const hb_table_view *v = side ? r : l;
+    const char *s = v->col_base[col] + row * v->col_stride[col];
+    return std::string_view(s, strnlen(s, (size_t)v->col_stride[col]));
+  };
+  return partition_rows(l->n_rows, l->n_cols, r->n_rows, r->n_cols, plan, n_partitions,
+                        prefix_idx, cell, out);
 }
 
 int64_t hb_partitions_count(const hb_partitions *p) { return p ? (int64_t)p->l_rows.size() : 0; }

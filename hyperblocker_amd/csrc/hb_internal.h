@@ -9,6 +9,8 @@
 //              by a similarity predicate
 //   code[c]  : uint32 dictionary code per row, built on first use by an
 //              equality predicate; equal codes <=> equal byte strings
+// and, per list of two or more equality columns used together by a rule, a
+// composite code per row (equal <=> every listed cell equal).
 //
 // Work is submitted in batches of jobs (a plain job is a batch of one).  A
 // job-rule (JR) is one rule of one job's plan; every kernel of a batch
@@ -46,7 +48,6 @@ struct __attribute__((aligned(16))) SigRow {
 static_assert(sizeof(SigRow) == 48, "SigRow layout");
 
 constexpr uint32_t kLongFlag = 0x80000000u;
-constexpr int kMaxEq = 4;        // equality predicates per rule beyond the join key
 
 // Bit-planes of a similarity column (positions 0..32*nwc-1, nwc <= 8):
 // uint32 planes[row][w][k], bit b of word (w, k) is bit k of byte 32w+b.
@@ -61,13 +62,13 @@ struct DevColumn {
 };
 
 // One side (0 = left, 1 = right) of a JR, in its join order: position i is
-// row perm[i].  Runs of equal first-equality code: run g covers positions
-// [run_off[g], run_off[g + 1]) and has code run_code[g] (codes ascending; a
-// rule without equalities has a single run of code 0).
+// row perm[i].  Runs of equal join key (the code of the rule's equality
+// column, or the composite code of its equality columns): run g covers
+// positions [run_off[g], run_off[g + 1]) and has code run_code[g] (codes
+// ascending; a rule without equalities has a single run of code 0).
 struct SideDesc {
   const uint32_t *perm;
   const SigRow *jsig;      // signatures of the rule's first similarity column, or null
-  const uint32_t *jcode;   // kMaxEq extra equality codes per position, or null
   const uint32_t *run_code;
   const uint32_t *run_off;  // n_runs + 1 entries
   uint32_t n_runs;
@@ -78,11 +79,10 @@ struct JRDesc {
   SideDesc s[2];
   float c0, c1, c2, c3;  // bound-filter coefficients of the first similarity
   uint32_t job;
-  int32_t n_eq;          // extra equalities (beyond the join key)
   int32_t n_sims;
   uint32_t sim_off;      // first SimDesc of this JR
   uint32_t multi;        // the job has several rules: its final pairs are deduplicated
-  uint32_t pad[3];
+  uint32_t pad[4];
 };
 
 struct SimDesc {
@@ -121,6 +121,7 @@ struct BatchDev {
   unsigned long long *job_cnt;   // kJobCounters * cnt_stride
   JobSets sets;
   unsigned long long *overflow;  // set to 1 when any list or set is full
+  unsigned long long *set_overflow;  // set to 1 when a job's dedupe set is full
 };
 
 // Join groups of a batch: group gi pairs left positions [l_off, l_off + l_cnt)
@@ -131,12 +132,11 @@ struct GroupArrays {
   uint32_t *l_off, *l_cnt, *r_off, *r_cnt;
   uint32_t *jr;            // the group's JR
   uint64_t *small_pairs;   // pairs of groups with <= small_pairs_max pairs, else 0
-  uint64_t *pair_off;      // exclusive scan of small_pairs
   uint64_t *items;         // work items of the other groups, else 0
   uint64_t *item_off;      // exclusive scan of items
   uint32_t tile_a;         // rows of the larger side per work item (256 * rows per lane)
   uint32_t tile_b;         // rows of the smaller side per work item
-  uint32_t small_pairs_max;  // groups with at most this many pairs take the list path
+  uint32_t small_pairs_max;  // groups with at most this many pairs run one wave per group
 };
 
 // Launch shape of the filter stage (chosen per device by the autotuner).
@@ -144,6 +144,7 @@ struct FilterShape {
   int rows_per_lane;   // 1, 2 or 4
   uint32_t tile_b;     // multiple of 256
   unsigned grid_per_cu;
+  uint32_t small_pairs_max = 2048;  // groups up to this many pairs run one wave per group
 };
 
 // Pair list entry: rows l, r of JR jr (in the final list: of job jr).
@@ -191,23 +192,14 @@ void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
 void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
                         const uint32_t *run_id, uint32_t n, uint32_t *run_code,
                         uint32_t *run_off, uint32_t *n_runs, hipStream_t stream);
-// kMaxEq codes per join-order position from up to kMaxEq code columns (three
-// columns are stored as four, the third repeated).
-struct CodeCols {
-  int32_t n;
-  const uint32_t *col[kMaxEq];
-};
-void launch_gather_codes(const uint32_t *perm, uint32_t n, CodeCols cols, uint32_t *jcode,
-                         hipStream_t stream);
-
 // Groups of every JR of a batch (one per left run), with their work.
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream);
 
-// Stage 0 over groups [g_begin, g_end) (JRs of one variant): remaining
-// equalities + bound filter of the first similarity predicate; kept pairs
-// are appended to `out` tagged with their JR.
-void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, int n_eq,
-                   bool has_sim, PairList out, FilterShape shape, int n_cu, hipStream_t stream);
+// Stage 0 over groups [g_begin, g_end) (JRs with, or JRs without, a
+// similarity predicate): bound filter of the first similarity predicate (or
+// every pair); kept pairs are appended to `out` tagged with their JR.
+void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, bool has_sim,
+                   PairList out, FilterShape shape, int n_cu, hipStream_t stream);
 
 // Exact stage `stage` (1-based similarity index) on list `in`: JRs without
 // similarity predicates pass their pairs to the output directly; pairs

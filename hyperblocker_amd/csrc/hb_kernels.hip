@@ -106,51 +106,47 @@ __device__ __forceinline__ unsigned long long job_slot(bool ok, uint32_t job,
   return mine;
 }
 
-// Survivor buffer of one work-group in LDS.  Waves append with one LDS
-// atomic each; the group moves the buffer to the global list with a single
-// global atomic (block_flush), so a busy list counter is not contended by
-// every wave.  Appends past the buffer go to the global list directly, one
-// atomic per wave.
-constexpr uint32_t kBlockOut = 512;
-struct BlockOut {
-  PairEntry e[kBlockOut];
-  uint32_t n;
-  unsigned long long base;
-};
+// Survivor buffer of one wave in LDS, for kernels whose waves run
+// independent loops (no work-group barrier): entries accumulate under a
+// wave-uniform count held in a register, and the wave moves them to the
+// global list with one atomic when the buffer would overflow, and at the end.
+// LDS accesses of one wave complete in order; the wave barriers keep the
+// compiler from moving them across the hand-over between lanes.
+constexpr uint32_t kWaveOut = 128;
 
-__device__ __forceinline__ void block_append(bool ok, uint32_t l, uint32_t r, uint32_t jr,
-                                             BlockOut &bo, const PairList &out,
-                                             unsigned long long *overflow) {
-  const uint64_t mask = __ballot(ok);
-  if (mask == 0) return;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if ((int)lane_id() == leader) base = atomicAdd(&bo.n, (uint32_t)__popcll(mask));
-  base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-  const uint32_t pos = base + rank_in(mask);
-  const PairEntry e{l, r, jr, 0u};
-  if (ok && pos < kBlockOut) bo.e[pos] = e;
-  list_append(ok && pos >= kBlockOut, e, out, overflow);
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Called by every thread of the group.  Leaves bo.n == 0.
-__device__ __forceinline__ void block_flush(BlockOut &bo, const PairList &out,
-                                            unsigned long long *overflow) {
-  __syncthreads();
-  const uint32_t n = min(bo.n, kBlockOut);
-  if (threadIdx.x == 0) bo.base = n ? atomicAdd(out.count, (unsigned long long)n) : 0ull;
-  __syncthreads();
-  const unsigned long long base = bo.base;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+__device__ __forceinline__ void wave_flush(PairEntry *buf, uint32_t &n, const PairList &out,
+                                           unsigned long long *overflow) {
+  if (n == 0) return;
+  wave_sync();
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(out.count, (unsigned long long)n);
+  base = readlane64(base, 0);
+  for (uint32_t i = lane_id(); i < n; i += 64) {
     const unsigned long long pos = base + i;
     if (pos < out.cap)
-      out.e[pos] = bo.e[i];
+      out.e[pos] = buf[i];
     else
       *overflow = 1ull;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) bo.n = 0;
-  __syncthreads();
+  wave_sync();
+  n = 0;
+}
+
+__device__ __forceinline__ void wave_append(bool ok, const PairEntry &e, PairEntry *buf,
+                                            uint32_t &n, const PairList &out,
+                                            unsigned long long *overflow) {
+  const uint64_t mask = __ballot(ok);
+  if (mask == 0) return;
+  const uint32_t c = (uint32_t)__popcll(mask);
+  if (n + c > kWaveOut) wave_flush(buf, n, out, overflow);
+  if (ok) buf[n + rank_in(mask)] = e;
+  n += c;
 }
 
 // Survivor buffer of a filter work item: 4-byte entries a | b << 10, with a
@@ -474,11 +470,10 @@ __device__ __forceinline__ uint32_t find_group(const uint64_t *off, uint32_t n, 
   return lo;
 }
 
-// ---- stage 1: equalities + bound filter over join groups ------------------
+// ---- stage 1: bound filter over join groups ---------------------------------
 
-// Signatures and extra equality codes arrive in join order (the JR's
-// layouts): element i of jsig / jcode belongs to row perm[i], so the rows of
-// a join group are contiguous.  jcode holds kMaxEq codes per row.
+// Signatures arrive in join order (the JR's layouts): element i of jsig
+// belongs to row perm[i], so the rows of a join group are contiguous.
 
 // The bound test of may_pass in allowance form, for a lane-held A row and a
 // B row shared by the wave.  may_pass rejects when
@@ -543,8 +538,8 @@ __device__ __forceinline__ bool bound_keep(const uint32_t (&ah)[8], uint32_t apr
 // LDS round trip feeds 2*AR independent pair tests.  Inside a join group the
 // rows are ordered by length, so a chunk splits into a few runs of equal B
 // length; the per-lane allowances are computed once per run and the rows of
-// a run are tested without further branching.  NEQ = equality predicates
-// checked per pair beyond the join key.
+// a run are tested without further branching.  Every pair of a group
+// already satisfies all of its rule's equalities (they form the join key).
 // B row as staged in LDS: byte-class counters, byte-swapped prefix, length.
 struct __attribute__((aligned(8))) BRow {
   uint32_t h[8];
@@ -552,13 +547,12 @@ struct __attribute__((aligned(8))) BRow {
   uint32_t len;
 };
 
-template <int NEQ, bool SIM, int AR, int BU>
+template <bool SIM, int AR, int BU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6 : 8))) void
 k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_t g_end,
                PairList out) {
   constexpr uint32_t kChunk = 256;
   __shared__ BRow s_b[SIM ? kChunk + 1 : 1];
-  __shared__ uint32_t s_code[NEQ > 0 ? NEQ : 1][kChunk + 1];
   __shared__ unsigned long long s_starts[kChunk / 64];  // bit i: B row i starts a run
   __shared__ float s_rlen[SIM ? kChunk + 1 : 1];
   __shared__ TileOut to;
@@ -581,13 +575,11 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     const uint32_t *perm_b = sb.perm;
     const SigRow *sig_a = sa.jsig + a_off;
     const SigRow *sig_b = sb.jsig + b_off;
-    const uint32_t *code_a = sa.jcode + (uint64_t)a_off * kMaxEq;
-    const uint32_t *code_b = sb.jcode + (uint64_t)b_off * kMaxEq;
     const uint64_t local = item - g.item_off[gi];
     const uint32_t n_bseg = (b_cnt + tile_b - 1) / tile_b;
     const uint32_t a_tile = (uint32_t)(local / n_bseg), b_seg = (uint32_t)(local % n_bseg);
     bool a_valid[AR];
-    uint32_t a_idx[AR], ah[AR][8], apre[AR], la[AR], acode[AR][NEQ > 0 ? NEQ : 1];
+    uint32_t a_idx[AR], ah[AR][8], apre[AR], la[AR];
     uint32_t nal0[AR], nal3[AR];
     float rla[AR];
 #pragma unroll
@@ -608,9 +600,6 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
         la[r] = s.len;
         rla[r] = s.rlen;
       }
-#pragma unroll
-      for (int k = 0; k < NEQ; k++)
-        acode[r][k] = a_valid[r] ? code_a[(uint64_t)a_idx[r] * kMaxEq + k] : 0u;
     }
     const uint32_t b_begin = b_seg * tile_b;
     const uint32_t b_end = min(b_begin + tile_b, b_cnt);
@@ -631,14 +620,8 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
           s_b[tid] = br;
           s_rlen[tid] = bs.rlen;
         }
-#pragma unroll
-        for (int k = 0; k < NEQ; k++) s_code[k][tid] = code_b[(uint64_t)(cb + tid) * kMaxEq + k];
       }
-      if (tid == 0) {  // odd run end: a padding row, masked out below
-        if (SIM) s_b[n_b] = BRow{};
-#pragma unroll
-        for (int k = 0; k < NEQ; k++) s_code[k][n_b] = 0;
-      }
+      if (tid == 0 && SIM) s_b[n_b] = BRow{};  // odd run end: a padding row, masked out below
       __syncthreads();
       if (SIM) {  // run starts: rows whose length differs from the previous row's
         const bool start = tid < n_b && (tid == 0 || s_b[tid - 1].len != my_len);
@@ -677,8 +660,6 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
 #pragma unroll
             for (int r = 0; r < AR; r++) {
               bool o = a_valid[r] && (u == 0 || second);
-#pragma unroll
-              for (int k = 0; k < NEQ; k++) o &= (acode[r][k] == s_code[k][jj + u]);
               if (SIM) o &= bound_keep(ah[r], apre[r], nal0[r], nal3[r], b.h, b.pre);
               ok[u][r] = o;
             }
@@ -705,75 +686,90 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
   }
 }
 
-// Pairs of small groups (at most small_pairs_max pairs each), packed across
-// groups.  A wave takes kSmallPerLane x 64 consecutive pairs; two
-// wave-uniform binary searches bound the groups they span, and each lane
-// finds its pair's group from the previous one (a short walk, or a binary
-// search when the span is wide).  Trip counts are uniform per wave, so the
-// ballots inside see all lanes.
-constexpr uint32_t kSmallPerLane = 8;
+// A pointer the whole wave holds, moved to scalar registers.
+template <class T>
+__device__ __forceinline__ T *wave_ptr(T *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T *)(((uint64_t)hi << 32) | lo);
+}
 
-template <int NEQ, bool SIM>
+// Read-only data the kernel reads at wave-uniform addresses: through the
+// constant address space such loads become scalar loads.
+typedef const __attribute__((address_space(4))) SigRow ConstSigRow;
+
+// Small groups (at most small_pairs_max pairs each): one wave per group,
+// groups taken in a grid-stride loop.  The larger side sits in lanes, 64
+// rows at a time; the other side's rows are read with scalar loads in join
+// order, where they are ordered by length, so the lanes' allowances are
+// recomputed only where the length changes.
+template <bool SIM>
 __global__ __launch_bounds__(256) void k_filter_small(const BatchDev bd, const GroupArrays g,
                                                       uint32_t g_begin, uint32_t g_end,
                                                       PairList out) {
-  __shared__ BlockOut bo;
-  if (threadIdx.x == 0) bo.n = 0;
-  __syncthreads();
-  const uint64_t begin = g.pair_off[g_begin], total = g.pair_off[g_end];
-  const uint32_t n_groups = g_end - g_begin;
+  __shared__ PairEntry s_out[4][kWaveOut];
+  PairEntry *wbuf = s_out[threadIdx.x >> 6];
+  uint32_t wn = 0;  // entries in wbuf (wave-uniform)
+  const uint32_t lane = lane_id();
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const uint64_t n_waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  constexpr uint64_t kSpan = 64 * kSmallPerLane;
-  for (uint64_t q0 = begin + (uint64_t)wave * kSpan; q0 < total; q0 += n_waves * kSpan) {
-    const uint64_t q_last = min(q0 + kSpan, total) - 1;
-    const uint32_t g_lo = g_begin + find_group(g.pair_off + g_begin, n_groups, q0);
-    const uint32_t g_hi = g_begin + find_group(g.pair_off + g_begin, n_groups, q_last);
-    uint32_t gi = g_lo;  // pair_off[gi] <= every q of this lane below
-    for (uint32_t k = 0; k < kSmallPerLane; k++) {
-      const uint64_t q = q0 + k * 64 + lane_id();
-      bool ok = q < total;
-      uint32_t l_row = 0, r_row = 0, li = 0, ri = 0, jr = 0;
-      const JRDesc *d = nullptr;
-      if (ok) {
-        if (g_hi - g_lo <= 8) {
-          while (g.pair_off[gi + 1] <= q) gi++;
-        } else {
-          gi += find_group(g.pair_off + gi, g_hi + 1 - gi, q);
-        }
-        jr = g.jr[gi];
-        d = bd.jrs + jr;
-        const uint32_t local = (uint32_t)(q - g.pair_off[gi]);  // < small_pairs_max
-        const uint32_t r_cnt = g.r_cnt[gi];
-        li = g.l_off[gi] + local / r_cnt;
-        ri = g.r_off[gi] + local % r_cnt;
-        l_row = d->s[0].perm[li];
-        r_row = d->s[1].perm[ri];
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t gi = g_begin + wave; gi < g_end; gi += n_waves) {
+    if (__builtin_amdgcn_readfirstlane((uint32_t)g.small_pairs[gi]) == 0 &&
+        __builtin_amdgcn_readfirstlane((uint32_t)(g.small_pairs[gi] >> 32)) == 0)
+      continue;  // tiled or empty
+    const uint32_t jr = __builtin_amdgcn_readfirstlane(g.jr[gi]);
+    const uint32_t l_off = __builtin_amdgcn_readfirstlane(g.l_off[gi]);
+    const uint32_t l_cnt = __builtin_amdgcn_readfirstlane(g.l_cnt[gi]);
+    const uint32_t r_off = __builtin_amdgcn_readfirstlane(g.r_off[gi]);
+    const uint32_t r_cnt = __builtin_amdgcn_readfirstlane(g.r_cnt[gi]);
+    const JRDesc *d = bd.jrs + jr;
+    const bool a_is_l = l_cnt >= r_cnt;
+    const uint32_t a_cnt = a_is_l ? l_cnt : r_cnt, b_cnt = a_is_l ? r_cnt : l_cnt;
+    const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
+    const SideDesc &sa = d->s[a_is_l ? 0 : 1], &sb = d->s[a_is_l ? 1 : 0];
+    const uint32_t *perm_a = wave_ptr(sa.perm) + a_off;
+    const uint32_t *perm_b = wave_ptr(sb.perm) + b_off;
+    const SigRow *sig_a = SIM ? wave_ptr(sa.jsig) + a_off : nullptr;
+    ConstSigRow *sig_b = SIM ? (ConstSigRow *)(wave_ptr(sb.jsig) + b_off) : nullptr;
+    const float k0 = 2.0f * d->c0, k3 = 2.0f * d->c3;
+    for (uint32_t a0 = 0; a0 < a_cnt; a0 += 64) {
+      const uint32_t ai = a0 + lane;
+      const bool av = ai < a_cnt;
+      uint32_t ah[8] = {0, 0, 0, 0, 0, 0, 0, 0}, apre = 0, la = 0;
+      float rla = INFINITY;
+      if (SIM && av) {
+        const SigRow s = sig_a[ai];
 #pragma unroll
-        for (int e = 0; e < NEQ; e++)
-          ok = ok && (d->s[0].jcode[(uint64_t)li * kMaxEq + e] ==
-                      d->s[1].jcode[(uint64_t)ri * kMaxEq + e]);
+        for (int k = 0; k < 8; k++) ah[k] = s.h[k];
+        apre = __builtin_bswap32(s.pre);
+        la = s.len;
+        rla = s.rlen;
       }
-      if (SIM && ok) {
-        const SigRow sa = d->s[0].jsig[li], sb = d->s[1].jsig[ri];
-        ok = may_pass(sa.h, sa.pre, sa.len, sb.h, sb.pre, sb.len, d->c0, d->c1, d->c2, d->c3);
+      uint32_t cur_len = 0xffffffffu, nal0 = (uint32_t)(-kPassAllow), nal3 = nal0;
+      for (uint32_t bj = 0; bj < b_cnt; bj++) {
+        bool ok = av;
+        if (SIM) {
+          uint32_t bh[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) bh[k] = sig_b[bj].h[k];
+          const uint32_t blen = sig_b[bj].len;
+          if (blen != cur_len) {
+            cur_len = blen;
+            allowances(la, rla, blen, sig_b[bj].rlen, k0, k3, nal0, nal3);
+          }
+          ok = ok && bound_keep(ah, apre, nal0, nal3, bh, __builtin_bswap32(sig_b[bj].pre));
+        }
+        if (__builtin_amdgcn_ballot_w64(ok)) {
+          const uint32_t b_row = perm_b[bj];
+          const uint32_t a_row = ok ? perm_a[ai] : 0u;
+          wave_append(ok, PairEntry{a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, jr, 0u},
+                      wbuf, wn, out, bd.overflow);
+        }
       }
-      block_append(ok, l_row, r_row, jr, bo, out, bd.overflow);
     }
   }
-  block_flush(bo, out, bd.overflow);  // every wave of the group reaches this point
-}
-
-// kMaxEq codes per join-order position from the rule's extra equality
-// columns; three columns are stored as four, the third repeated, so the
-// filter can check them as four.
-__global__ void k_gather_codes(const uint32_t *__restrict__ perm, uint32_t n, CodeCols cols,
-                               uint32_t *__restrict__ jcode) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t row = perm[i];
-  for (int k = 0; k < cols.n; k++) jcode[(uint64_t)i * kMaxEq + k] = cols.col[k][row];
-  if (cols.n == 3) jcode[(uint64_t)i * kMaxEq + 3] = jcode[(uint64_t)i * kMaxEq + 2];
+  wave_flush(wbuf, wn, out, bd.overflow);
 }
 
 // ---- exact Jaro-Winkler -----------------------------------------------------
@@ -873,13 +869,18 @@ __device__ bool set_insert(const BatchDev &b, uint32_t job, uint32_t l, uint32_t
   unsigned long long *slots = b.sets.slots + b.sets.off[job];
   const unsigned long long key = ((unsigned long long)l << 32) | r;
   unsigned long long slot = mix64(key) & (cap - 1);
-  for (unsigned long long probe = 0; probe < cap; probe++) {
+  // Sets are sized for a load of at most one half: a probe sequence of 512
+  // slots means the set is (nearly) full, and the batch reruns with a larger
+  // one.
+  const unsigned long long max_probe = cap < 512 ? cap : 512;
+  for (unsigned long long probe = 0; probe < max_probe; probe++) {
     const unsigned long long prev = atomicCAS(slots + slot, kEmpty, key);
     if (prev == kEmpty) return true;
     if (prev == key) return false;
     slot = (slot + 1) & (cap - 1);
   }
   *b.overflow = 1ull;
+  *b.set_overflow = 1ull;
   return false;
 }
 
@@ -1248,57 +1249,37 @@ void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
                      flags, run_id, n, run_code, run_off, n_runs);
 }
 
-void launch_gather_codes(const uint32_t *perm, uint32_t n, CodeCols cols, uint32_t *jcode,
-                         hipStream_t stream) {
-  if (!n || cols.n == 0) return;
-  hipLaunchKernelGGL(k_gather_codes, dim3(grid_for(n, 256)), dim3(256), 0, stream, perm, n, cols,
-                     jcode);
-}
-
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream) {
   hipLaunchKernelGGL(k_batch_join, dim3(grid_for((uint64_t)g.n + 1, 256)), dim3(256), 0, stream,
                      b, g);
 }
 
-void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, int n_eq,
-                   bool has_sim, PairList out, FilterShape shape, int n_cu, hipStream_t stream) {
+void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, bool has_sim,
+                   PairList out, FilterShape shape, int n_cu, hipStream_t stream) {
   if (g_end <= g_begin) return;
   // The tile grid launches grid_per_cu work-groups per CU (16 by default,
   // two rounds of the 8 that fit at once), each taking work items in a
   // grid-stride loop; surplus groups find no item and exit.
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
-  const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 4;
-#define HB_FILTER(NEQ, SIM)                                                                   \
-  do {                                                                                        \
-    if (shape.rows_per_lane == 4)                                                             \
-      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 4, 1>), dim3(grid), dim3(256), 0, stream,  \
-                         b, g, g_begin, g_end, out);                                          \
-    else if (shape.rows_per_lane == 2)                                                        \
-      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 2, 2>), dim3(grid), dim3(256), 0, stream,  \
-                         b, g, g_begin, g_end, out);                                          \
-    else                                                                                      \
-      hipLaunchKernelGGL((k_filter_tiles<NEQ, SIM, 1, 2>), dim3(grid), dim3(256), 0, stream,  \
-                         b, g, g_begin, g_end, out);                                          \
-    hipLaunchKernelGGL((k_filter_small<NEQ, SIM>), dim3(grid_small), dim3(256), 0, stream, b, \
-                       g, g_begin, g_end, out);                                               \
+  const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 8;
+#define HB_FILTER(SIM)                                                                       \
+  do {                                                                                       \
+    if (shape.rows_per_lane == 4)                                                            \
+      hipLaunchKernelGGL((k_filter_tiles<SIM, 4, 1>), dim3(grid), dim3(256), 0, stream, b, g, \
+                         g_begin, g_end, out);                                               \
+    else if (shape.rows_per_lane == 2)                                                       \
+      hipLaunchKernelGGL((k_filter_tiles<SIM, 2, 2>), dim3(grid), dim3(256), 0, stream, b, g, \
+                         g_begin, g_end, out);                                               \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_filter_tiles<SIM, 1, 2>), dim3(grid), dim3(256), 0, stream, b, g, \
+                         g_begin, g_end, out);                                               \
+    hipLaunchKernelGGL((k_filter_small<SIM>), dim3(grid_small), dim3(256), 0, stream, b, g,   \
+                       g_begin, g_end, out);                                                 \
   } while (0)
-  // Three extra equalities run as four: the gathered codes repeat the third.
-  const int neq = n_eq == 3 ? 4 : n_eq;
-  if (has_sim) {
-    switch (neq) {
-    case 0: HB_FILTER(0, true); break;
-    case 1: HB_FILTER(1, true); break;
-    case 2: HB_FILTER(2, true); break;
-    default: HB_FILTER(4, true); break;
-    }
-  } else {
-    switch (neq) {
-    case 0: HB_FILTER(0, false); break;
-    case 1: HB_FILTER(1, false); break;
-    case 2: HB_FILTER(2, false); break;
-    default: HB_FILTER(4, false); break;
-    }
-  }
+  if (has_sim)
+    HB_FILTER(true);
+  else
+    HB_FILTER(false);
 #undef HB_FILTER
 }
 

@@ -76,6 +76,7 @@ class DevicePool {
       if (it != free_.end() && !it->second.empty()) {
         *out = it->second.back();
         it->second.pop_back();
+        cached_ -= bytes;
         return HB_OK;
       }
     }
@@ -89,19 +90,42 @@ class DevicePool {
         return fail(HB_ERR_OOM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     }
     *out = p;
+    live_ += bytes;
     return HB_OK;
   }
+  // Blocks beyond the cache limit go back to the driver (hipFree waits for
+  // the device, so this is for the rare very large block).
   void free(void *p, size_t bytes) {
     if (!p) return;
-    std::lock_guard<std::mutex> lk(mu_);
-    free_[round(bytes)].push_back(p);
+    bytes = round(bytes);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (bytes <= limit_ / 4 && cached_ + bytes <= limit_) {
+        free_[bytes].push_back(p);
+        cached_ += bytes;
+        return;
+      }
+    }
+    (void)hipFree(p);
+    live_ -= bytes;
   }
   void release_all() {
     std::lock_guard<std::mutex> lk(mu_);
     for (auto &kv : free_)
       for (void *p : kv.second) (void)hipFree(p);
     free_.clear();
+    live_ -= cached_;
+    cached_ = 0;
   }
+  void set_limit(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    limit_ = bytes;
+  }
+  size_t cached() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cached_;
+  }
+  size_t live() const { return live_.load(); }
 
  private:
   static size_t round(size_t b) {
@@ -111,6 +135,9 @@ class DevicePool {
     return p;
   }
   std::mutex mu_;
+  size_t limit_ = 8ull << 30;  // cached (free) bytes kept at most
+  size_t cached_ = 0;
+  std::atomic<size_t> live_{0};  // bytes obtained from hipMalloc and not yet freed
   std::unordered_map<size_t, std::vector<void *>> free_;
 };
 
@@ -129,6 +156,7 @@ class HostPool {
       if (it != free_.end() && !it->second.empty()) {
         p = it->second.back();
         it->second.pop_back();
+        cached_ -= bytes;
       }
     }
     if (!p) {
@@ -150,16 +178,26 @@ class HostPool {
     }
     return HB_OK;
   }
+  // Blocks beyond the cache limit are unpinned and returned at once.
   void free(void *p, size_t bytes) {
     if (!p) return;
-    std::lock_guard<std::mutex> lk(mu_);
-    free_[round(bytes)].push_back(p);
+    bytes = round(bytes);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (bytes <= kLimit / 4 && cached_ + bytes <= kLimit) {
+        free_[bytes].push_back(p);
+        cached_ += bytes;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
   }
   void release_all() {
     std::lock_guard<std::mutex> lk(mu_);
     for (auto &kv : free_)
       for (void *p : kv.second) (void)hipHostFree(p);
     free_.clear();
+    cached_ = 0;
   }
 
  private:
@@ -168,7 +206,17 @@ class HostPool {
     while (p < b) p <<= 1;
     return p;
   }
+  static constexpr size_t kLimit = 4ull << 30;  // cached (free) pinned bytes kept at most
+
+ public:
+  size_t cached() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return cached_;
+  }
+
+ private:
   std::mutex mu_;
+  size_t cached_ = 0;
   std::unordered_map<size_t, std::vector<void *>> free_;
 };
 
@@ -203,35 +251,70 @@ class EventPool {
 
 // Byte-string -> code dictionary shared by every table of a context, so that
 // codes from any two columns compare equal exactly when the strings do.
+// Dictionary codes of equality cells, shared by every table of a context so
+// that equal byte strings get equal codes across tables.  Sharded by hash,
+// each shard with its own lock, so that many threads code one column at
+// once; code = shard + kShards * (index within the shard).
 class Dictionary {
  public:
-  size_t size() const { return map_.size(); }
+  static constexpr uint32_t kShards = 64;
+  // One more than the largest code handed out so far.
+  uint64_t code_bound() const { return bound_.load(); }
   uint32_t code(std::string_view s) {
-    auto it = map_.find(s);
-    if (it != map_.end()) return it->second;
-    char *dst = store(s);
-    uint32_t c = (uint32_t)map_.size();
-    map_.emplace(std::string_view(dst, s.size()), c);
+    const size_t h = std::hash<std::string_view>{}(s);
+    Shard &sh = shards_[h % kShards];
+    std::lock_guard<std::mutex> lk(sh.mu);
+    auto it = sh.map.find(s);
+    if (it != sh.map.end()) return it->second;
+    char *dst = sh.store(s);
+    const uint32_t c = (uint32_t)(h % kShards) + kShards * (uint32_t)sh.map.size();
+    sh.map.emplace(std::string_view(dst, s.size()), c);
+    uint64_t b = bound_.load();
+    while (b < (uint64_t)c + 1 && !bound_.compare_exchange_weak(b, (uint64_t)c + 1)) {
+    }
     return c;
   }
-  std::mutex mu;
 
  private:
-  char *store(std::string_view s) {
-    if (blocks_.empty() || used_ + s.size() > kBlock) {
-      blocks_.emplace_back(new char[std::max(kBlock, s.size())]);
-      used_ = 0;
+  struct Shard {
+    std::mutex mu;
+    std::unordered_map<std::string_view, uint32_t> map;
+    std::vector<std::unique_ptr<char[]>> blocks;
+    size_t used = 0;
+    char *store(std::string_view s) {
+      if (blocks.empty() || used + s.size() > kBlock) {
+        blocks.emplace_back(new char[std::max(kBlock, s.size())]);
+        used = 0;
+      }
+      char *dst = blocks.back().get() + used;
+      if (!s.empty()) std::memcpy(dst, s.data(), s.size());
+      used += s.size();
+      return dst;
     }
-    char *dst = blocks_.back().get() + used_;
-    if (!s.empty()) std::memcpy(dst, s.data(), s.size());
-    used_ += s.size();
-    return dst;
-  }
-  static constexpr size_t kBlock = 1 << 20;
-  std::unordered_map<std::string_view, uint32_t> map_;
-  std::vector<std::unique_ptr<char[]>> blocks_;
-  size_t used_ = 0;
+  };
+  static constexpr size_t kBlock = 1 << 16;
+  Shard shards_[kShards];
+  std::atomic<uint64_t> bound_{1};
 };
+
+// Runs fn(begin, end) over [0, n) in chunks on up to 16 threads (one when n
+// is below min_chunk).
+template <class F>
+void parallel_for(int64_t n, int64_t min_chunk, F fn) {
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  const int64_t T = std::min<int64_t>({hw, 16, std::max<int64_t>(1, n / std::max<int64_t>(min_chunk, 1))});
+  if (T <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int64_t step = (n + T - 1) / T;
+  for (int64_t k = 0; k < T; k++) {
+    const int64_t b = k * step, e = std::min(n, b + step);
+    if (b < e) th.emplace_back([=] { fn(b, e); });
+  }
+  for (auto &t : th) t.join();
+}
 
 }  // namespace hbrt
 
@@ -254,6 +337,15 @@ struct hb_ctx {
   Dictionary dict;
   std::mutex mu;  // jobs map, shape
   std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
+  // Memory of released tables, returned to the pools once every stream has
+  // passed the point where the table was released.
+  struct Grave {
+    std::vector<std::pair<void *, size_t>> dev, host;
+    std::vector<hipEvent_t> owned;  // the table's own events, destroyed with it
+    std::vector<hipEvent_t> wait;   // from the event pool
+  };
+  std::mutex grave_mu;
+  std::deque<Grave> graves;
 };
 
 namespace {
@@ -280,11 +372,19 @@ struct Column {
   uint32_t *h_code = nullptr;  // pinned
 };
 
-// The rows of a table in join order for one rule shape: sorted by the code
-// of its first equality column, then by the length of its first similarity
-// column (either may be absent, -1), with the runs of equal code and, per
-// list of extra equality columns, their codes gathered into that order.
-// Kept for the table's lifetime.
+// Join key of a table for a list of two or more equality columns: one
+// dictionary code per row for the row's cells of those columns together.
+struct Composite {
+  Lazy code;
+  uint32_t *d_code = nullptr;
+  uint32_t *h_code = nullptr;  // pinned
+};
+
+// The rows of a table in join order for one rule shape: sorted by the join
+// key (the code of the rule's equality column, or the composite code of its
+// equality columns), then by the length of its first similarity column
+// (either may be absent), with the runs of equal key.  Kept for the table's
+// lifetime.
 struct Layout {
   Lazy lz;
   uint64_t *d_keys = nullptr;     // sorted (code << 8) | min(length, 255)
@@ -293,7 +393,6 @@ struct Layout {
   uint32_t *d_run_code = nullptr;
   uint32_t *d_run_off = nullptr;  // n_runs + 1
   uint32_t n_runs = 0;
-  std::map<std::vector<int>, uint32_t *> jcode;  // extra equality columns -> codes
 };
 
 }  // namespace
@@ -303,7 +402,8 @@ struct hb_table {
   int64_t n_rows = 0;
   int32_t n_cols = 0;
   std::vector<Column> cols;
-  std::map<std::pair<int, int>, Layout> layouts;  // (equality column, similarity column)
+  std::map<std::pair<std::vector<int>, int>, Layout> layouts;  // (equality columns, similarity column)
+  std::map<std::vector<int>, Composite> composites;             // (equality columns)
   hipEvent_t ready = nullptr;
   std::mutex mu;  // guards lazy builds
   // A slice borrows rows [row_off, row_off + n_rows) of its parent's column
@@ -311,7 +411,75 @@ struct hb_table {
   hb_table *parent = nullptr;
   int64_t row_off = 0;
   std::atomic<int> slices{0};  // live slices of this table
+  // Pool blocks owned by this table (a slice: only its layouts').
+  std::vector<std::pair<void *, size_t>> dev_mem, host_mem;
+
+  hb_status dev_alloc(size_t bytes, void **p);
+  hb_status host_alloc(size_t bytes, void **h, void **d = nullptr);
 };
+
+namespace {
+
+// Returns the memory of released tables whose release point every stream
+// has passed (of all of them, waiting, when `wait` is set).
+hb_status reap_graves(hb_ctx *ctx, bool wait) {
+  std::vector<hb_ctx::Grave> done;
+  {
+    std::lock_guard<std::mutex> lk(ctx->grave_mu);
+    for (auto it = ctx->graves.begin(); it != ctx->graves.end();) {
+      bool ready = true;
+      for (hipEvent_t e : it->wait) {
+        if (wait) {
+          HIP_OK(hipEventSynchronize(e));
+          continue;
+        }
+        const hipError_t q = hipEventQuery(e);
+        if (q == hipErrorNotReady) {
+          ready = false;
+          break;
+        }
+        if (q != hipSuccess) return fail(HB_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+      }
+      if (ready) {
+        done.push_back(std::move(*it));
+        it = ctx->graves.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (auto &g : done) {
+    for (auto &m : g.dev) ctx->pool.free(m.first, m.second);
+    for (auto &m : g.host) ctx->hpool.free(m.first, m.second);
+    for (hipEvent_t e : g.wait) ctx->events.put(e);
+    for (hipEvent_t e : g.owned) (void)hipEventDestroy(e);
+  }
+  return HB_OK;
+}
+
+}  // namespace
+
+hb_status hb_table::dev_alloc(size_t bytes, void **p) {
+  hb_status st = ctx->pool.alloc(bytes, p);
+  if (st == HB_ERR_OOM) {  // released tables' memory may still be pending
+    HB_TRY(reap_graves(ctx, true));
+    st = ctx->pool.alloc(bytes, p);
+  }
+  HB_TRY(st);
+  dev_mem.push_back({*p, bytes});
+  return HB_OK;
+}
+
+hb_status hb_table::host_alloc(size_t bytes, void **h, void **d) {
+  hb_status st = ctx->hpool.alloc(bytes, h, d);
+  if (st == HB_ERR_OOM) {
+    HB_TRY(reap_graves(ctx, true));
+    st = ctx->hpool.alloc(bytes, h, d);
+  }
+  HB_TRY(st);
+  host_mem.push_back({*h, bytes});
+  return HB_OK;
+}
 
 struct PlanRule {
   std::vector<std::pair<int32_t, int32_t>> eq;
@@ -328,6 +496,8 @@ struct PlanRule {
 // batches reserve more instead of running twice).
 struct SizeHint {
   double list = 1.0, defer = 1.0 / 64, final_ = 0.5, audit = 1.0 / 64;
+  // largest list and final list one batch of the plan needed (entries)
+  double max_list = 0, max_final = 0;
 };
 
 struct hb_plan {
@@ -350,6 +520,8 @@ struct hb_plan {
     hint.defer = std::max(hint.defer, h.defer);
     hint.final_ = std::max(hint.final_, h.final_);
     hint.audit = std::max(hint.audit, h.audit);
+    hint.max_list = std::max(hint.max_list, h.max_list);
+    hint.max_final = std::max(hint.max_final, h.max_final);
   }
 };
 
@@ -364,6 +536,7 @@ enum : int {
   kCntOverflow = 0,
   kCntFinal = 1,
   kCntAudit = 2,
+  kCntSetOverflow = 3,
   kCntFixed = 4,
 };
 
@@ -399,6 +572,11 @@ struct Batch {
   int cnt_stage_off = 0, cnt_job_off = 0, cnt_out_off = 0, cnt_audit_off = 0;
   uint32_t *h_out = nullptr;           // pinned, job-contiguous (l, r) pairs
   hb_audit_entry *h_audit = nullptr;   // pinned, job-contiguous audit entries
+  // After completion: the counters and results, moved out of pinned memory
+  // (h_cnt, h_out and h_audit then point here).
+  std::vector<unsigned long long> res_cnt;
+  std::vector<uint32_t> res_pairs;
+  std::vector<hb_audit_entry> res_audit;
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
   hipEvent_t marks[3] = {nullptr, nullptr, nullptr};  // after join, filter, exact
   float join_ms = 0, filter_ms = 0, exact_ms = 0, device_ms = 0;
@@ -491,10 +669,10 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   std::lock_guard<std::mutex> lk(t->mu);
   if (col.sig.built) return wait_lazy(col.sig, stream);
   const size_t n = (size_t)std::max<int64_t>(t->n_rows, 1);
-  HIP_OK(hipMalloc(&col.d_len, n * sizeof(uint16_t)));
-  HIP_OK(hipMalloc(&col.d_sig, n * sizeof(hb::SigRow)));
+  HB_TRY(t->dev_alloc(n * sizeof(uint16_t), (void **)&col.d_len));
+  HB_TRY(t->dev_alloc(n * sizeof(hb::SigRow), (void **)&col.d_sig));
   col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
-  HIP_OK(hipMalloc(&col.d_planes, n * col.nwc * 32));
+  HB_TRY(t->dev_alloc(n * col.nwc * 32, (void **)&col.d_planes));
   hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
   hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
   HIP_OK(hipGetLastError());
@@ -517,20 +695,71 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   std::lock_guard<std::mutex> lk(t->mu);
   if (col.code.built) return wait_lazy(col.code, stream);
   const size_t n = (size_t)t->n_rows;
-  HIP_OK(hipHostMalloc((void **)&col.h_code, std::max<size_t>(n, 1) * sizeof(uint32_t),
-                       hipHostMallocDefault));
+  HB_TRY(t->host_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.h_code));
   {
     Dictionary &d = t->ctx->dict;
-    std::lock_guard<std::mutex> dl(d.mu);
-    for (size_t r = 0; r < n; r++) {
-      const char *s = col.h_bytes + r * col.width;
-      col.h_code[r] = d.code(std::string_view(s, strnlen(s, col.width)));
-    }
+    const char *bytes = col.h_bytes;
+    uint32_t *codes = col.h_code;
+    const uint32_t w = col.width;
+    parallel_for((int64_t)n, 1 << 15, [&](int64_t b, int64_t e) {
+      for (int64_t r = b; r < e; r++) {
+        const char *s = bytes + (size_t)r * w;
+        codes[r] = d.code(std::string_view(s, strnlen(s, w)));
+      }
+    });
   }
-  HIP_OK(hipMalloc(&col.d_code, std::max<size_t>(n, 1) * sizeof(uint32_t)));
+  HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.d_code));
   if (n) HIP_OK(hipMemcpyAsync(col.d_code, col.h_code, n * sizeof(uint32_t),
                                hipMemcpyHostToDevice, stream));
   return record_lazy(col.code, stream);
+}
+
+// Join key codes of the equality columns `cols` (one column: its codes;
+// several: the composite codes, built from the columns' codes on the host,
+// shared by every table of the context through the dictionary).
+hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stream,
+                     const uint32_t **out) {
+  if (cols.size() == 1) {
+    HB_TRY(ensure_code(t, cols[0], stream));
+    *out = t->cols[cols[0]].d_code;
+    return HB_OK;
+  }
+  if (t->parent) {  // borrow the parent's composite codes at the slice's offset
+    const uint32_t *pc = nullptr;
+    HB_TRY(ensure_key(t->parent, cols, stream, &pc));
+    *out = pc + t->row_off;
+    return HB_OK;
+  }
+  for (int c : cols) HB_TRY(ensure_code(t, c, stream));
+  std::lock_guard<std::mutex> lk(t->mu);
+  Composite &cp = t->composites[cols];
+  if (cp.code.built) {
+    *out = cp.d_code;
+    return wait_lazy(cp.code, stream);
+  }
+  const size_t n = (size_t)t->n_rows;
+  HB_TRY(t->host_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.h_code));
+  {
+    Dictionary &d = t->ctx->dict;
+    std::vector<const uint32_t *> src;
+    for (int c : cols) src.push_back(t->cols[c].h_code);
+    uint32_t *codes = cp.h_code;
+    const size_t k = cols.size();
+    parallel_for((int64_t)n, 1 << 15, [&](int64_t b, int64_t e) {
+      // key bytes: a marker byte, then the column codes (4 bytes each)
+      std::string key(1 + 4 * k, '\0');
+      key[0] = (char)(0xC0 + k);
+      for (int64_t r = b; r < e; r++) {
+        for (size_t q = 0; q < k; q++) std::memcpy(&key[1 + 4 * q], &src[q][r], 4);
+        codes[r] = d.code(key);
+      }
+    });
+  }
+  HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.d_code));
+  if (n) HIP_OK(hipMemcpyAsync(cp.d_code, cp.h_code, n * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, stream));
+  *out = cp.d_code;
+  return record_lazy(cp.code, stream);
 }
 
 // Builds (once per table and rule shape) the join-order layout: keys from the
@@ -541,7 +770,8 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
 // The build runs on the context's build stream, which the host waits for
 // (the run count sizes the batch's groups); jobs' streams wait on the
 // layout's event, so a build never waits for other batches' kernels.
-hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t caller, Layout **out) {
+hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStream_t caller,
+                        Layout **out) {
   {
     std::lock_guard<std::mutex> lk(t->mu);
     auto it = t->layouts.find({eq, sim});
@@ -552,34 +782,33 @@ hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t caller, Layout
   }
   const hipStream_t stream = t->ctx->build_stream;
   HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
-  if (eq >= 0) HB_TRY(ensure_code(t, eq, stream));
+  const uint32_t *key_codes = nullptr;
+  if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes));
   if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
   uint64_t n_codes = 1;
-  if (eq >= 0) {
-    Dictionary &d = t->ctx->dict;
-    std::lock_guard<std::mutex> dl(d.mu);
-    n_codes = std::max<uint64_t>(d.size(), 1);
-  }
+  if (!eq.empty()) n_codes = t->ctx->dict.code_bound();
   std::lock_guard<std::mutex> lk(t->mu);
   Layout &ly = t->layouts[{eq, sim}];
   *out = &ly;
   if (ly.lz.built) return wait_lazy(ly.lz, caller);
   const uint32_t n = (uint32_t)t->n_rows;
   const size_t n1 = std::max<size_t>(n, 1);
-  HIP_OK(hipMalloc(&ly.d_keys, n1 * sizeof(uint64_t)));
-  HIP_OK(hipMalloc(&ly.d_perm, n1 * sizeof(uint32_t)));
-  HIP_OK(hipMalloc(&ly.d_run_code, n1 * sizeof(uint32_t)));
-  HIP_OK(hipMalloc(&ly.d_run_off, (n1 + 1) * sizeof(uint32_t)));
-  if (sim >= 0) HIP_OK(hipMalloc(&ly.d_jsig, n1 * sizeof(hb::SigRow)));
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint64_t), (void **)&ly.d_keys));
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_perm));
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_run_code));
+  HB_TRY(t->dev_alloc((n1 + 1) * sizeof(uint32_t), (void **)&ly.d_run_off));
+  if (sim >= 0) HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig));
   if (n) {
+    DevicePool &pool = t->ctx->pool;
     uint64_t *keys_in = nullptr;
     uint32_t *iota = nullptr, *flags = nullptr, *run_id = nullptr, *h_nruns = nullptr;
-    HIP_OK(hipMalloc(&keys_in, n1 * sizeof(uint64_t)));
-    HIP_OK(hipMalloc(&iota, n1 * sizeof(uint32_t)));
-    HIP_OK(hipMalloc(&flags, n1 * sizeof(uint32_t)));
-    HIP_OK(hipMalloc(&run_id, (n1 + 1) * sizeof(uint32_t)));
-    HIP_OK(hipHostMalloc((void **)&h_nruns, sizeof(uint32_t), hipHostMallocMapped));
-    hb::launch_layout_keys(eq >= 0 ? t->cols[eq].d_code : nullptr,
+    uint32_t *d_nruns = nullptr;
+    HB_TRY(pool.alloc(n1 * sizeof(uint64_t), (void **)&keys_in));
+    HB_TRY(pool.alloc(n1 * sizeof(uint32_t), (void **)&iota));
+    HB_TRY(pool.alloc(n1 * sizeof(uint32_t), (void **)&flags));
+    HB_TRY(pool.alloc((n1 + 1) * sizeof(uint32_t), (void **)&run_id));
+    HB_TRY(t->ctx->hpool.alloc(sizeof(uint32_t), (void **)&h_nruns, (void **)&d_nruns));
+    hb::launch_layout_keys(key_codes,
                            sim >= 0 ? t->cols[sim].d_len : nullptr, n, keys_in, stream);
     hb::launch_iota(iota, n, stream);
     int end_bit = 8;
@@ -589,50 +818,26 @@ hb_status ensure_layout(hb_table *t, int eq, int sim, hipStream_t caller, Layout
                               stream));
     HIP_OK(hb::exscan_u32(nullptr, ts, flags, run_id, n, stream));
     void *tmp = nullptr;
-    HIP_OK(hipMalloc(&tmp, std::max<size_t>(std::max(tb, ts), 16)));
+    const size_t tmp_bytes = std::max<size_t>(std::max(tb, ts), 16);
+    HB_TRY(pool.alloc(tmp_bytes, &tmp));
     HIP_OK(hb::sort_pairs_u64(tmp, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit, stream));
     hb::launch_run_flags(ly.d_keys, n, flags, stream);
     HIP_OK(hb::exscan_u32(tmp, ts, flags, run_id, n, stream));
-    uint32_t *d_nruns = nullptr;
-    HIP_OK(hipHostGetDevicePointer((void **)&d_nruns, h_nruns, 0));
     hb::launch_layout_runs(ly.d_keys, flags, run_id, n, ly.d_run_code, ly.d_run_off, d_nruns,
                            stream);
     if (sim >= 0) hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(stream));  // one-off per layout: run count, frees below
     ly.n_runs = *h_nruns;
-    HIP_OK(hipFree(tmp));
-    HIP_OK(hipFree(keys_in));
-    HIP_OK(hipFree(iota));
-    HIP_OK(hipFree(flags));
-    HIP_OK(hipFree(run_id));
-    HIP_OK(hipHostFree(h_nruns));
+    pool.free(tmp, tmp_bytes);
+    pool.free(keys_in, n1 * sizeof(uint64_t));
+    pool.free(iota, n1 * sizeof(uint32_t));
+    pool.free(flags, n1 * sizeof(uint32_t));
+    pool.free(run_id, (n1 + 1) * sizeof(uint32_t));
+    t->ctx->hpool.free(h_nruns, sizeof(uint32_t));
   }
   HB_TRY(record_lazy(ly.lz, stream));
   return wait_lazy(ly.lz, caller);
-}
-
-// Extra equality codes of `cols` in the layout's order (built once).
-hb_status ensure_jcode(hb_table *t, Layout *ly, const std::vector<int> &cols, hipStream_t stream,
-                       const uint32_t **out) {
-  for (int c : cols) HB_TRY(ensure_code(t, c, stream));
-  std::lock_guard<std::mutex> lk(t->mu);
-  auto it = ly->jcode.find(cols);
-  if (it != ly->jcode.end()) {
-    *out = it->second;
-    return HB_OK;
-  }
-  const size_t n1 = (size_t)std::max<int64_t>(t->n_rows, 1);
-  uint32_t *p = nullptr;
-  HIP_OK(hipMalloc(&p, n1 * hb::kMaxEq * sizeof(uint32_t)));
-  hb::CodeCols cc{};
-  cc.n = (int32_t)cols.size();
-  for (size_t k = 0; k < cols.size(); k++) cc.col[k] = t->cols[cols[k]].d_code;
-  hb::launch_gather_codes(ly->d_perm, (uint32_t)t->n_rows, cc, p, stream);
-  HIP_OK(hipGetLastError());
-  ly->jcode.emplace(cols, p);
-  *out = p;
-  return HB_OK;
 }
 
 float filter_coef(double thr, int p) {
@@ -649,12 +854,9 @@ bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
          tile_b <= 65536 && tile_b % 256 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
 }
 
-// Filter variant of a JR: the kernel template it needs.
-int filter_variant(const PlanRule &rule) {
-  const int n_eq = rule.eq.empty() ? 0 : (int)rule.eq.size() - 1;
-  const int neq = n_eq == 3 ? 4 : n_eq;
-  return (rule.sims.empty() ? 0 : 8) + neq;
-}
+// Filter variant of a JR: the kernel template it needs (with or without a
+// similarity predicate; every equality is part of the join key).
+int filter_variant(const PlanRule &rule) { return rule.sims.empty() ? 0 : 1; }
 
 unsigned long long pow2_at_least(unsigned long long x) {
   unsigned long long p = 1024;
@@ -686,8 +888,11 @@ hb_status enqueue_batch(Batch *b) {
         HB_TRY(ensure_sig(it.R, s.cr, st));
       }
       JR jr{j, k, filter_variant(rule), {nullptr, nullptr}};
-      const int el = rule.eq.empty() ? -1 : rule.eq[0].first;
-      const int er = rule.eq.empty() ? -1 : rule.eq[0].second;
+      std::vector<int> el, er;
+      for (const auto &e : rule.eq) {
+        el.push_back(e.first);
+        er.push_back(e.second);
+      }
       const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
       const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
       HB_TRY(ensure_layout(it.L, el, sl, st, &jr.ly[0]));
@@ -709,12 +914,6 @@ hb_status enqueue_batch(Batch *b) {
     const PlanRule &rule = it.plan->rules[jr.rule];
     hb::JRDesc &d = hd[q];
     std::memset(&d, 0, sizeof(d));
-    hb_table *tab[2] = {it.L, it.R};
-    std::vector<int> extra[2];
-    for (size_t k = 1; k < rule.eq.size(); k++) {
-      extra[0].push_back(rule.eq[k].first);
-      extra[1].push_back(rule.eq[k].second);
-    }
     for (int side = 0; side < 2; side++) {
       Layout *ly = jr.ly[side];
       hb::SideDesc &sd = d.s[side];
@@ -723,8 +922,6 @@ hb_status enqueue_batch(Batch *b) {
       sd.run_code = ly->d_run_code;
       sd.run_off = ly->d_run_off;
       sd.n_runs = ly->n_runs;
-      sd.jcode = nullptr;
-      if (!extra[side].empty()) HB_TRY(ensure_jcode(tab[side], ly, extra[side], st, &sd.jcode));
     }
     if (!rule.sims.empty()) {
       const double thr = (double)rule.sims[0].thr;
@@ -734,7 +931,6 @@ hb_status enqueue_batch(Batch *b) {
       d.c3 = filter_coef(thr, 3);
     }
     d.job = jr.job;
-    d.n_eq = rule.eq.empty() ? 0 : (int32_t)rule.eq.size() - 1;
     d.n_sims = (int32_t)rule.sims.size();
     d.sim_off = (uint32_t)sims.size();
     d.multi = it.plan->rules.size() > 1 ? 1u : 0u;
@@ -795,6 +991,7 @@ hb_status enqueue_batch(Batch *b) {
   bd.sims = (const hb::SimDesc *)place(sims.data(), sims.size() * sizeof(hb::SimDesc));
   bd.job_cnt = d_cnt + b->cnt_job_off;
   bd.overflow = d_cnt + kCntOverflow;
+  bd.set_overflow = d_cnt + kCntSetOverflow;
   // per-job output sets (jobs with several rules)
   std::vector<unsigned long long> set_off(n_jobs, 0);
   unsigned long long set_total = 0;
@@ -816,7 +1013,7 @@ hb_status enqueue_batch(Batch *b) {
   g.n = n_groups;
   g.tile_a = 256u * (uint32_t)b->shape.rows_per_lane;
   g.tile_b = b->shape.tile_b;
-  g.small_pairs_max = 16384;
+  g.small_pairs_max = b->shape.small_pairs_max;
   uint32_t *u32arr[5];
   for (auto &a : u32arr) {
     HB_TRY(batch_alloc(b, ((size_t)n_groups + 1) * 4, &p));
@@ -827,18 +1024,17 @@ hb_status enqueue_batch(Batch *b) {
   g.r_off = u32arr[2];
   g.r_cnt = u32arr[3];
   g.jr = u32arr[4];
-  uint64_t *u64arr[4];
+  uint64_t *u64arr[3];
   for (auto &a : u64arr) {
     HB_TRY(batch_alloc(b, ((size_t)n_groups + 1) * 8, &p));
     a = (uint64_t *)p;
   }
   g.small_pairs = u64arr[0];
-  g.pair_off = u64arr[1];
-  g.items = u64arr[2];
-  g.item_off = u64arr[3];
+  g.items = u64arr[1];
+  g.item_off = u64arr[2];
   size_t tb64 = 0, tbj = 0;
-  HIP_OK(hb::exscan_u64(nullptr, tb64, g.small_pairs, g.pair_off, n_groups + 1, st));
-  HIP_OK(hb::exscan_u64(nullptr, tbj, g.small_pairs, g.pair_off, stride, st));
+  HIP_OK(hb::exscan_u64(nullptr, tb64, g.items, g.item_off, n_groups + 1, st));
+  HIP_OK(hb::exscan_u64(nullptr, tbj, g.items, g.item_off, stride, st));
   void *tmp = nullptr;
   const size_t tmp_bytes = std::max<size_t>(std::max(tb64, tbj), 16);
   HB_TRY(batch_alloc(b, tmp_bytes, &tmp));
@@ -870,8 +1066,6 @@ hb_status enqueue_batch(Batch *b) {
   hb::launch_batch_join(bd, g, st);
   {
     size_t t = tmp_bytes;
-    HIP_OK(hb::exscan_u64(tmp, t, g.small_pairs, g.pair_off, n_groups + 1, st));
-    t = tmp_bytes;
     HIP_OK(hb::exscan_u64(tmp, t, g.items, g.item_off, n_groups + 1, st));
   }
   HIP_OK(hipEventRecord(b->marks[0], st));
@@ -882,8 +1076,8 @@ hb_status enqueue_batch(Batch *b) {
     uint32_t e = q;
     while (e < n_jr && jrs[e].variant == jrs[q].variant) e++;
     const int variant = jrs[q].variant;
-    hb::launch_filter(bd, g, jr_group_off[q], jr_group_off[e], variant & 7, variant >= 8, list0,
-                      b->shape, ctx->n_cu, st);
+    hb::launch_filter(bd, g, jr_group_off[q], jr_group_off[e], variant != 0, list0, b->shape,
+                      ctx->n_cu, st);
     q = e;
   }
   HIP_OK(hipGetLastError());
@@ -936,8 +1130,10 @@ hb_status batch_complete(Batch *b) {
       SizeHint h;
       h.list = 1.25 * (double)need_list / rows;
       h.defer = 1.25 * (double)need_defer / rows;
-      h.final_ = 1.25 * (double)std::max(c[kCntFinal], c[kCntOverflow] ? need_list : 0ull) / rows;
+      h.final_ = 1.25 * (double)c[kCntFinal] / rows;
       h.audit = 1.25 * (double)c[kCntAudit] / rows;
+      h.max_list = 1.25 * (double)need_list;
+      h.max_final = 1.25 * (double)c[kCntFinal];
       const hb_plan *last = nullptr;
       for (const auto &it : b->items)
         if (it.plan != last) (last = it.plan)->raise_hint(h);
@@ -949,18 +1145,27 @@ hb_status batch_complete(Batch *b) {
       while (n < need + need / 4 + 1024) n *= 2;
       return n;
     };
-    b->list_cap = grow(b->list_cap, need_list);
-    b->defer_cap = grow(b->defer_cap, need_defer);
-    // a list that overflowed hid pairs from the later stages: size the final
-    // list for the largest list seen as well
-    b->final_cap = grow(b->final_cap, std::max(c[kCntFinal], need_list));
-    b->audit_cap = grow(b->audit_cap, c[kCntAudit]);
-    for (uint32_t j = 0; j < b->n_jobs(); j++)
-      if (b->set_cap[j])
-        b->set_cap[j] = std::max(2 * b->set_cap[j],
-                                 pow2_at_least(4 * b->job_counter(hb::kJobMatches, j)));
+    // Only what overflowed grows (counters count every append, also those
+    // past a full buffer).  A full list hides pairs from the later stages,
+    // so their counts may be low this time; a further rerun grows those.
+    if (need_list > b->list_cap) b->list_cap = grow(b->list_cap, need_list);
+    if (need_defer > b->defer_cap) b->defer_cap = grow(b->defer_cap, need_defer);
+    if (c[kCntFinal] > b->final_cap) b->final_cap = grow(b->final_cap, c[kCntFinal]);
+    if (c[kCntAudit] > b->audit_cap) b->audit_cap = grow(b->audit_cap, c[kCntAudit]);
+    // a set that filled up holds at least half its capacity in matches
+    if (c[kCntSetOverflow]) {
+      bool grew = false;
+      for (uint32_t j = 0; j < b->n_jobs(); j++)
+        if (b->set_cap[j] && 2 * b->job_counter(hb::kJobMatches, j) >= b->set_cap[j]) {
+          b->set_cap[j] = std::max(2 * b->set_cap[j],
+                                   pow2_at_least(4 * b->job_counter(hb::kJobMatches, j)));
+          grew = true;
+        }
+      if (!grew)
+        for (uint32_t j = 0; j < b->n_jobs(); j++) b->set_cap[j] *= 2;
+    }
     b->retries++;
-    if (b->retries > 8) return fail(HB_ERR_CAPACITY, "batch: output keeps overflowing");
+    if (b->retries > 12) return fail(HB_ERR_CAPACITY, "batch: output keeps overflowing");
     HIP_OK(hipStreamSynchronize(b->stream));
     batch_free_memory(b);
     HB_TRY(enqueue_batch(b));
@@ -971,6 +1176,21 @@ hb_status batch_complete(Batch *b) {
   (void)hipEventElapsedTime(&b->join_ms, b->ev_start, b->marks[0]);
   (void)hipEventElapsedTime(&b->filter_ms, b->marks[0], b->marks[1]);
   (void)hipEventElapsedTime(&b->exact_ms, b->marks[1], b->marks[2]);
+  // The device buffers are idle now and go back to the pool at once; the
+  // results move from the pinned staging buffers to heap memory of their
+  // exact size, so a finished job holds only its own output until released.
+  {
+    const uint32_t n = b->n_jobs();
+    const unsigned long long n_pairs = b->h_cnt[b->cnt_out_off + n];
+    const unsigned long long n_audit = b->h_cnt[b->cnt_audit_off + n];
+    b->res_cnt.assign(b->h_cnt, b->h_cnt + b->cnt_words);
+    b->res_pairs.assign(b->h_out, b->h_out + 2 * n_pairs);
+    b->res_audit.assign(b->h_audit, b->h_audit + n_audit);
+    batch_free_memory(b);
+    b->h_cnt = b->res_cnt.data();
+    b->h_out = b->res_pairs.data();
+    b->h_audit = b->res_audit.data();
+  }
   b->state = BatchState::kDone;
   return HB_OK;
 }
@@ -1091,7 +1311,7 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
   b->ctx = ctx;
   b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
   const uint32_t n = (uint32_t)jobs.size();
-  SizeHint want{0, 0, 0, 0};
+  SizeHint want{0, 0, 0, 0, 0, 0};
   b->set_cap.assign(n, 0);
   for (uint32_t j = 0; j < n; j++) {
     Batch::Item it{jobs[j]->plan, jobs[j]->L, jobs[j]->R};
@@ -1102,13 +1322,27 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
     want.defer += h.defer * jr;
     want.final_ += h.final_ * jr;
     want.audit += h.audit * jr;
-    if (it.plan->rules.size() > 1) b->set_cap[j] = pow2_at_least(2 * jr);
+    want.max_list = std::max(want.max_list, h.max_list);
+    want.max_final = std::max(want.max_final, h.max_final);
+    // dedupe set of a multi-rule job: twice its expected distinct matches,
+    // taken as the smaller side's rows (grown on overflow)
+    if (it.plan->rules.size() > 1)
+      b->set_cap[j] = pow2_at_least(4 * ((unsigned long long)std::min(it.L->n_rows, it.R->n_rows) + 256));
     b->mem_estimate += job_memory_estimate(jobs[j]);
   }
-  b->list_cap = std::max<unsigned long long>(1ull << 17, (unsigned long long)want.list);
-  b->defer_cap = std::max<unsigned long long>(1ull << 12, (unsigned long long)want.defer);
-  b->final_cap = std::max<unsigned long long>(1ull << 16, (unsigned long long)want.final_);
-  b->audit_cap = std::max<unsigned long long>(1ull << 12, (unsigned long long)want.audit);
+  // The row-proportional estimates are bounded (a batch of many jobs over
+  // one large resident table would otherwise reserve for every job as if it
+  // kept all its rows): at most 2^27 list entries (2 GiB), or what a batch
+  // of these plans once needed.  A batch that overflows grows and reruns.
+  const double list_max = std::max((double)(1ull << 27), want.max_list);
+  const double final_max = std::max((double)(1ull << 26), want.max_final);
+  b->list_cap = (unsigned long long)std::max((double)(1ull << 17), std::min(want.list, list_max));
+  b->defer_cap = (unsigned long long)std::max((double)(1ull << 12),
+                                              std::min(want.defer, (double)(1ull << 24)));
+  b->final_cap =
+      (unsigned long long)std::max((double)(1ull << 16), std::min(want.final_, final_max));
+  b->audit_cap = (unsigned long long)std::max((double)(1ull << 12),
+                                              std::min(want.audit, (double)(1ull << 24)));
   HB_TRY(ctx->events.get(&b->ev_start));
   HB_TRY(ctx->events.get(&b->ev_end));
   for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));
@@ -1124,6 +1358,18 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
       b->state = BatchState::kFailed;
       b->error = st;
       b->error_text = g_last_error;
+      if (st == HB_ERR_OOM) {  // say what held the memory
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        char buf[512];
+        std::snprintf(buf, sizeof(buf),
+                      " [device free %.2f of %.2f GiB; cached device %.2f GiB, pinned %.2f GiB;"
+                      " batch of %u jobs, lists %llu, final %llu, defer %llu, audit %llu]",
+                      free_b / 1073741824.0, total_b / 1073741824.0,
+                      ctx->pool.cached() / 1073741824.0, ctx->hpool.cached() / 1073741824.0, n,
+                      b->list_cap, b->final_cap, b->defer_cap, b->audit_cap);
+        b->error_text += buf;
+      }
     }
   }
   const auto now = Clock::now();
@@ -1206,6 +1452,15 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
     if (std::sscanf(env, "%d,%d,%d", &rpl, &tb, &gpc) == 3 && valid_filter_shape(rpl, tb, gpc))
       ctx->shape = hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc};
   }
+  if (const char *env = std::getenv("HB_SMALL_PAIRS")) {
+    const long v = std::atol(env);
+    if (v >= 0 && v <= (1l << 24)) ctx->shape.small_pairs_max = (uint32_t)v;
+  }
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b)
+      ctx->pool.set_limit(std::max<size_t>(8ull << 30, total_b / 8));
+  }
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&ctx->build_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; i++) {
@@ -1226,7 +1481,8 @@ hb_status hb_ctx_set_filter_shape(hb_ctx *ctx, int32_t rows_per_lane, int32_t ti
                 "of 256 up to 65536, groups per CU 1..64",
                 rows_per_lane, tile_b, groups_per_cu);
   std::lock_guard<std::mutex> lk(ctx->mu);
-  ctx->shape = hb::FilterShape{rows_per_lane, (uint32_t)tile_b, (unsigned)groups_per_cu};
+  ctx->shape = hb::FilterShape{rows_per_lane, (uint32_t)tile_b, (unsigned)groups_per_cu,
+                               ctx->shape.small_pairs_max};
   return HB_OK;
 }
 
@@ -1239,11 +1495,24 @@ hb_status hb_ctx_get_filter_shape(const hb_ctx *ctx, int32_t *rows_per_lane, int
   return HB_OK;
 }
 
+hb_status hb_ctx_memory_info(hb_ctx *ctx, hb_memory_info *info) {
+  if (!ctx || !info) return fail(HB_ERR_INVALID_ARG, "hb_ctx_memory_info: null argument");
+  HB_TRY(set_device(ctx));
+  size_t free_b = 0, total_b = 0;
+  HIP_OK(hipMemGetInfo(&free_b, &total_b));
+  info->device_free = free_b;
+  info->device_total = total_b;
+  info->pool_live = ctx->pool.live();
+  info->pool_cached = ctx->pool.cached();
+  info->pinned_cached = ctx->hpool.cached();
+  return HB_OK;
+}
+
 hb_status hb_ctx_synchronize(hb_ctx *ctx) {
   if (!ctx) return fail(HB_ERR_INVALID_ARG, "hb_ctx_synchronize: null context");
   HB_TRY(set_device(ctx));
   HIP_OK(hipDeviceSynchronize());
-  return HB_OK;
+  return reap_graves(ctx, false);
 }
 
 hb_status hb_ctx_destroy(hb_ctx *ctx) {
@@ -1254,6 +1523,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->jobs.clear();  // each batch returns its memory to the pools with its last job
   }
+  (void)reap_graves(ctx, true);
   ctx->pool.release_all();
   ctx->hpool.release_all();
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
@@ -1286,21 +1556,45 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
   t->n_cols = v->n_cols;
   t->cols.resize(v->n_cols);
   auto src_row = [&](int64_t i) { return rows ? rows[i] : i; };
-  for (int c = 0; c < v->n_cols; c++) {
-    Column &col = t->cols[c];
-    const int64_t stride = v->col_stride[c];
-    size_t maxlen = 0;
-    for (int64_t i = 0; i < n_rows; i++)
-      maxlen = std::max(maxlen, strnlen(v->col_base[c] + src_row(i) * stride, (size_t)stride));
-    col.width = (uint32_t)((maxlen + 1 + 15) & ~size_t(15));
-    const size_t bytes = std::max<size_t>((size_t)n_rows * col.width, 16);
-    HIP_OK(hipHostMalloc((void **)&col.h_bytes, bytes, hipHostMallocDefault));
-    std::memset(col.h_bytes, 0, bytes);
-    for (int64_t i = 0; i < n_rows; i++) {
-      const char *s = v->col_base[c] + src_row(i) * stride;
-      std::memcpy(col.h_bytes + i * col.width, s, strnlen(s, (size_t)stride));
+  const int nc = v->n_cols;
+  // cell widths: longest cell + NUL, rounded up to 16
+  std::vector<size_t> maxlen(nc, 0);
+  std::mutex mm;
+  parallel_for(n_rows, 1 << 14, [&](int64_t b, int64_t e) {
+    std::vector<size_t> ml(nc, 0);
+    for (int c = 0; c < nc; c++) {
+      const int64_t stride = v->col_stride[c];
+      for (int64_t i = b; i < e; i++)
+        ml[c] = std::max(ml[c], strnlen(v->col_base[c] + src_row(i) * stride, (size_t)stride));
     }
-    HIP_OK(hipMalloc(&col.d_bytes, bytes));
+    std::lock_guard<std::mutex> lk(mm);
+    for (int c = 0; c < nc; c++) maxlen[c] = std::max(maxlen[c], ml[c]);
+  });
+  for (int c = 0; c < nc; c++) {
+    Column &col = t->cols[c];
+    col.width = (uint32_t)((maxlen[c] + 1 + 15) & ~size_t(15));
+    const size_t bytes = std::max<size_t>((size_t)n_rows * col.width, 16);
+    HB_TRY(t->host_alloc(bytes, (void **)&col.h_bytes));
+    HB_TRY(t->dev_alloc(bytes, (void **)&col.d_bytes));
+    if (n_rows == 0) std::memset(col.h_bytes, 0, bytes);
+  }
+  // pack (each cell NUL padded to its column's width)
+  parallel_for(n_rows, 1 << 14, [&](int64_t b, int64_t e) {
+    for (int c = 0; c < nc; c++) {
+      Column &col = t->cols[c];
+      const int64_t stride = v->col_stride[c];
+      for (int64_t i = b; i < e; i++) {
+        const char *s = v->col_base[c] + src_row(i) * stride;
+        const size_t len = strnlen(s, (size_t)stride);
+        char *dst = col.h_bytes + (size_t)i * col.width;
+        std::memcpy(dst, s, len);
+        std::memset(dst + len, 0, col.width - len);
+      }
+    }
+  });
+  for (int c = 0; c < nc; c++) {
+    Column &col = t->cols[c];
+    const size_t bytes = std::max<size_t>((size_t)n_rows * col.width, 16);
     HIP_OK(hipMemcpyAsync(col.d_bytes, col.h_bytes, bytes, hipMemcpyHostToDevice,
                           ctx->copy_stream));
   }
@@ -1389,50 +1683,37 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
     return fail(HB_ERR_INVALID_ARG, "hb_table_release: %d slices of this table are alive",
                 t->slices.load());
   HB_TRY(set_device(ctx));
-  // Jobs hold no table memory of their own, so every stream that may still
-  // read this table must drain first.
-  for (hipStream_t s : ctx->streams) HIP_OK(hipStreamSynchronize(s));
-  HIP_OK(hipStreamSynchronize(ctx->copy_stream));
-  HIP_OK(hipStreamSynchronize(ctx->build_stream));
-  if (t->parent) {  // only the slice's own layouts are freed
-    for (auto &kv : t->layouts) {
-      Layout &ly = kv.second;
-      (void)hipFree(ly.d_keys);
-      (void)hipFree(ly.d_perm);
-      (void)hipFree(ly.d_jsig);
-      (void)hipFree(ly.d_run_code);
-      (void)hipFree(ly.d_run_off);
-      for (auto &jc : ly.jcode) (void)hipFree(jc.second);
-      if (ly.lz.ev) (void)hipEventDestroy(ly.lz.ev);
-    }
-    t->parent->slices--;
-    delete t;
-    return HB_OK;
+  // Work already submitted may still read the table: its memory goes back
+  // to the pools once every stream has passed this point (no host wait).
+  hb_ctx::Grave g;
+  g.dev = std::move(t->dev_mem);
+  g.host = std::move(t->host_mem);
+  for (auto &kv : t->layouts)
+    if (kv.second.lz.ev) g.owned.push_back(kv.second.lz.ev);
+  if (!t->parent) {
+    for (Column &c : t->cols)
+      for (Lazy *lz : {&c.sig, &c.code})
+        if (lz->ev) g.owned.push_back(lz->ev);
+    for (auto &kv : t->composites)
+      if (kv.second.code.ev) g.owned.push_back(kv.second.code.ev);
+    if (t->ready) g.owned.push_back(t->ready);
   }
-  for (Column &c : t->cols) {
-    (void)hipFree(c.d_bytes);
-    if (c.h_bytes) (void)hipHostFree(c.h_bytes);
-    (void)hipFree(c.d_len);
-    (void)hipFree(c.d_sig);
-    (void)hipFree(c.d_planes);
-    (void)hipFree(c.d_code);
-    if (c.h_code) (void)hipHostFree(c.h_code);
-    for (Lazy *lz : {&c.sig, &c.code})
-      if (lz->ev) (void)hipEventDestroy(lz->ev);
+  std::vector<hipStream_t> all(ctx->streams.begin(), ctx->streams.end());
+  all.push_back(ctx->copy_stream);
+  all.push_back(ctx->build_stream);
+  for (hipStream_t s : all) {
+    hipEvent_t e = nullptr;
+    HB_TRY(ctx->events.get(&e));
+    g.wait.push_back(e);
+    HIP_OK(hipEventRecord(e, s));
   }
-  for (auto &kv : t->layouts) {
-    Layout &ly = kv.second;
-    (void)hipFree(ly.d_keys);
-    (void)hipFree(ly.d_perm);
-    (void)hipFree(ly.d_jsig);
-    (void)hipFree(ly.d_run_code);
-    (void)hipFree(ly.d_run_off);
-    for (auto &jc : ly.jcode) (void)hipFree(jc.second);
-    if (ly.lz.ev) (void)hipEventDestroy(ly.lz.ev);
-  }
-  if (t->ready) (void)hipEventDestroy(t->ready);
+  if (t->parent) t->parent->slices--;
   delete t;
-  return HB_OK;
+  {
+    std::lock_guard<std::mutex> lk(ctx->grave_mu);
+    ctx->graves.push_back(std::move(g));
+  }
+  return reap_graves(ctx, false);
 }
 
 hb_status hb_plan_create(hb_ctx *ctx, const hb_serialized_plan *sp, hb_plan **out) {
@@ -1472,9 +1753,6 @@ hb_status hb_plan_create(hb_ctx *ctx, const hb_serialized_plan *sp, hb_plan **ou
       i += 2;
     }
     i++;  // checkpoint
-    if ((int)rule.eq.size() > 1 + hb::kMaxEq)
-      return fail(HB_ERR_CAPACITY, "plan: rule %zu has %zu equalities (at most %d)",
-                  plan->rules.size(), rule.eq.size(), 1 + hb::kMaxEq);
     if (rule.sims.size() > 16)
       return fail(HB_ERR_CAPACITY, "plan: rule %zu has %zu similarity predicates (at most 16)",
                   plan->rules.size(), rule.sims.size());
@@ -1742,12 +2020,15 @@ hb_status hb_autotune_filter_shape(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *
       if (!valid_filter_shape(c[0], c[1], c[2]))
         return fail(HB_ERR_INVALID_ARG, "candidate %u (%d, %d, %d) is not a valid shape", k, c[0],
                     c[1], c[2]);
-      cands.push_back(hb::FilterShape{c[0], (uint32_t)c[1], (unsigned)c[2]});
+      cands.push_back(hb::FilterShape{c[0], (uint32_t)c[1], (unsigned)c[2],
+                                      ctx->shape.small_pairs_max});
     }
   } else {
     for (int rpl : {1, 2, 4})
       for (int tb : {256, 512})
-        for (int gpc : {8, 16, 32}) cands.push_back(hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc});
+        for (int gpc : {8, 16, 32})
+          cands.push_back(
+              hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc, ctx->shape.small_pairs_max});
   }
   static std::atomic<uint64_t> seq{0};
   std::vector<uint64_t> ids(n_jobs);

@@ -15,8 +15,15 @@ served by the continuous-batching server (hb_server_*) and, on the same
 arrival trace, by plain one-at-a-time launches; both report p50/p99 latency
 (arrival to results ready) and throughput.
 
+Streamed (c3): 1M x 1M rows of the 8-column schema, three rules, split
+into 16 partitions by the reference's hash partitioning; each partition's
+rows are packed and copied to the device while earlier partitions compute
+(a few partitions in flight).  Pairs/s counts every stage after
+partitioning: host packing, host-to-device copies, kernels and result copies.
+
 Nothing here computes a predicate: every job runs through the C ABI.
 """
+import collections
 import math
 import random
 import threading
@@ -161,29 +168,50 @@ def _serve_trace(submit, specs, arrivals, plans, tables, n_l_pairs):
     each job's stats.  Returns (latencies ms, wall s)."""
     handles = [None] * len(specs)
     lat = []
-    done = threading.Event()
+    errors = []
+    stop = threading.Event()
 
     def consume():
-        for k in range(len(specs)):
-            while handles[k] is None:
-                time.sleep(0.0001)
-            j = handles[k]
-            j.wait()
-            s = j.stats()
-            lat.append(s["latency_ms"])
-            j.release()
-        done.set()
+        try:
+            for k in range(len(specs)):
+                while handles[k] is None:
+                    if stop.is_set():
+                        return
+                    time.sleep(0.0001)
+                j = handles[k]
+                j.wait()
+                lat.append(j.stats()["latency_ms"])
+                j.release()
+        except Exception as e:  # reported by the submitting thread
+            errors.append(RuntimeError("job %d of %d: %s" % (len(lat), len(specs), e)))
 
     th = threading.Thread(target=consume)
     th.start()
     t0 = time.perf_counter()
-    for k, ((off_l, n_l, rules), at) in enumerate(zip(specs, arrivals)):
-        while time.perf_counter() - t0 < at:
-            pass
-        handles[k] = submit(plans[rules], tables[k])
-    done.wait()
-    th.join()
+    try:
+        for k, ((off_l, n_l, rules), at) in enumerate(zip(specs, arrivals)):
+            while True:  # sleep (letting the consumer run), then spin the last 0.3 ms
+                left = at - (time.perf_counter() - t0)
+                if left <= 0:
+                    break
+                if left > 0.0006:
+                    time.sleep(left - 0.0003)
+            if errors:
+                break
+            handles[k] = submit(plans[rules], tables[k])
+    finally:
+        stop.set()
+        th.join()
+    if errors:
+        raise errors[0]
     return lat, time.perf_counter() - t0
+
+
+def _mem(ctx):
+    m = ctx.memory()
+    return "device free %.1f of %.1f GiB, library live %.1f GiB (cached %.1f)" % (
+        m["device_free"] / 2**30, m["device_total"] / 2**30, m["pool_live"] / 2**30,
+        m["pool_cached"] / 2**30)
 
 
 def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
@@ -201,7 +229,7 @@ def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
         # warm: per-table structures of every slice, and plain capacity
         t = time.perf_counter()
         _drain(ctx.submit_batch([(plans[r], sl, tr) for (_, _, r), sl in zip(specs, slices)]))
-        log("c5: cold batch %.2f s" % (time.perf_counter() - t))
+        log("c5: cold batch %.2f s; %s" % (time.perf_counter() - t, _mem(ctx)))
         n_cap = min(200, n_jobs)
         t = time.perf_counter()
         for (_, _, r), sl in zip(specs[:n_cap], slices[:n_cap]):
@@ -214,16 +242,28 @@ def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
         for _ in range(n_jobs):
             at += rng.expovariate(lam)
             arrivals.append(at)
-        log("c5: plain service %.3f ms/job, arrival rate %.0f jobs/s" % (service * 1e3, lam))
+        log("c5: plain service %.3f ms/job, arrival rate %.0f jobs/s; %s"
+            % (service * 1e3, lam, _mem(ctx)))
         # served: continuous batching
         with hbk.Server(ctx, max_batches_in_flight=2, max_jobs_per_batch=64) as srv:
             lat_s, wall_s = _serve_trace(lambda p, sl: srv.submit(p, sl, tr), specs, arrivals,
                                          plans, slices, pairs_total)
             info = srv.info()
+        log("c5: served trace done; %s" % _mem(ctx))
         # plain: the same trace, one job at a time
         with hbk.Server(ctx, max_batches_in_flight=1, max_jobs_per_batch=1) as srv:
             lat_p, wall_p = _serve_trace(lambda p, sl: srv.submit(p, sl, tr), specs, arrivals,
                                          plans, slices, pairs_total)
+        # saturated: every job queued at once (throughput when the offered
+        # load exceeds what plain launches sustain)
+        now = [0.0] * n_jobs
+        with hbk.Server(ctx, max_batches_in_flight=2, max_jobs_per_batch=64) as srv:
+            lat_ss, wall_ss = _serve_trace(lambda p, sl: srv.submit(p, sl, tr), specs, now,
+                                           plans, slices, pairs_total)
+        with hbk.Server(ctx, max_batches_in_flight=1, max_jobs_per_batch=1) as srv:
+            lat_sp, wall_sp = _serve_trace(lambda p, sl: srv.submit(p, sl, tr), specs, now,
+                                           plans, slices, pairs_total)
+        log("c5: saturated: served %.3f s, plain %.3f s" % (wall_ss, wall_sp))
         p50s, p99s = _percentiles(lat_s)
         p50p, p99p = _percentiles(lat_p)
         return {
@@ -234,6 +274,12 @@ def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
             "served_p99_ms": p99s, "served_batches": info["batches"],
             "plain_pairs_per_s": pairs_total / wall_p, "plain_p50_ms": p50p,
             "plain_p99_ms": p99p,
+            "saturated_served_pairs_per_s": pairs_total / wall_ss,
+            "saturated_served_p50_ms": _percentiles(lat_ss)[0],
+            "saturated_served_p99_ms": _percentiles(lat_ss)[1],
+            "saturated_plain_pairs_per_s": pairs_total / wall_sp,
+            "saturated_plain_p50_ms": _percentiles(lat_sp)[0],
+            "saturated_plain_p99_ms": _percentiles(lat_sp)[1],
         }
     finally:
         for s in slices:
@@ -241,3 +287,67 @@ def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
         tl.release()
         tr.release()
         plans.release()
+
+
+def run_c3(ctx, inflight=3, reps=2, log=print):
+    t = time.perf_counter()
+    L, R, _, cfg = datagen.make("c3")
+    lt, rt = hbk.ColumnTable(L), hbk.ColumnTable(R)
+    plan = hbk.HostPlan.from_yaml(cfg["rules"])
+    log("c3: tables generated in %.1f s" % (time.perf_counter() - t))
+    t = time.perf_counter()
+    parts = hbk.partition(lt, rt, plan, cfg["n_partitions"])
+    part_s = time.perf_counter() - t
+    pairs_total = sum(len(a) * len(b) for a, b in parts)
+    log("c3: %d partitions (%.3g pairs) in %.2f s" % (len(parts), pairs_total, part_s))
+    dp = ctx.plan(plan)
+
+    def stream():
+        t0 = time.perf_counter()
+        pending = collections.deque()
+        out = {"matches": 0, "device_ms": 0.0, "upload_s": 0.0, "candidates": 0,
+               "jw_exact": 0, "audit": 0}
+
+        def drain():
+            job, tl, tr = pending.popleft()
+            pairs, audit, st = job.results()
+            job.release()
+            tl.release()
+            tr.release()
+            out["matches"] += len(pairs)
+            out["audit"] += len(audit)
+            out["device_ms"] += st["device_ms"]
+            out["candidates"] += st["candidates"]
+            out["jw_exact"] += st["jw_exact"]
+
+        for lr, rr in parts:
+            tu = time.perf_counter()
+            tl, tr = ctx.upload(lt, rows=lr), ctx.upload(rt, rows=rr)
+            out["upload_s"] += time.perf_counter() - tu
+            pending.append((ctx.submit(dp, tl, tr), tl, tr))
+            if len(pending) >= inflight:
+                drain()
+        while pending:
+            drain()
+        out["wall_s"] = time.perf_counter() - t0
+        return out
+
+    try:
+        runs = []
+        for k in range(reps + 1):
+            r = stream()
+            log("c3: stream %d: %.3f s (uploads %.3f s on the host thread, device %.1f ms)"
+                % (k, r["wall_s"], r["upload_s"], r["device_ms"]))
+            runs.append(r)
+        best = min(runs[1:], key=lambda r: r["wall_s"])
+        return {
+            "partitions": len(parts), "pairs": pairs_total, "partition_s": part_s,
+            "streamed_s": best["wall_s"], "streamed_pairs_per_s": pairs_total / best["wall_s"],
+            "first_stream_s": runs[0]["wall_s"], "device_ms": best["device_ms"],
+            "device_pairs_per_s": pairs_total / (best["device_ms"] * 1e-3),
+            "host_upload_s": best["upload_s"], "matches": best["matches"],
+            "candidates": best["candidates"], "jw_exact": best["jw_exact"],
+            "audit": best["audit"], "inflight": inflight,
+        }
+    finally:
+        dp.release()

@@ -126,6 +126,17 @@ typedef struct {
 hb_status hb_ctx_create(int32_t device, hb_ctx **out);
 hb_status hb_ctx_destroy(hb_ctx *ctx);
 /* Waits until all work of every stream on the context's device is done. */
+/* Device memory as the library sees it: the device's free and total bytes,
+ * bytes the library holds from the driver (live) and, of those, bytes kept
+ * in its cache for reuse, and pinned host bytes in its cache.  No
+ * counterpart in the reference (which never frees device memory). */
+typedef struct hb_memory_info {
+  uint64_t device_free, device_total;
+  uint64_t pool_live, pool_cached;
+  uint64_t pinned_cached;
+} hb_memory_info;
+hb_status hb_ctx_memory_info(hb_ctx *ctx, hb_memory_info *info);
+
 hb_status hb_ctx_synchronize(hb_ctx *ctx);
 /* Launch shape of the bound-filter kernel on this context: rows of the
  * larger side per lane (1, 2 or 4), rows of the other side per work item
@@ -275,6 +286,11 @@ void hb_host_table_free(hb_host_table *t);
 hb_status hb_partition(const hb_host_table *l, const hb_host_table *r,
                        const hb_serialized_plan *plan, uint64_t n_partitions,
                        int64_t prefix_idx, hb_partitions **out);
+/* The same partitioning over column-addressed views (for tables that never
+ * went through the CSV reader); identical buckets for identical cells. */
+hb_status hb_partition_views(const hb_table_view *l, const hb_table_view *r,
+                             const hb_serialized_plan *plan, uint64_t n_partitions,
+                             int64_t prefix_idx, hb_partitions **out);
 int64_t hb_partitions_count(const hb_partitions *p);
 hb_status hb_partitions_get(const hb_partitions *p, int64_t k, const int64_t **l_rows,
                             int64_t *n_l, const int64_t **r_rows, int64_t *n_r);

@@ -284,6 +284,18 @@ def test_partition_matches_bucket_rule(std_hash, P, prefix):
     assert len(parts) <= P
 
 
+@pytest.mark.parametrize("P,prefix", [(1, None), (7, None), (16, 1)])
+def test_partition_of_column_views_equals_csv_tables(P, prefix):
+    """hb_partition_views buckets the same cells as hb_partition does for
+    the CSV-loaded tables."""
+    Lh, Rh, plan = H.load_fixture("dblp_acm")
+    L, R = hb.ColumnTable(H.host_columns(Lh)), hb.ColumnTable(H.host_columns(Rh))
+    pre = CP if prefix is None else prefix
+    a = hb.partition(Lh, Rh, plan, P, pre)
+    b = hb.partition(L, R, plan, P, pre)
+    assert [(x.tolist(), y.tolist()) for x, y in a] == [(x.tolist(), y.tolist()) for x, y in b]
+
+
 def test_partition_without_leading_equality_keeps_one_bucket(std_hash):
     Lh, Rh, plan = H.load_fixture("dblp_google_self")  # slot 0 is a JW predicate
     parts = hb.partition(Lh, Rh, plan, 8)
