@@ -49,6 +49,12 @@ static_assert(sizeof(SigRow) == 48, "SigRow layout");
 
 constexpr uint32_t kLongFlag = 0x80000000u;
 
+// 128-class byte counters of a similarity column: class = byte & 127, one u8
+// counter per class, counter k in byte (k & 3) of word k >> 2.  Counters of
+// cells longer than 255 bytes may wrap; such cells carry kLongFlag in their
+// signature and no bound rejects them.
+constexpr int kHistWords = 32;
+
 // Bit-planes of a similarity column (positions 0..32*nwc-1, nwc <= 8):
 // uint32 planes[row][w][k], bit b of word (w, k) is bit k of byte 32w+b.
 constexpr int kMaxPlaneWords = 8;
@@ -59,6 +65,8 @@ struct DevColumn {
   uint32_t width;
   const uint32_t *planes;
   uint32_t nwc;
+  const SigRow *sig;      // per row
+  const uint32_t *hist;   // kHistWords per row
 };
 
 // One side (0 = left, 1 = right) of a JR, in its join order: position i is
@@ -82,13 +90,15 @@ struct JRDesc {
   int32_t n_sims;
   uint32_t sim_off;      // first SimDesc of this JR
   uint32_t multi;        // the job has several rules: its final pairs are deduplicated
-  uint32_t pad[4];
+  uint32_t refine;       // the refine pass tests the first similarity predicate
+  uint32_t pad[3];
 };
 
 struct SimDesc {
   DevColumn col_l, col_r;
   float thr;
   int32_t slot;
+  float c0, c1, c2, c3;  // bound coefficients of thr (as in JRDesc)
 };
 
 // Per-job counters, structure of arrays: counter k of job j lives at
@@ -174,6 +184,8 @@ void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint
                       SigRow *sig, hipStream_t stream);
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
                          uint32_t *planes, hipStream_t stream);
+void launch_build_hist(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t *hist,
+                       hipStream_t stream);
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long long v,
                      hipStream_t stream);
@@ -201,13 +213,20 @@ void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream);
 void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, bool has_sim,
                    PairList out, FilterShape shape, int n_cu, hipStream_t stream);
 
+// Second bound on the filter's survivors (`in`): the filter's JW bound for
+// the rule's first similarity predicate with 128 byte classes instead of 32
+// (for JRs with `refine` set; others pass); the pairs it does not reject go
+// to `out`.
+void launch_refine(BatchDev b, PairList in, PairList out, int n_cu, hipStream_t stream);
+
 // Exact stage `stage` (1-based similarity index) on list `in`: JRs without
 // similarity predicates pass their pairs to the output directly; pairs
 // passing their last predicate go to `final_` (deduplicated per job where
 // needed), the others to `next`.  Pairs with a shorter string above 256
 // bytes run one wave per pair (second kernel, via `defer`).
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
-                  PairList defer, AuditList audit, int n_cu, hipStream_t stream);
+                  PairList defer, AuditList audit, bool count_filtered, int n_cu,
+                  hipStream_t stream);
 
 // Copies the final list and the audit list into per-job contiguous ranges:
 // job j's pairs start at out_off[j] (exclusive scan of its match count).

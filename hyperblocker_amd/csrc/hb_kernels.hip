@@ -156,9 +156,11 @@ __device__ __forceinline__ void wave_append(bool ok, const PairEntry &e, PairEnt
 // translating entries to table rows.  Entries past the buffer go to the
 // global list directly, one atomic per wave.
 constexpr uint32_t kTileOut = 2048;
+static_assert(256 * 4 <= kTileOut / 2, "one filter step must fit half the tile buffer");
 struct TileOut {
   uint32_t e[kTileOut];
   uint32_t n;
+  uint32_t dense;  // the last flush held many entries: flush between segments of B rows too
   unsigned long long base;
 };
 
@@ -197,7 +199,10 @@ __device__ __forceinline__ void tile_flush(TileOut &to, const TileCtx &c, const 
                                            unsigned long long *overflow) {
   __syncthreads();
   const uint32_t n = min(to.n, kTileOut);
-  if (threadIdx.x == 0) to.base = n ? atomicAdd(out.count, (unsigned long long)n) : 0ull;
+  if (threadIdx.x == 0) {
+    to.dense = to.n >= kTileOut / 4 ? 1u : 0u;
+    to.base = n ? atomicAdd(out.count, (unsigned long long)n) : 0ull;
+  }
   __syncthreads();
   const unsigned long long base = to.base;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -303,6 +308,40 @@ __global__ void k_build_sig(const uint8_t *__restrict__ bytes, uint32_t width, i
   s.rlen = (L >= 1u && L <= 255u) ? 1.0f / (float)L : INFINITY;
   s.pad = 0;
   sig_out[r] = s;
+}
+
+// 128-class counters of each cell (layout: kHistWords above).  Each thread
+// counts one row in its own column of an LDS array, so no synchronisation
+// is needed.
+__global__ __launch_bounds__(256) void k_build_hist(const uint8_t *__restrict__ bytes,
+                                                   uint32_t width, int64_t n,
+                                                   uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_h[kHistWords][256];
+  const uint32_t t = threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * 256 + t;
+#pragma unroll
+  for (int w = 0; w < kHistWords; w++) s_h[w][t] = 0;
+  if (r >= n) return;
+  const uint4 *cell = reinterpret_cast<const uint4 *>(bytes + r * (int64_t)width);
+  const uint32_t n16 = width >> 4;
+  bool end = false;
+  for (uint32_t k = 0; k < n16 && !end; k++) {
+    const uint4 v = cell[k];
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const uint32_t c = (w4[q] >> (8 * b)) & 0xffu;
+        end |= (c == 0);
+        if (!end) s_h[(c & 127u) >> 2][t] += 1u << ((c & 3u) * 8u);
+      }
+    }
+  }
+  uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords);
+#pragma unroll
+  for (int w = 0; w < kHistWords / 4; w++)
+    dst[w] = make_uint4(s_h[4 * w][t], s_h[4 * w + 1][t], s_h[4 * w + 2][t], s_h[4 * w + 3][t]);
 }
 
 __global__ void k_build_planes(const uint8_t *__restrict__ bytes, uint32_t width, int64_t n,
@@ -557,7 +596,10 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
   __shared__ float s_rlen[SIM ? kChunk + 1 : 1];
   __shared__ TileOut to;
   const uint32_t tid = threadIdx.x;
-  if (tid == 0) to.n = 0;
+  if (tid == 0) {
+    to.n = 0;
+    to.dense = 0;
+  }
   const uint32_t tile_b = g.tile_b;
   const uint64_t item_begin = g.item_off[g_begin], item_end = g.item_off[g_end];
   for (uint64_t item = item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
@@ -565,6 +607,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     const uint32_t jr = g.jr[gi];
     const JRDesc &d = bd.jrs[jr];
     const float k0 = 2.0f * d.c0, k3 = 2.0f * d.c3;
+    const bool dense_item = !SIM || d.refine != 0;
     const uint32_t l_off = g.l_off[gi], l_cnt = g.l_cnt[gi];
     const uint32_t r_off = g.r_off[gi], r_cnt = g.r_cnt[gi];
     const bool a_is_l = l_cnt >= r_cnt;
@@ -652,6 +695,16 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
           for (int r = 0; r < AR; r++) allowances(la[r], rla[r], blen, rlb, k0, k3, nal0[r], nal3[r]);
         }
         for (uint32_t jj = run; jj < run_end; jj += BU) {
+          // Where survivors are dense (no similarity test, a loose first
+          // threshold, or a large last flush) the buffer is also flushed
+          // between steps: a step adds at most 256 * AR * BU <= kTileOut / 2
+          // entries, so it never overflows.  Every wave of the group runs
+          // the same (run, jj) sequence and reads the same `dense` (written
+          // only between barriers), so all reach the barrier together.
+          if ((dense_item || to.dense) && jj != run) {
+            __syncthreads();
+            if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
+          }
           const bool second = jj + 1 < run_end;
           bool ok[BU][AR];
 #pragma unroll
@@ -845,6 +898,7 @@ struct ExactArgs {
   BatchDev b;
   PairList in;
   int stage;        // 1-based index of the similarity predicate evaluated
+  int count_filtered;  // count the input as the filter's survivors (no refine pass ran)
   PairList next;    // pairs with further predicates
   PairList final_;  // pairs that passed their JR's last predicate (tagged with the job)
   PairList defer;   // pairs whose shorter string exceeds 256 bytes
@@ -1037,6 +1091,64 @@ __device__ __forceinline__ int lane_matches(const uint32_t *row_planes, int nwc,
   return lane_greedy<NW, false>(P, S, h, n_out, outer, n_steps);
 }
 
+// JW bound of a pair for one similarity predicate with 128-class counters:
+// false only when JW is certainly below thr - 2e-6 (the test of may_pass on
+// a tighter m_ub: a matched pair of bytes is a pair of equal bytes, so
+// m <= sum over classes of min(count_a, count_b) for any class partition).
+__device__ __forceinline__ bool hist_pass(const SimDesc &sd, uint32_t l, uint32_t r) {
+  const SigRow sa = sd.col_l.sig[l], sb = sd.col_r.sig[r];
+  const uint32_t la = sa.len & 0xffffu, lb = sb.len & 0xffffu;
+  if (((sa.len | sb.len) & kLongFlag) || la == 0 || lb == 0) return true;
+  const uint32_t x = sa.pre ^ sb.pre;
+  if (x == 0) return true;
+  const uint4 *ha = reinterpret_cast<const uint4 *>(sd.col_l.hist + (uint64_t)l * kHistWords);
+  const uint4 *hb = reinterpret_cast<const uint4 *>(sd.col_r.hist + (uint64_t)r * kHistWords);
+  uint32_t sad = 0;
+#pragma unroll
+  for (int k = 0; k < kHistWords / 4; k++) {
+    const uint4 u = ha[k], v = hb[k];
+    sad = __builtin_amdgcn_sad_u8(u.x, v.x, sad);
+    sad = __builtin_amdgcn_sad_u8(u.y, v.y, sad);
+    sad = __builtin_amdgcn_sad_u8(u.z, v.z, sad);
+    sad = __builtin_amdgcn_sad_u8(u.w, v.w, sad);
+  }
+  const uint32_t mub = (la + lb - sad) >> 1;
+  const uint32_t S = min(la, lb), T = max(la, lb);
+  const uint32_t p = ((uint32_t)(__ffs(x) - 1)) >> 3;
+  const float coef = p == 0 ? sd.c0 : (p == 1 ? sd.c1 : (p == 2 ? sd.c2 : sd.c3));
+  return !((float)(mub * (S + T)) < coef * (float)(S * T));
+}
+
+// One pair per lane: the first similarity predicate of the pair's rule must
+// pass hist_pass (JRs with `refine` set).  Only the first: the predicates are evaluated in order, and
+// a later one failing does not spare an earlier one's evaluation (or its
+// audit entry).  Rules without similarity predicates pass through.
+__global__ __launch_bounds__(256) void k_refine(const BatchDev b, const PairList in,
+                                                const PairList out) {
+  const unsigned long long n_in = *in.count;
+  const unsigned long long n_use = n_in < in.cap ? n_in : in.cap;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  for (uint64_t q0 = base0; q0 < n_use; q0 += stride) {
+    const uint64_t idx = q0 + lane_id();
+    const bool act = idx < n_use;
+    PairEntry e{0u, 0u, 0u, 0u};
+    uint32_t job = 0, sim_off = 0, refine = 0;
+    int32_t n_sims = 0;
+    if (act) {
+      e = in.e[idx];
+      const JRDesc &d = b.jrs[e.jr];
+      job = d.job;
+      n_sims = d.n_sims;
+      sim_off = d.sim_off;
+      refine = d.refine;
+    }
+    job_count(b, kJobFiltered, act, job);
+    const bool keep = act && (n_sims == 0 || !refine || hist_pass(b.sims[sim_off], e.l, e.r));
+    list_append(keep, e, out, b.overflow);
+  }
+}
+
 // One pair per lane.  Pairs whose shorter string is longer than 256 bytes
 // are deferred to k_exact (one wave per pair, 257-slot semantics).
 __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
@@ -1057,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
       n_sims = d.n_sims;
       sim_off = d.sim_off;
     }
-    if (a.stage == 1) job_count(a.b, kJobFiltered, act, job);
+    if (a.count_filtered) job_count(a.b, kJobFiltered, act, job);
     // a rule without similarity predicates is complete after the filter
     finalize(a, act && n_sims == 0, e.l, e.r, job);
     act = act && n_sims >= a.stage;
@@ -1200,6 +1312,18 @@ void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint
                      bytes, width, n_rows, len, sig);
 }
 
+void launch_build_hist(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t *hist,
+                       hipStream_t stream) {
+  if (n_rows <= 0) return;
+  hipLaunchKernelGGL(k_build_hist, dim3(grid_for((uint64_t)n_rows, 256)), dim3(256), 0, stream,
+                     bytes, width, n_rows, hist);
+}
+
+void launch_refine(BatchDev b, PairList in, PairList out, int n_cu, hipStream_t stream) {
+  const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
+  hipLaunchKernelGGL(k_refine, dim3(cus * 8), dim3(256), 0, stream, b, in, out);
+}
+
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
                          uint32_t *planes, hipStream_t stream) {
   if (n_rows <= 0) return;
@@ -1284,11 +1408,12 @@ void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, 
 }
 
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
-                  PairList defer, AuditList audit, int n_cu, hipStream_t stream) {
+                  PairList defer, AuditList audit, bool count_filtered, int n_cu,
+                  hipStream_t stream) {
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
-  ExactArgs a{b, in, stage, next, final_, defer, audit};
+  ExactArgs a{b, in, stage, count_filtered ? 1 : 0, next, final_, defer, audit};
   hipLaunchKernelGGL(k_exact_lanes, dim3(cus * 8), dim3(256), 0, stream, a);
-  ExactArgs d{b, defer, stage, next, final_, defer, audit};
+  ExactArgs d{b, defer, stage, 0, next, final_, defer, audit};
   d.defer.cap = 0;  // deferred pairs are not deferred again
   hipLaunchKernelGGL(k_exact, dim3(cus * 4), dim3(256), 0, stream, d);
 }

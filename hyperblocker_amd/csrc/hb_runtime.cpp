@@ -24,6 +24,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <thread>
 #include <memory>
@@ -253,13 +254,21 @@ class EventPool {
 // codes from any two columns compare equal exactly when the strings do.
 // Dictionary codes of equality cells, shared by every table of a context so
 // that equal byte strings get equal codes across tables.  Sharded by hash,
-// each shard with its own lock, so that many threads code one column at
-// once; code = shard + kShards * (index within the shard).
+// each shard with its own lock and count, so that many threads code one
+// column at once without a shared write; code = shard + kShards * (index
+// within the shard).
 class Dictionary {
  public:
-  static constexpr uint32_t kShards = 64;
+  static constexpr uint32_t kShards = 256;
   // One more than the largest code handed out so far.
-  uint64_t code_bound() const { return bound_.load(); }
+  uint64_t code_bound() const {
+    uint64_t b = 1;
+    for (uint32_t s = 0; s < kShards; s++) {
+      const uint32_t c = shards_[s].count.load(std::memory_order_acquire);
+      if (c) b = std::max<uint64_t>(b, (uint64_t)s + (uint64_t)kShards * (c - 1) + 1);
+    }
+    return b;
+  }
   uint32_t code(std::string_view s) {
     const size_t h = std::hash<std::string_view>{}(s);
     Shard &sh = shards_[h % kShards];
@@ -267,17 +276,17 @@ class Dictionary {
     auto it = sh.map.find(s);
     if (it != sh.map.end()) return it->second;
     char *dst = sh.store(s);
-    const uint32_t c = (uint32_t)(h % kShards) + kShards * (uint32_t)sh.map.size();
+    const uint32_t idx = (uint32_t)sh.map.size();
+    const uint32_t c = (uint32_t)(h % kShards) + kShards * idx;
     sh.map.emplace(std::string_view(dst, s.size()), c);
-    uint64_t b = bound_.load();
-    while (b < (uint64_t)c + 1 && !bound_.compare_exchange_weak(b, (uint64_t)c + 1)) {
-    }
+    sh.count.store(idx + 1, std::memory_order_release);
     return c;
   }
 
  private:
-  struct Shard {
+  struct alignas(64) Shard {
     std::mutex mu;
+    std::atomic<uint32_t> count{0};
     std::unordered_map<std::string_view, uint32_t> map;
     std::vector<std::unique_ptr<char[]>> blocks;
     size_t used = 0;
@@ -294,26 +303,104 @@ class Dictionary {
   };
   static constexpr size_t kBlock = 1 << 16;
   Shard shards_[kShards];
-  std::atomic<uint64_t> bound_{1};
 };
 
-// Runs fn(begin, end) over [0, n) in chunks on up to 16 threads (one when n
-// is below min_chunk).
+// Host worker threads (up to 15, plus the calling thread), started once and
+// shared by every context: table packing and dictionary coding split rows
+// over them.  One parallel loop runs at a time; other callers wait.  Each
+// loop has its own task counter, so a worker that wakes late for a finished
+// loop finds no task left in it.
+class WorkerPool {
+ public:
+  static WorkerPool &get() {
+    static WorkerPool pool;
+    return pool;
+  }
+  int size() const { return (int)workers_.size() + 1; }
+  // Runs task(k) for k in [0, n_tasks) and returns when all are done.
+  void run(int n_tasks, const std::function<void(int)> &task) {
+    std::lock_guard<std::mutex> one(run_mu_);
+    auto r = std::make_shared<Run>();
+    r->task = &task;
+    r->n = n_tasks;
+    r->pending = n_tasks;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      cur_ = r;
+      gen_++;
+    }
+    cv_.notify_all();
+    work(*r);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_cv_.wait(lk, [&] { return r->pending == 0; });
+    cur_.reset();
+  }
+
+ private:
+  struct Run {
+    const std::function<void(int)> *task = nullptr;
+    int n = 0;
+    int pending = 0;  // guarded by mu_
+    std::atomic<int> next{0};
+  };
+  WorkerPool() {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    for (int k = 0; k < std::min(hw, 16) - 1; k++) workers_.emplace_back([this] { loop(); });
+  }
+  ~WorkerPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : workers_) t.join();
+  }
+  void work(Run &r) {
+    for (;;) {
+      const int k = r.next.fetch_add(1);
+      if (k >= r.n) return;
+      (*r.task)(k);  // r.task outlives every task not yet counted done
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--r.pending == 0) done_cv_.notify_all();
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      std::shared_ptr<Run> r;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        r = cur_;
+      }
+      if (r) work(*r);
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex run_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  std::shared_ptr<Run> cur_;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// Runs fn(begin, end) over [0, n) in chunks on the worker pool (on the
+// calling thread alone when n is below min_chunk).
 template <class F>
 void parallel_for(int64_t n, int64_t min_chunk, F fn) {
-  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
-  const int64_t T = std::min<int64_t>({hw, 16, std::max<int64_t>(1, n / std::max<int64_t>(min_chunk, 1))});
+  WorkerPool &pool = WorkerPool::get();
+  const int64_t T = std::min<int64_t>(pool.size(), std::max<int64_t>(1, n / std::max<int64_t>(min_chunk, 1)));
   if (T <= 1) {
     fn((int64_t)0, n);
     return;
   }
-  std::vector<std::thread> th;
   const int64_t step = (n + T - 1) / T;
-  for (int64_t k = 0; k < T; k++) {
+  pool.run((int)T, [&](int k) {
     const int64_t b = k * step, e = std::min(n, b + step);
-    if (b < e) th.emplace_back([=] { fn(b, e); });
-  }
-  for (auto &t : th) t.join();
+    if (b < e) fn(b, e);
+  });
 }
 
 }  // namespace hbrt
@@ -365,6 +452,7 @@ struct Column {
   uint16_t *d_len = nullptr;
   hb::SigRow *d_sig = nullptr;
   uint32_t *d_planes = nullptr;  // bit-planes of the first 32*nwc bytes
+  uint32_t *d_hist = nullptr;    // 128-class counters
   uint32_t nwc = 0;
   // equality predicates
   Lazy code;
@@ -662,6 +750,7 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
       col.d_sig = pc.d_sig + off;
       col.nwc = pc.nwc;
       col.d_planes = pc.d_planes + off * pc.nwc * 8;
+      col.d_hist = pc.d_hist + off * hb::kHistWords;
       col.sig.built = true;
     }
     return HB_OK;
@@ -673,8 +762,10 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   HB_TRY(t->dev_alloc(n * sizeof(hb::SigRow), (void **)&col.d_sig));
   col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
   HB_TRY(t->dev_alloc(n * col.nwc * 32, (void **)&col.d_planes));
+  HB_TRY(t->dev_alloc(n * hb::kHistWords * 4, (void **)&col.d_hist));
   hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
   hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
+  hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
   HIP_OK(hipGetLastError());
   return record_lazy(col.sig, stream);
 }
@@ -701,10 +792,21 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
     const char *bytes = col.h_bytes;
     uint32_t *codes = col.h_code;
     const uint32_t w = col.width;
-    parallel_for((int64_t)n, 1 << 15, [&](int64_t b, int64_t e) {
+    parallel_for((int64_t)n, 4096, [&](int64_t b, int64_t e) {
+      // a thread-local cache in front of the shared dictionary: columns with
+      // few distinct values would otherwise have every thread wait on the
+      // same few shard locks
+      std::unordered_map<std::string_view, uint32_t> local;
       for (int64_t r = b; r < e; r++) {
         const char *s = bytes + (size_t)r * w;
-        codes[r] = d.code(std::string_view(s, strnlen(s, w)));
+        const std::string_view v(s, strnlen(s, w));
+        auto it = local.find(v);
+        if (it != local.end()) {
+          codes[r] = it->second;
+          continue;
+        }
+        codes[r] = d.code(v);
+        if (local.size() < 4096) local.emplace(v, codes[r]);
       }
     });
   }
@@ -718,16 +820,18 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
 // several: the composite codes, built from the columns' codes on the host,
 // shared by every table of the context through the dictionary).
 hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stream,
-                     const uint32_t **out) {
+                     const uint32_t **out, const uint32_t **h_out) {
   if (cols.size() == 1) {
     HB_TRY(ensure_code(t, cols[0], stream));
     *out = t->cols[cols[0]].d_code;
+    *h_out = t->cols[cols[0]].h_code;
     return HB_OK;
   }
   if (t->parent) {  // borrow the parent's composite codes at the slice's offset
-    const uint32_t *pc = nullptr;
-    HB_TRY(ensure_key(t->parent, cols, stream, &pc));
+    const uint32_t *pc = nullptr, *ph = nullptr;
+    HB_TRY(ensure_key(t->parent, cols, stream, &pc, &ph));
     *out = pc + t->row_off;
+    *h_out = ph + t->row_off;
     return HB_OK;
   }
   for (int c : cols) HB_TRY(ensure_code(t, c, stream));
@@ -735,6 +839,7 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
   Composite &cp = t->composites[cols];
   if (cp.code.built) {
     *out = cp.d_code;
+    *h_out = cp.h_code;
     return wait_lazy(cp.code, stream);
   }
   const size_t n = (size_t)t->n_rows;
@@ -745,13 +850,20 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
     for (int c : cols) src.push_back(t->cols[c].h_code);
     uint32_t *codes = cp.h_code;
     const size_t k = cols.size();
-    parallel_for((int64_t)n, 1 << 15, [&](int64_t b, int64_t e) {
+    parallel_for((int64_t)n, 4096, [&](int64_t b, int64_t e) {
       // key bytes: a marker byte, then the column codes (4 bytes each)
       std::string key(1 + 4 * k, '\0');
       key[0] = (char)(0xC0 + k);
+      std::unordered_map<std::string, uint32_t> local;  // as in ensure_code
       for (int64_t r = b; r < e; r++) {
         for (size_t q = 0; q < k; q++) std::memcpy(&key[1 + 4 * q], &src[q][r], 4);
+        auto it = local.find(key);
+        if (it != local.end()) {
+          codes[r] = it->second;
+          continue;
+        }
         codes[r] = d.code(key);
+        if (local.size() < 4096) local.emplace(key, codes[r]);
       }
     });
   }
@@ -759,7 +871,23 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
   if (n) HIP_OK(hipMemcpyAsync(cp.d_code, cp.h_code, n * sizeof(uint32_t),
                                hipMemcpyHostToDevice, stream));
   *out = cp.d_code;
+  *h_out = cp.h_code;
   return record_lazy(cp.code, stream);
+}
+
+// Number of distinct values among codes[0..n) (all below bound).
+uint32_t count_distinct(const uint32_t *codes, size_t n, uint64_t bound) {
+  if (n == 0) return 0;
+  if (bound <= 64 * (uint64_t)n + 4096) {
+    std::vector<uint64_t> bits((size_t)((bound + 63) / 64), 0);
+    for (size_t r = 0; r < n; r++) bits[codes[r] >> 6] |= 1ull << (codes[r] & 63);
+    uint64_t c = 0;
+    for (uint64_t w : bits) c += (uint64_t)__builtin_popcountll(w);
+    return (uint32_t)c;
+  }
+  std::vector<uint32_t> v(codes, codes + n);
+  std::sort(v.begin(), v.end());
+  return (uint32_t)(std::unique(v.begin(), v.end()) - v.begin());
 }
 
 // Builds (once per table and rule shape) the join-order layout: keys from the
@@ -767,9 +895,10 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
 // radix-sorted with the row ids, the runs of equal code, and the similarity
 // signatures gathered into that order.
 //
-// The build runs on the context's build stream, which the host waits for
-// (the run count sizes the batch's groups); jobs' streams wait on the
-// layout's event, so a build never waits for other batches' kernels.
+// The build runs on the context's build stream; jobs' streams wait on the
+// layout's event, so a build never waits for other batches' kernels, and the
+// host never waits for the build (the run count, which sizes the batch's
+// groups, is counted on the host).
 hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStream_t caller,
                         Layout **out) {
   {
@@ -782,8 +911,8 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   }
   const hipStream_t stream = t->ctx->build_stream;
   HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
-  const uint32_t *key_codes = nullptr;
-  if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes));
+  const uint32_t *key_codes = nullptr, *h_key = nullptr;
+  if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes, &h_key));
   if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
   uint64_t n_codes = 1;
   if (!eq.empty()) n_codes = t->ctx->dict.code_bound();
@@ -799,15 +928,24 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   HB_TRY(t->dev_alloc((n1 + 1) * sizeof(uint32_t), (void **)&ly.d_run_off));
   if (sim >= 0) HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig));
   if (n) {
+    // The run count (distinct keys) is counted on the host from the codes it
+    // holds, so the build is asynchronous: batches wait on its event, and its
+    // temporaries return to the pool once the build stream passes them.
+    ly.n_runs = eq.empty() ? 1u : count_distinct(h_key, n, n_codes);
     DevicePool &pool = t->ctx->pool;
+    hb_ctx::Grave temps;
+    auto temp = [&](size_t bytes, void **p) -> hb_status {
+      HB_TRY(pool.alloc(bytes, p));
+      temps.dev.push_back({*p, bytes});
+      return HB_OK;
+    };
     uint64_t *keys_in = nullptr;
-    uint32_t *iota = nullptr, *flags = nullptr, *run_id = nullptr, *h_nruns = nullptr;
-    uint32_t *d_nruns = nullptr;
-    HB_TRY(pool.alloc(n1 * sizeof(uint64_t), (void **)&keys_in));
-    HB_TRY(pool.alloc(n1 * sizeof(uint32_t), (void **)&iota));
-    HB_TRY(pool.alloc(n1 * sizeof(uint32_t), (void **)&flags));
-    HB_TRY(pool.alloc((n1 + 1) * sizeof(uint32_t), (void **)&run_id));
-    HB_TRY(t->ctx->hpool.alloc(sizeof(uint32_t), (void **)&h_nruns, (void **)&d_nruns));
+    uint32_t *iota = nullptr, *flags = nullptr, *run_id = nullptr, *d_nruns = nullptr;
+    HB_TRY(temp(n1 * sizeof(uint64_t), (void **)&keys_in));
+    HB_TRY(temp(n1 * sizeof(uint32_t), (void **)&iota));
+    HB_TRY(temp(n1 * sizeof(uint32_t), (void **)&flags));
+    HB_TRY(temp((n1 + 1) * sizeof(uint32_t), (void **)&run_id));
+    HB_TRY(temp(sizeof(uint32_t), (void **)&d_nruns));
     hb::launch_layout_keys(key_codes,
                            sim >= 0 ? t->cols[sim].d_len : nullptr, n, keys_in, stream);
     hb::launch_iota(iota, n, stream);
@@ -819,7 +957,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     HIP_OK(hb::exscan_u32(nullptr, ts, flags, run_id, n, stream));
     void *tmp = nullptr;
     const size_t tmp_bytes = std::max<size_t>(std::max(tb, ts), 16);
-    HB_TRY(pool.alloc(tmp_bytes, &tmp));
+    HB_TRY(temp(tmp_bytes, &tmp));
     HIP_OK(hb::sort_pairs_u64(tmp, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit, stream));
     hb::launch_run_flags(ly.d_keys, n, flags, stream);
     HIP_OK(hb::exscan_u32(tmp, ts, flags, run_id, n, stream));
@@ -827,14 +965,12 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
                            stream);
     if (sim >= 0) hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(stream));  // one-off per layout: run count, frees below
-    ly.n_runs = *h_nruns;
-    pool.free(tmp, tmp_bytes);
-    pool.free(keys_in, n1 * sizeof(uint64_t));
-    pool.free(iota, n1 * sizeof(uint32_t));
-    pool.free(flags, n1 * sizeof(uint32_t));
-    pool.free(run_id, (n1 + 1) * sizeof(uint32_t));
-    t->ctx->hpool.free(h_nruns, sizeof(uint32_t));
+    hipEvent_t e = nullptr;
+    HB_TRY(t->ctx->events.get(&e));
+    temps.wait.push_back(e);
+    HIP_OK(hipEventRecord(e, stream));
+    std::lock_guard<std::mutex> gl(t->ctx->grave_mu);
+    t->ctx->graves.push_back(std::move(temps));
   }
   HB_TRY(record_lazy(ly.lz, stream));
   return wait_lazy(ly.lz, caller);
@@ -854,6 +990,10 @@ bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
          tile_b <= 65536 && tile_b % 256 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
 }
 
+// First similarity thresholds below this get the refine pass: there the
+// 32-class filter bound keeps many pairs whose JW is far below threshold.
+constexpr float kRefineBelow = 0.85f;
+
 // Filter variant of a JR: the kernel template it needs (with or without a
 // similarity predicate; every equality is part of the join key).
 int filter_variant(const PlanRule &rule) { return rule.sims.empty() ? 0 : 1; }
@@ -864,10 +1004,31 @@ unsigned long long pow2_at_least(unsigned long long x) {
   return p;
 }
 
+// Host time of the phases of enqueue_batch, printed to stderr when the
+// environment sets HB_HOST_TRACE (a diagnostic for host-bound pipelines).
+struct HostTrace {
+  bool on = std::getenv("HB_HOST_TRACE") != nullptr;
+  Clock::time_point t0 = Clock::now(), last = t0;
+  std::string line;
+  void mark(const char *what) {
+    if (!on) return;
+    const auto now = Clock::now();
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), " %s %.3f", what,
+                  std::chrono::duration<double, std::milli>(now - last).count());
+    line += buf;
+    last = now;
+  }
+  ~HostTrace() {
+    if (on) std::fprintf(stderr, "enqueue_batch ms:%s\n", line.c_str());
+  }
+};
+
 hb_status enqueue_batch(Batch *b) {
   hb_ctx *ctx = b->ctx;
   hipStream_t st = b->stream;
   const uint32_t n_jobs = b->n_jobs();
+  HostTrace trace;
 
   // Tables ready, lazy structures and layouts; job-rule descriptors.
   struct JR {
@@ -887,6 +1048,7 @@ hb_status enqueue_batch(Batch *b) {
         HB_TRY(ensure_sig(it.L, s.cl, st));
         HB_TRY(ensure_sig(it.R, s.cr, st));
       }
+      trace.mark("sig");
       JR jr{j, k, filter_variant(rule), {nullptr, nullptr}};
       std::vector<int> el, er;
       for (const auto &e : rule.eq) {
@@ -897,6 +1059,7 @@ hb_status enqueue_batch(Batch *b) {
       const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
       HB_TRY(ensure_layout(it.L, el, sl, st, &jr.ly[0]));
       HB_TRY(ensure_layout(it.R, er, sr, st, &jr.ly[1]));
+      trace.mark("layout");
       jrs.push_back(jr);
     }
   }
@@ -907,6 +1070,7 @@ hb_status enqueue_batch(Batch *b) {
   std::vector<hb::JRDesc> hd(n_jr);
   std::vector<uint32_t> jr_group_off(n_jr + 1, 0);
   std::vector<hb::SimDesc> sims;
+  bool any_refine = false;
   b->max_stage = 1;
   for (uint32_t q = 0; q < n_jr; q++) {
     const JR &jr = jrs[q];
@@ -934,14 +1098,22 @@ hb_status enqueue_batch(Batch *b) {
     d.n_sims = (int32_t)rule.sims.size();
     d.sim_off = (uint32_t)sims.size();
     d.multi = it.plan->rules.size() > 1 ? 1u : 0u;
+    d.refine = (!rule.sims.empty() && rule.sims[0].thr < kRefineBelow) ? 1u : 0u;
+    any_refine |= d.refine != 0;
     for (const auto &s : rule.sims) {
       const Column &cl = it.L->cols[s.cl];
       const Column &cr = it.R->cols[s.cr];
       hb::SimDesc x{};
-      x.col_l = hb::DevColumn{cl.d_bytes, cl.d_len, cl.width, cl.d_planes, cl.nwc};
-      x.col_r = hb::DevColumn{cr.d_bytes, cr.d_len, cr.width, cr.d_planes, cr.nwc};
+      x.col_l = hb::DevColumn{cl.d_bytes, cl.d_len, cl.width, cl.d_planes, cl.nwc, cl.d_sig,
+                              cl.d_hist};
+      x.col_r = hb::DevColumn{cr.d_bytes, cr.d_len, cr.width, cr.d_planes, cr.nwc, cr.d_sig,
+                              cr.d_hist};
       x.thr = s.thr;
       x.slot = s.slot;
+      x.c0 = filter_coef((double)s.thr, 0);
+      x.c1 = filter_coef((double)s.thr, 1);
+      x.c2 = filter_coef((double)s.thr, 2);
+      x.c3 = filter_coef((double)s.thr, 3);
       sims.push_back(x);
     }
     b->max_stage = std::max(b->max_stage, (int)rule.sims.size());
@@ -949,10 +1121,11 @@ hb_status enqueue_batch(Batch *b) {
   }
   const uint32_t n_groups = jr_group_off[n_jr];
 
+  trace.mark("desc");
   // Counter block: fixed slots, list counts per stage (0..max_stage), defer
   // counts per stage, per-job counters, per-job fill counters, per-job
   // output and audit offsets (scans).
-  const int n_stage_slots = b->max_stage + 1;
+  const int n_stage_slots = b->max_stage + 2;
   const uint32_t stride = b->stride();
   b->cnt_stage_off = kCntFixed;
   b->cnt_job_off = b->cnt_stage_off + 2 * n_stage_slots;
@@ -1058,6 +1231,7 @@ hb_status enqueue_batch(Batch *b) {
                                  sizeof(hb_audit_entry), &hp, &daud));
   b->h_audit = (hb_audit_entry *)hp;
 
+  trace.mark("alloc");
   // Launch sequence.
   HIP_OK(hipEventRecord(b->ev_start, st));
   HIP_OK(hipMemcpyAsync(ddesc, hdesc, o, hipMemcpyHostToDevice, st));
@@ -1084,11 +1258,17 @@ hb_status enqueue_batch(Batch *b) {
   HIP_OK(hipEventRecord(b->marks[1], st));
   const hb::PairList fin{final_buf, d_cnt + kCntFinal, b->final_cap};
   const hb::AuditList aud{audit_buf, d_cnt + kCntAudit, b->audit_cap};
+  // list slots: 0 filter output, then (1 refined and) one per exact stage
+  const int off = any_refine ? 1 : 0;
+  if (any_refine)
+    hb::launch_refine(bd, list0, hb::PairList{lists[1], list_cnt + 1, b->list_cap}, ctx->n_cu,
+                      st);
   for (int s = 1; s <= b->max_stage; s++) {
-    const hb::PairList in{lists[(s - 1) & 1], list_cnt + (s - 1), b->list_cap};
-    const hb::PairList next{lists[s & 1], list_cnt + s, b->list_cap};
+    const int q = s - 1 + off;  // slot of this stage's input
+    const hb::PairList in{lists[q & 1], list_cnt + q, b->list_cap};
+    const hb::PairList next{lists[(q + 1) & 1], list_cnt + q + 1, b->list_cap};
     const hb::PairList defer{defer_buf, defer_cnt + s, b->defer_cap};
-    hb::launch_exact(bd, in, s, next, fin, defer, aud, ctx->n_cu, st);
+    hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, ctx->n_cu, st);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b->marks[2], st));
@@ -1110,13 +1290,14 @@ hb_status enqueue_batch(Batch *b) {
   HIP_OK(hipMemcpyAsync(b->h_cnt, d_cnt, b->cnt_words * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, st));
   HIP_OK(hipEventRecord(b->ev_end, st));
+  trace.mark("launch");
   return HB_OK;
 }
 
 // Called with b->mu held once ev_end has completed.
 hb_status batch_complete(Batch *b) {
   const unsigned long long *c = b->h_cnt;
-  const int n_stage_slots = b->max_stage + 1;
+  const int n_stage_slots = b->max_stage + 2;
   unsigned long long need_list = 0, need_defer = 0;
   for (int s = 0; s < n_stage_slots; s++) {
     need_list = std::max(need_list, c[b->cnt_stage_off + s]);
@@ -1307,6 +1488,7 @@ size_t job_memory_estimate(const Job *j) {
 hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
                        std::shared_ptr<Batch> *out = nullptr) {
   HB_TRY(set_device(ctx));
+  HB_TRY(reap_graves(ctx, false));  // memory of released tables and finished layout builds
   auto b = std::make_shared<Batch>();
   b->ctx = ctx;
   b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
@@ -1560,7 +1742,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
   // cell widths: longest cell + NUL, rounded up to 16
   std::vector<size_t> maxlen(nc, 0);
   std::mutex mm;
-  parallel_for(n_rows, 1 << 14, [&](int64_t b, int64_t e) {
+  parallel_for(n_rows, 4096, [&](int64_t b, int64_t e) {
     std::vector<size_t> ml(nc, 0);
     for (int c = 0; c < nc; c++) {
       const int64_t stride = v->col_stride[c];
@@ -1579,7 +1761,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
     if (n_rows == 0) std::memset(col.h_bytes, 0, bytes);
   }
   // pack (each cell NUL padded to its column's width)
-  parallel_for(n_rows, 1 << 14, [&](int64_t b, int64_t e) {
+  parallel_for(n_rows, 4096, [&](int64_t b, int64_t e) {
     for (int c = 0; c < nc; c++) {
       Column &col = t->cols[c];
       const int64_t stride = v->col_stride[c];

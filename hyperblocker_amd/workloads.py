@@ -305,10 +305,11 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
     def stream():
         t0 = time.perf_counter()
         pending = collections.deque()
-        out = {"matches": 0, "device_ms": 0.0, "upload_s": 0.0, "candidates": 0,
-               "jw_exact": 0, "audit": 0}
+        out = {"matches": 0, "device_ms": 0.0, "upload_s": 0.0, "submit_s": 0.0,
+               "drain_s": 0.0, "candidates": 0, "jw_exact": 0, "audit": 0}
 
         def drain():
+            td = time.perf_counter()
             job, tl, tr = pending.popleft()
             pairs, audit, st = job.results()
             job.release()
@@ -319,12 +320,15 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
             out["device_ms"] += st["device_ms"]
             out["candidates"] += st["candidates"]
             out["jw_exact"] += st["jw_exact"]
+            out["drain_s"] += time.perf_counter() - td
 
         for lr, rr in parts:
             tu = time.perf_counter()
             tl, tr = ctx.upload(lt, rows=lr), ctx.upload(rt, rows=rr)
-            out["upload_s"] += time.perf_counter() - tu
+            ts = time.perf_counter()
+            out["upload_s"] += ts - tu
             pending.append((ctx.submit(dp, tl, tr), tl, tr))
+            out["submit_s"] += time.perf_counter() - ts
             if len(pending) >= inflight:
                 drain()
         while pending:
@@ -336,8 +340,9 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
         runs = []
         for k in range(reps + 1):
             r = stream()
-            log("c3: stream %d: %.3f s (uploads %.3f s on the host thread, device %.1f ms)"
-                % (k, r["wall_s"], r["upload_s"], r["device_ms"]))
+            log("c3: stream %d: %.3f s (host: uploads %.3f s, submits %.3f s, drains %.3f s;"
+                " device %.1f ms)" % (k, r["wall_s"], r["upload_s"], r["submit_s"], r["drain_s"],
+                                      r["device_ms"]))
             runs.append(r)
         best = min(runs[1:], key=lambda r: r["wall_s"])
         return {
@@ -345,7 +350,8 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
             "streamed_s": best["wall_s"], "streamed_pairs_per_s": pairs_total / best["wall_s"],
             "first_stream_s": runs[0]["wall_s"], "device_ms": best["device_ms"],
             "device_pairs_per_s": pairs_total / (best["device_ms"] * 1e-3),
-            "host_upload_s": best["upload_s"], "matches": best["matches"],
+            "host_upload_s": best["upload_s"], "host_submit_s": best["submit_s"],
+            "host_drain_s": best["drain_s"], "matches": best["matches"],
             "candidates": best["candidates"], "jw_exact": best["jw_exact"],
             "audit": best["audit"], "inflight": inflight,
         }
