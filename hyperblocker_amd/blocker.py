@@ -248,6 +248,34 @@ class Context:
         check(lib().hb_batch_submit(self._h, n, plans, ls, rs, idarr))
         return [Job(self, i) for i in ids]
 
+    def batch_results(self, job):
+        """All results of the batch `job` ran in (hb_batch_results): (job ids,
+        pairs uint32[n, 2], pair offsets uint64[n_jobs + 1], audit entries,
+        audit offsets), copied out.  Job k's pairs are
+        pairs[pair_off[k]:pair_off[k + 1]]."""
+        n = C.c_uint32()
+        ids, po, ao = (C.POINTER(C.c_uint64)() for _ in range(3))
+        p = C.POINTER(C.c_uint32)()
+        a = C.POINTER(capi.AuditEntry)()
+        check(lib().hb_batch_results(self._h, job.id, C.byref(n), C.byref(ids), C.byref(p),
+                                     C.byref(po), C.byref(a), C.byref(ao)))
+        nj = n.value
+        ids_a = np.ctypeslib.as_array(ids, (nj,)).copy()
+        po_a = np.ctypeslib.as_array(po, (nj + 1,)).copy()
+        ao_a = np.ctypeslib.as_array(ao, (nj + 1,)).copy()
+        n_pairs, n_audit = int(po_a[-1]), int(ao_a[-1])
+        pairs = (np.ctypeslib.as_array(p, (2 * n_pairs,)).reshape(-1, 2).copy()
+                 if n_pairs else np.zeros((0, 2), np.uint32))
+        audit = (np.frombuffer(C.string_at(a, n_audit * capi.AUDIT_DTYPE.itemsize),
+                               dtype=capi.AUDIT_DTYPE).copy()
+                 if n_audit else np.zeros(0, capi.AUDIT_DTYPE))
+        return ids_a, pairs, po_a, audit, ao_a
+
+    def release_jobs(self, jobs):
+        """hb_job_release of every job, in one call."""
+        ids = (C.c_uint64 * len(jobs))(*[j.id for j in jobs])
+        check(lib().hb_jobs_release(self._h, len(jobs), ids))
+
     def autotune(self, jobs, candidates=None):
         """Picks the bound filter's launch shape for these jobs (list of
         (plan, l_table, r_table)) and makes it the context's; returns

@@ -33,7 +33,7 @@ EXPORTS = [
     "hb_table_slice", "hb_table_info", "hb_table_release",
     "hb_plan_create", "hb_plan_release",
     "hb_job_submit", "hb_batch_submit", "hb_job_query", "hb_job_wait", "hb_job_results",
-    "hb_job_stats_get", "hb_job_release", "hb_block", "hb_free", "hb_autotune_filter_shape",
+    "hb_job_stats_get", "hb_job_release", "hb_batch_results", "hb_jobs_release", "hb_block", "hb_free", "hb_autotune_filter_shape",
     "hb_server_create", "hb_server_submit", "hb_server_info", "hb_server_destroy",
     "hb_rules_load", "hb_rules_parse", "hb_host_plan_view", "hb_host_plan_free",
     "hb_csv_load", "hb_host_table_view", "hb_host_table_cell", "hb_host_table_free",
@@ -138,6 +138,11 @@ def lib():
                             C.POINTER(C.c_uint64)], st),
         "hb_job_stats_get": ([VP, C.c_uint64, C.POINTER(JobStats)], st),
         "hb_job_release": ([VP, C.c_uint64], st),
+        "hb_batch_results": ([VP, C.c_uint64, C.POINTER(C.c_uint32),
+                              C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.POINTER(C.c_uint32)),
+                              C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.POINTER(AuditEntry)),
+                              C.POINTER(C.POINTER(C.c_uint64))], st),
+        "hb_jobs_release": ([VP, C.c_uint32, C.POINTER(C.c_uint64)], st),
         "hb_block": ([VP, VP, VP, VP, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint64),
                       C.POINTER(C.POINTER(AuditEntry)), C.POINTER(C.c_uint64),
                       C.POINTER(JobStats)], st),

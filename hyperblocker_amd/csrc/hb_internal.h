@@ -223,9 +223,10 @@ void launch_refine(BatchDev b, PairList in, PairList out, int n_cu, hipStream_t 
 // similarity predicates pass their pairs to the output directly; pairs
 // passing their last predicate go to `final_` (deduplicated per job where
 // needed), the others to `next`.  Pairs with a shorter string above 256
-// bytes run one wave per pair (second kernel, via `defer`).
+// bytes run one wave per pair (second kernel, via `defer`; not launched
+// when `may_defer` is false, i.e. no column pair can hold such a pair).
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
-                  PairList defer, AuditList audit, bool count_filtered, int n_cu,
+                  PairList defer, AuditList audit, bool count_filtered, bool may_defer, int n_cu,
                   hipStream_t stream);
 
 // Copies the final list and the audit list into per-job contiguous ranges:

@@ -672,6 +672,9 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
         if (lane_id() == 0) s_starts[tid >> 6] = m;
         __syncthreads();
       }
+      // read once per chunk: `to.dense` changes only inside flushes, and a
+      // chunk that starts sparse flushes only at its end
+      const bool dense = dense_item || to.dense;
       uint32_t run = 0;
       while (run < n_b) {  // wave-uniform
         uint32_t run_end = n_b;
@@ -701,7 +704,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
           // entries, so it never overflows.  Every wave of the group runs
           // the same (run, jj) sequence and reads the same `dense` (written
           // only between barriers), so all reach the barrier together.
-          if ((dense_item || to.dense) && jj != run) {
+          if (dense && jj != run) {
             __syncthreads();
             if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
           }
@@ -1408,11 +1411,12 @@ void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, 
 }
 
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
-                  PairList defer, AuditList audit, bool count_filtered, int n_cu,
+                  PairList defer, AuditList audit, bool count_filtered, bool may_defer, int n_cu,
                   hipStream_t stream) {
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
   ExactArgs a{b, in, stage, count_filtered ? 1 : 0, next, final_, defer, audit};
   hipLaunchKernelGGL(k_exact_lanes, dim3(cus * 8), dim3(256), 0, stream, a);
+  if (!may_defer) return;
   ExactArgs d{b, defer, stage, 0, next, final_, defer, audit};
   d.defer.cap = 0;  // deferred pairs are not deferred again
   hipLaunchKernelGGL(k_exact, dim3(cus * 4), dim3(256), 0, stream, d);

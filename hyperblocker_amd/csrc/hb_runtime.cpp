@@ -643,6 +643,7 @@ struct Batch {
     hb_table *L, *R;
   };
   std::vector<Item> items;
+  std::vector<uint64_t> job_ids;  // id of each job of the batch
   hb::FilterShape shape{2, 256, 16};
   int max_stage = 1;             // exact stages launched (>= 1)
   // capacities (entries)
@@ -1071,6 +1072,7 @@ hb_status enqueue_batch(Batch *b) {
   std::vector<uint32_t> jr_group_off(n_jr + 1, 0);
   std::vector<hb::SimDesc> sims;
   bool any_refine = false;
+  bool may_defer = false;  // some column pair can hold a pair with both cells over 256 bytes
   b->max_stage = 1;
   for (uint32_t q = 0; q < n_jr; q++) {
     const JR &jr = jrs[q];
@@ -1110,6 +1112,7 @@ hb_status enqueue_batch(Batch *b) {
                               cr.d_hist};
       x.thr = s.thr;
       x.slot = s.slot;
+      may_defer |= std::min(cl.width, cr.width) > 257;
       x.c0 = filter_coef((double)s.thr, 0);
       x.c1 = filter_coef((double)s.thr, 1);
       x.c2 = filter_coef((double)s.thr, 2);
@@ -1268,7 +1271,8 @@ hb_status enqueue_batch(Batch *b) {
     const hb::PairList in{lists[q & 1], list_cnt + q, b->list_cap};
     const hb::PairList next{lists[(q + 1) & 1], list_cnt + q + 1, b->list_cap};
     const hb::PairList defer{defer_buf, defer_cnt + s, b->defer_cap};
-    hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, ctx->n_cu, st);
+    hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, may_defer,
+                     ctx->n_cu, st);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b->marks[2], st));
@@ -1498,6 +1502,7 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
   for (uint32_t j = 0; j < n; j++) {
     Batch::Item it{jobs[j]->plan, jobs[j]->L, jobs[j]->R};
     b->items.push_back(it);
+    b->job_ids.push_back(jobs[j]->id);
     const unsigned long long jr = (unsigned long long)(it.L->n_rows + it.R->n_rows);
     const SizeHint h = it.plan->get_hint();
     want.list += h.list * jr;
@@ -2023,6 +2028,30 @@ hb_status hb_job_release(hb_ctx *ctx, uint64_t job_id) {
   }
   HB_TRY(set_device(ctx));
   job.reset();  // the batch's memory returns to the pools with its last job
+  return HB_OK;
+}
+
+hb_status hb_batch_results(hb_ctx *ctx, uint64_t job_id, uint32_t *n_jobs,
+                           const uint64_t **job_ids, const uint32_t **pairs,
+                           const uint64_t **pair_off, const hb_audit_entry **audit,
+                           const uint64_t **audit_off) {
+  Job *j = ctx ? find_job(ctx, job_id) : nullptr;
+  if (!j) return fail(HB_ERR_NOT_FOUND, "unknown job %llu", (unsigned long long)job_id);
+  Batch *b = job_batch(j, true).get();
+  HB_TRY(batch_poll(b, true));
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (n_jobs) *n_jobs = b->n_jobs();
+  if (job_ids) *job_ids = b->job_ids.data();
+  if (pairs) *pairs = b->h_out;
+  if (pair_off) *pair_off = (const uint64_t *)(b->h_cnt + b->cnt_out_off);
+  if (audit) *audit = b->h_audit;
+  if (audit_off) *audit_off = (const uint64_t *)(b->h_cnt + b->cnt_audit_off);
+  return HB_OK;
+}
+
+hb_status hb_jobs_release(hb_ctx *ctx, uint32_t n, const uint64_t *job_ids) {
+  if (!ctx || (n && !job_ids)) return fail(HB_ERR_INVALID_ARG, "hb_jobs_release: null argument");
+  for (uint32_t k = 0; k < n; k++) HB_TRY(hb_job_release(ctx, job_ids[k]));
   return HB_OK;
 }
 

@@ -126,8 +126,10 @@ def run_c4(ctx, n_jobs=4096, reps=2, inflight=4, autotune=True, log=print):
         for _ in range(reps):
             t = time.perf_counter()
             handles = ctx.submit_batch(jobs)
+            _, pairs, _, _, _ = ctx.batch_results(handles[0])  # every job's results
             stats = handles[0].stats()
-            matches = _drain(handles)
+            matches = len(pairs)
+            ctx.release_jobs(handles)
             batched.append(time.perf_counter() - t)
         # plain: one job per submit, a few in flight
         plain = []

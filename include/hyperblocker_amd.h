@@ -213,6 +213,20 @@ hb_status hb_job_results(hb_ctx *ctx, uint64_t job_id, const uint32_t **pairs,
 hb_status hb_job_stats_get(hb_ctx *ctx, uint64_t job_id, hb_job_stats *out);
 hb_status hb_job_release(hb_ctx *ctx, uint64_t job_id);
 
+/* Results of the whole batch that job_id ran in, in one block (no
+ * counterpart in the reference; for callers of hb_batch_submit that read
+ * thousands of jobs): job k of the batch has id job_ids[k], its pairs are
+ * pairs[2*pair_off[k] .. 2*pair_off[k+1]) and its audit entries
+ * audit[audit_off[k] .. audit_off[k+1]), as hb_job_results gives them.
+ * Waits for the batch.  The arrays are library-owned and valid while job_id
+ * is not released. */
+hb_status hb_batch_results(hb_ctx *ctx, uint64_t job_id, uint32_t *n_jobs,
+                           const uint64_t **job_ids, const uint32_t **pairs,
+                           const uint64_t **pair_off, const hb_audit_entry **audit,
+                           const uint64_t **audit_off);
+/* hb_job_release of n jobs in one call (no counterpart in the reference). */
+hb_status hb_jobs_release(hb_ctx *ctx, uint32_t n, const uint64_t *job_ids);
+
 /* Synchronous convenience: submit + wait + copy results into malloc'd
  * arrays the caller frees with hb_free. */
 hb_status hb_block(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,

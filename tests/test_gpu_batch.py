@@ -213,3 +213,29 @@ def test_multi_rule_dedupe_set_regrowth(ctx):
         tl.release()
         tr.release()
         dp.release()
+
+
+def test_batch_results_block_equals_each_job(ctx):
+    """hb_batch_results gives every job's pairs and audit entries in one
+    block, with offsets, equal to each job's own hb_job_results."""
+    specs = _jobs(ctx, 12, 17)
+    plans = {n: ctx.plan(hb.HostPlan.from_yaml(PLANS[n])) for n in PLANS}
+    tables = [(ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))) for _, L, R in specs]
+    try:
+        jobs = ctx.submit_batch([(plans[n], tl, tr) for (n, _, _), (tl, tr) in zip(specs, tables)])
+        ids, pairs, po, audit, ao = ctx.batch_results(jobs[5])
+        assert list(ids) == [j.id for j in jobs] and len(po) == len(ao) == len(jobs) + 1
+        for k, j in enumerate(jobs):
+            p, a, st = j.results()
+            H.assert_same_pairs(hb.sort_pairs(pairs[po[k]:po[k + 1]]), hb.sort_pairs(p), "job %d" % k)
+            H.assert_same_audit(audit[ao[k]:ao[k + 1]], a, "job %d" % k)
+            assert st["matches"] == po[k + 1] - po[k]
+        ctx.release_jobs(jobs)
+        with pytest.raises(hb.HBError):
+            jobs[0].stats()
+    finally:
+        for tl, tr in tables:
+            tl.release()
+            tr.release()
+        for p in plans.values():
+            p.release()
