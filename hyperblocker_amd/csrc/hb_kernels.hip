@@ -47,6 +47,26 @@ __device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Values the whole wave holds (loaded from uniform addresses, which the
+// compiler cannot prove uniform), moved to scalar registers.  The builtin
+// returns a signed int: each half is taken back to uint32_t before the
+// halves are joined (an int low half would sign-extend into the high one).
+__device__ __forceinline__ uint32_t wave_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ uint64_t wave_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ float wave_f32(float v) {
+  return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+template <class T>
+__device__ __forceinline__ T *wave_ptr(T *p) {
+  return (T *)wave_u64((uint64_t)p);
+}
+
 __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int lane) {
   uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, lane);
   uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
@@ -601,24 +621,28 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     to.dense = 0;
   }
   const uint32_t tile_b = g.tile_b;
-  const uint64_t item_begin = g.item_off[g_begin], item_end = g.item_off[g_end];
+  // Item-level values are wave-uniform; they are moved to scalar registers
+  // so that the per-pair loop keeps its vector registers.
+  const uint64_t item_begin = wave_u64(g.item_off[g_begin]);
+  const uint64_t item_end = wave_u64(g.item_off[g_end]);
   for (uint64_t item = item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
-    const uint32_t gi = g_begin + find_group(g.item_off + g_begin, g_end - g_begin, item);
-    const uint32_t jr = g.jr[gi];
+    const uint32_t gi =
+        wave_u32(g_begin + find_group(g.item_off + g_begin, g_end - g_begin, item));
+    const uint32_t jr = wave_u32(g.jr[gi]);
     const JRDesc &d = bd.jrs[jr];
-    const float k0 = 2.0f * d.c0, k3 = 2.0f * d.c3;
-    const bool dense_item = !SIM || d.refine != 0;
-    const uint32_t l_off = g.l_off[gi], l_cnt = g.l_cnt[gi];
-    const uint32_t r_off = g.r_off[gi], r_cnt = g.r_cnt[gi];
+    const float k0 = wave_f32(2.0f * d.c0), k3 = wave_f32(2.0f * d.c3);
+    const bool dense_item = !SIM || wave_u32(d.refine) != 0;
+    const uint32_t l_off = wave_u32(g.l_off[gi]), l_cnt = wave_u32(g.l_cnt[gi]);
+    const uint32_t r_off = wave_u32(g.r_off[gi]), r_cnt = wave_u32(g.r_cnt[gi]);
     const bool a_is_l = l_cnt >= r_cnt;
     const uint32_t a_cnt = a_is_l ? l_cnt : r_cnt, b_cnt = a_is_l ? r_cnt : l_cnt;
     const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
     const SideDesc &sa = d.s[a_is_l ? 0 : 1], &sb = d.s[a_is_l ? 1 : 0];
-    const uint32_t *perm_a = sa.perm;
-    const uint32_t *perm_b = sb.perm;
-    const SigRow *sig_a = sa.jsig + a_off;
-    const SigRow *sig_b = sb.jsig + b_off;
-    const uint64_t local = item - g.item_off[gi];
+    const uint32_t *perm_a = wave_ptr(sa.perm);
+    const uint32_t *perm_b = wave_ptr(sb.perm);
+    const SigRow *sig_a = wave_ptr(sa.jsig) + a_off;
+    const SigRow *sig_b = wave_ptr(sb.jsig) + b_off;
+    const uint64_t local = item - wave_u64(g.item_off[gi]);
     const uint32_t n_bseg = (b_cnt + tile_b - 1) / tile_b;
     const uint32_t a_tile = (uint32_t)(local / n_bseg), b_seg = (uint32_t)(local % n_bseg);
     bool a_valid[AR];
@@ -681,11 +705,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
         if (SIM) {
           // next set bit of s_starts after `run`
           for (uint32_t w = (run + 1) >> 6; w < kChunk / 64; w++) {
-            unsigned long long m = __builtin_amdgcn_readfirstlane(
-                                       (uint32_t)s_starts[w]) |
-                                   ((unsigned long long)__builtin_amdgcn_readfirstlane(
-                                        (uint32_t)(s_starts[w] >> 32))
-                                    << 32);
+            unsigned long long m = wave_u64(s_starts[w]);
             if (w == (run + 1) >> 6) m &= ~0ull << ((run + 1) & 63);
             if (m) {
               run_end = min(n_b, (w << 6) + (uint32_t)__builtin_ctzll(m));
@@ -740,15 +760,6 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     __syncthreads();
     if (to.n) tile_flush(to, tc, out, bd.overflow);  // group-uniform after the barrier
   }
-}
-
-// A pointer the whole wave holds, moved to scalar registers.
-template <class T>
-__device__ __forceinline__ T *wave_ptr(T *p) {
-  const uint64_t v = (uint64_t)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (T *)(((uint64_t)hi << 32) | lo);
 }
 
 // Read-only data the kernel reads at wave-uniform addresses: through the
@@ -1266,8 +1277,7 @@ __global__ void k_scatter(const BatchDev b, const PairList fin, const uint64_t *
     if (act) {
       const unsigned long long pos = out_off[e.jr] + k;
       if (pos < out_cap) {
-        out_pairs[2 * pos] = e.l;
-        out_pairs[2 * pos + 1] = e.r;
+        reinterpret_cast<uint2 *>(out_pairs)[pos] = make_uint2(e.l, e.r);  // one 8-byte store
       } else {
         *b.overflow = 1ull;
       }
