@@ -113,6 +113,16 @@ enum : int {
   kJobCounters = 5,
 };
 
+// The statistics-only counters (joined, filtered, JW) have kStatStripes
+// rows each, chosen by work-group, so that thousands of waves adding to one
+// job's count do not all wait on one address; the host sums the rows.  Rows
+// beyond the first kJobCounters hold stripes 1.. of those three counters.
+constexpr int kStatStripes = 16;
+constexpr int kJobRows = kJobCounters + 3 * (kStatStripes - 1);
+__host__ __device__ inline int job_row(int k, int stripe) {
+  return (k <= kJobJw && stripe != 0) ? kJobCounters + k * (kStatStripes - 1) + stripe - 1 : k;
+}
+
 // Per-job hash sets deduplicating the output of jobs with several rules:
 // job j owns slots [off[j], off[j] + cap[j]) (cap a power of two, 0 for jobs
 // with one rule); empty slots hold ~0.
@@ -128,7 +138,7 @@ struct BatchDev {
   uint32_t n_jr;
   uint32_t cnt_stride;
   const SimDesc *sims;
-  unsigned long long *job_cnt;   // kJobCounters * cnt_stride
+  unsigned long long *job_cnt;   // kJobRows * cnt_stride
   JobSets sets;
   unsigned long long *overflow;  // set to 1 when any list or set is full
   unsigned long long *set_overflow;  // set to 1 when a job's dedupe set is full

@@ -102,7 +102,8 @@ __device__ __forceinline__ void job_count(const BatchDev &b, int k, bool ok, uin
     const uint32_t lj = (uint32_t)__builtin_amdgcn_readlane((int)job, leader);
     const uint64_t same = __ballot(ok && job == lj);
     if ((int)lane_id() == leader)
-      atomicAdd(b.job_cnt + (size_t)k * b.cnt_stride + lj, (unsigned long long)__popcll(same));
+      atomicAdd(b.job_cnt + (size_t)job_row(k, blockIdx.x & (kStatStripes - 1)) * b.cnt_stride + lj,
+                (unsigned long long)__popcll(same));
     mask &= ~same;
   }
 }
@@ -516,7 +517,10 @@ __global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
   g.jr[gi] = jr;
   set_group_work(g, gi, l_cnt, r_cnt);
   const unsigned long long jp = (unsigned long long)l_cnt * r_cnt;
-  if (jp) atomicAdd(b.job_cnt + (size_t)kJobJoined * b.cnt_stride + d.job, jp);
+  if (jp)
+    atomicAdd(b.job_cnt + (size_t)job_row(kJobJoined, gi & (kStatStripes - 1)) * b.cnt_stride +
+                  d.job,
+              jp);
 }
 
 // Largest g in [0, n) with off[g] <= x (off non-decreasing, off[0] = 0).
@@ -1027,9 +1031,14 @@ __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, in
   }
   int m = 0;
   const uint4 *o16 = reinterpret_cast<const uint4 *>(outer);
+  // the next 16 outer bytes are loaded one block ahead, so the load overlaps
+  // the 16 steps of the current block
+  uint4 v_next = make_uint4(0, 0, 0, 0);
+  if (0 < n_out) v_next = o16[0];
   for (int base = 0; base < n_steps; base += 16) {
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (base < n_out) v = o16[base >> 4];
+    const uint4 v = v_next;
+    v_next = make_uint4(0, 0, 0, 0);
+    if (base + 16 < n_out) v_next = o16[(base >> 4) + 1];
     const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -1268,12 +1277,15 @@ __global__ void k_scatter(const BatchDev b, const PairList fin, const uint64_t *
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
   const unsigned long long n = min(*fin.count, fin.cap);
+  // A batch of one job (cnt_stride == 2) keeps the list order: no slots to
+  // hand out, so no atomics.
+  const bool one_job = b.cnt_stride == 2;
   for (uint64_t q0 = base0; q0 < n; q0 += stride) {
     const uint64_t idx = q0 + lane_id();
     const bool act = idx < n;
     PairEntry e{0u, 0u, 0u, 0u};
     if (act) e = fin.e[idx];
-    const unsigned long long k = job_slot(act, e.jr, fill);
+    const unsigned long long k = one_job ? idx : job_slot(act, e.jr, fill);
     if (act) {
       const unsigned long long pos = out_off[e.jr] + k;
       if (pos < out_cap) {
@@ -1289,7 +1301,7 @@ __global__ void k_scatter(const BatchDev b, const PairList fin, const uint64_t *
     const bool act = idx < na;
     uint32_t job = 0;
     if (act) job = au.e[idx].job;
-    const unsigned long long k = job_slot(act, job, audit_fill);
+    const unsigned long long k = one_job ? idx : job_slot(act, job, audit_fill);
     if (act) {
       const unsigned long long pos = audit_off[job] + k;
       if (pos < audit_cap)

@@ -675,7 +675,11 @@ struct Batch {
   uint32_t n_jobs() const { return (uint32_t)items.size(); }
   uint32_t stride() const { return n_jobs() + 1; }
   unsigned long long job_counter(int k, uint32_t j) const {
-    return h_cnt[cnt_job_off + (size_t)k * stride() + j];
+    unsigned long long v = h_cnt[cnt_job_off + (size_t)k * stride() + j];
+    if (k <= hb::kJobJw)  // striped statistics counter
+      for (int s = 1; s < hb::kStatStripes; s++)
+        v += h_cnt[cnt_job_off + (size_t)hb::job_row(k, s) * stride() + j];
+    return v;
   }
   ~Batch();
 };
@@ -1132,7 +1136,7 @@ hb_status enqueue_batch(Batch *b) {
   const uint32_t stride = b->stride();
   b->cnt_stage_off = kCntFixed;
   b->cnt_job_off = b->cnt_stage_off + 2 * n_stage_slots;
-  const int fill_off = b->cnt_job_off + (int)(hb::kJobCounters * stride);
+  const int fill_off = b->cnt_job_off + (int)(hb::kJobRows * stride);
   const int afill_off = fill_off + (int)stride;
   b->cnt_out_off = afill_off + (int)stride;
   b->cnt_audit_off = b->cnt_out_off + (int)stride;
