@@ -965,17 +965,15 @@ __device__ __forceinline__ void finalize(const ExactArgs &a, bool ok, uint32_t l
   job_count(a.b, kJobMatches, emit, job);
 }
 
-// Decision, output and audit of one evaluated pair (lane-local values,
-// called by every lane of the wave).
-__device__ __forceinline__ void emit_result(const ExactArgs &a, const PairEntry &e, uint32_t job,
-                                            bool last, float thr_f, int32_t slot, int m, int S,
-                                            int T, int p, bool ok) {
+// Decision and audit entry of one evaluated pair (lane-local values, called
+// by every lane of the wave); true when the pair passes the predicate.
+__device__ __forceinline__ bool decide(const ExactArgs &a, const PairEntry &e, uint32_t job,
+                                       float thr_f, int32_t slot, int m, int S, int T, int p,
+                                       bool ok) {
   const double thr = (double)thr_f;
   const float jaro = jaro_value(m, S, T);
   const double jw = winkler(jaro, p);
   const bool pass = ok && !(jw < thr);
-  finalize(a, pass && last, e.l, e.r, job);
-  list_append(pass && !last, e, a.next, a.b.overflow);
   const bool band = ok && fabs(jw - thr) < 1e-6;
   if (band) {
     const double jf = winkler_fused(jaro, p);
@@ -994,6 +992,88 @@ __device__ __forceinline__ void emit_result(const ExactArgs &a, const PairEntry 
     }
   }
   job_count(a.b, kJobAudit, band, job);
+  return pass;
+}
+
+// Decision, output and audit of one evaluated pair (lane-local values,
+// called by every lane of the wave).
+__device__ __forceinline__ void emit_result(const ExactArgs &a, const PairEntry &e, uint32_t job,
+                                            bool last, float thr_f, int32_t slot, int m, int S,
+                                            int T, int p, bool ok) {
+  const bool pass = decide(a, e, job, thr_f, slot, m, S, T, p, ok);
+  finalize(a, pass && last, e.l, e.r, job);
+  list_append(pass && !last, e, a.next, a.b.overflow);
+}
+
+// Output of one work-group pass of k_exact_lanes (256 pairs): entries for
+// the next stage and for the final list are gathered in LDS and leave with
+// one global atomic per list; the final entries' match count is added to
+// their job with one atomic when they all belong to one job.
+constexpr uint32_t kNoJob = 0xfffffffeu, kMixedJobs = 0xffffffffu;
+struct StageOut {
+  PairEntry next[256], fin[256];
+  uint32_t n_next, n_fin;
+  uint32_t fin_job;  // kNoJob, the one job of the final entries, or kMixedJobs
+  unsigned long long base_next, base_fin;
+};
+
+__device__ __forceinline__ void stage_put(bool ok, const PairEntry &e, PairEntry *buf,
+                                          uint32_t &n) {
+  const uint64_t mask = __ballot(ok);
+  if (mask == 0) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(&n, (uint32_t)__popcll(mask));
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+  if (ok) buf[base + rank_in(mask)] = e;
+}
+
+// finalize() for k_exact_lanes: into the group's final buffer.
+__device__ __forceinline__ void stage_final(const ExactArgs &a, StageOut &so, bool ok, uint32_t l,
+                                            uint32_t r, uint32_t job) {
+  const bool emit = ok && set_insert(a.b, job, l, r);
+  const uint64_t mask = __ballot(emit);
+  if (mask == 0) return;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  const uint32_t j0 = (uint32_t)__builtin_amdgcn_readlane((int)job, leader);
+  const uint32_t wave_job = __ballot(emit && job != j0) ? kMixedJobs : j0;
+  if ((int)lane_id() == leader) {
+    const uint32_t old = atomicCAS(&so.fin_job, kNoJob, wave_job);
+    if (old != kNoJob && old != wave_job) so.fin_job = kMixedJobs;
+  }
+  stage_put(emit, PairEntry{l, r, job, 0u}, so.fin, so.n_fin);
+}
+
+// Moves the group's buffers to the global lists (called by every thread).
+__device__ __forceinline__ void stage_flush(const ExactArgs &a, StageOut &so) {
+  __syncthreads();
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    so.base_next = so.n_next ? atomicAdd(a.next.count, (unsigned long long)so.n_next) : 0ull;
+    so.base_fin = so.n_fin ? atomicAdd(a.final_.count, (unsigned long long)so.n_fin) : 0ull;
+    if (so.n_fin && so.fin_job != kMixedJobs)
+      atomicAdd(a.b.job_cnt + (size_t)kJobMatches * a.b.cnt_stride + so.fin_job,
+                (unsigned long long)so.n_fin);
+  }
+  __syncthreads();
+  if (tid < so.n_next) {
+    const unsigned long long pos = so.base_next + tid;
+    if (pos < a.next.cap)
+      a.next.e[pos] = so.next[tid];
+    else
+      *a.b.overflow = 1ull;
+  }
+  if (tid < so.n_fin) {
+    const PairEntry e = so.fin[tid];
+    const unsigned long long pos = so.base_fin + tid;
+    if (pos < a.final_.cap)
+      a.final_.e[pos] = e;
+    else
+      *a.b.overflow = 1ull;
+    if (so.fin_job == kMixedJobs)  // the final entry's jr field holds its job
+      atomicAdd(a.b.job_cnt + (size_t)kJobMatches * a.b.cnt_stride + e.jr, 1ull);
+  }
+  __syncthreads();
 }
 
 // Common prefix length of two cells up to n bytes, one lane, 16 bytes at a
@@ -1175,12 +1255,20 @@ __global__ __launch_bounds__(256) void k_refine(const BatchDev b, const PairList
 // One pair per lane.  Pairs whose shorter string is longer than 256 bytes
 // are deferred to k_exact (one wave per pair, 257-slot semantics).
 __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
+  __shared__ StageOut so;
   const unsigned long long n_in = *a.in.count;
   const unsigned long long n_use = n_in < a.in.cap ? n_in : a.in.cap;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint64_t q0 = base0; q0 < n_use; q0 += stride) {
-    const uint64_t idx = q0 + lane_id();
+  const uint32_t tid = threadIdx.x;
+  // Passes of 256 pairs, the same number for every thread of the group (the
+  // output flush synchronises the group).
+  for (uint64_t blk = (uint64_t)blockIdx.x * 256; blk < n_use; blk += (uint64_t)gridDim.x * 256) {
+    if (tid == 0) {
+      so.n_next = 0;
+      so.n_fin = 0;
+      so.fin_job = kNoJob;
+    }
+    __syncthreads();
+    const uint64_t idx = blk + tid;
     bool act = idx < n_use;
     PairEntry e{0u, 0u, 0u, 0u};
     uint32_t job = 0, sim_off = 0;
@@ -1194,7 +1282,7 @@ __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
     }
     if (a.count_filtered) job_count(a.b, kJobFiltered, act, job);
     // a rule without similarity predicates is complete after the filter
-    finalize(a, act && n_sims == 0, e.l, e.r, job);
+    stage_final(a, so, act && n_sims == 0, e.l, e.r, job);
     act = act && n_sims >= a.stage;
     SimDesc sd{};
     int la = 0, lb = 0;
@@ -1224,8 +1312,12 @@ __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
     else if (!gt128) m = lane_matches<4>(planes, nwc, S, T, outer, live);
     else m = lane_matches<8>(planes, nwc, S, T, outer, live);
     if (!live) m = 0;
-    emit_result(a, e, job, n_sims == a.stage, sd.thr, sd.slot, m, S, T, p, act);
+    const bool last = n_sims == a.stage;
+    const bool pass = decide(a, e, job, sd.thr, sd.slot, m, S, T, p, act);
+    stage_final(a, so, pass && last, e.l, e.r, job);
+    stage_put(pass && !last, e, so.next, so.n_next);
     job_count(a.b, kJobJw, act, job);
+    stage_flush(a, so);
   }
 }
 
