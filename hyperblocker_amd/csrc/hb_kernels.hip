@@ -30,6 +30,8 @@
 // rounds once, in the reference's order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/hyperblocker_amd.h"
 #include "hb_internal.h"
 
@@ -551,6 +553,8 @@ __device__ __forceinline__ uint32_t find_group(const uint64_t *off, uint32_t n, 
 // leaves SAD - allow, positive iff rejected.  An empty or over-long cell has
 // rlen = +inf and is never rejected, nor is any pair when k <= 0.
 constexpr int32_t kPassAllow = 1 << 30;
+// Negated allowance of a padding A row: the accumulator stays > 0.
+constexpr uint32_t kRejectAll = 1u << 30;
 
 __device__ __forceinline__ uint32_t neg_allowance(int32_t tmax, float s, float q, float k) {
   // t * s crosses k within 1 of ceil(q), q ~ k / s: |k / s| <= 511 and q and
@@ -584,15 +588,14 @@ __device__ __forceinline__ void allowances(uint32_t la, float rla, uint32_t lb, 
 // coef[1..3], so never stricter) when only a later one of the first 4 bytes
 // differs, and nothing is rejected when the first 4 bytes are equal.  With
 // the prefixes byte-swapped (first byte most significant), x = pre_a ^ pre_b
-// is >= 2^24 exactly when the first bytes differ.
-__device__ __forceinline__ bool bound_keep(const uint32_t (&ah)[8], uint32_t apre_be,
-                                           uint32_t nal0, uint32_t nal3, const uint32_t (&bh)[8],
-                                           uint32_t bpre_be) {
-  const uint32_t x = apre_be ^ bpre_be;
+// is >= 2^24 exactly when the first bytes differ and 0 when all 4 are equal.
+// Returns the accumulator: the pair is rejected iff it is > 0 and x != 0.
+__device__ __forceinline__ int32_t bound_acc(const uint32_t (&ah)[8], uint32_t nal0, uint32_t nal3,
+                                             const uint32_t (&bh)[8], uint32_t x) {
   uint32_t acc = x >= 0x01000000u ? nal0 : nal3;
 #pragma unroll
   for (int k = 0; k < 8; k++) acc = __builtin_amdgcn_sad_u8(ah[k], bh[k], acc);
-  return (int32_t)acc <= 0 || x == 0u;
+  return (int32_t)acc;
 }
 
 // Work item = 256*AR rows of the larger side in lanes (A, AR rows per lane)
@@ -649,20 +652,26 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     const uint64_t local = item - wave_u64(g.item_off[gi]);
     const uint32_t n_bseg = (b_cnt + tile_b - 1) / tile_b;
     const uint32_t a_tile = (uint32_t)(local / n_bseg), b_seg = (uint32_t)(local % n_bseg);
+    // Rows of the A tile by wave: wave w holds rows [64*AR*w, 64*AR*(w+1)),
+    // so a partly filled last tile leaves whole waves idle, and idle waves
+    // skip the pair tests (they still take part in the group's barriers).
+    const uint32_t wave = tid >> 6;
+    const uint32_t a_wave0 = a_tile * 256 * AR + wave * 64 * AR;
+    const bool wave_active = a_wave0 < a_cnt;
     bool a_valid[AR];
     uint32_t a_idx[AR], ah[AR][8], apre[AR], la[AR];
     uint32_t nal0[AR], nal3[AR];
     float rla[AR];
 #pragma unroll
     for (int r = 0; r < AR; r++) {
-      a_idx[r] = a_tile * 256 * AR + r * 256 + tid;
+      a_idx[r] = a_wave0 + r * 64 + lane_id();
       a_valid[r] = a_idx[r] < a_cnt;
 #pragma unroll
       for (int k = 0; k < 8; k++) ah[r][k] = 0;
-      apre[r] = 0;
+      apre[r] = 0xffffffffu;
       la[r] = 0;
       rla[r] = INFINITY;
-      nal0[r] = nal3[r] = (uint32_t)(-kPassAllow);
+      nal0[r] = nal3[r] = kRejectAll;
       if (SIM && a_valid[r]) {
         const SigRow s = sig_a[a_idx[r]];
 #pragma unroll
@@ -692,7 +701,6 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
           s_rlen[tid] = bs.rlen;
         }
       }
-      if (tid == 0 && SIM) s_b[n_b] = BRow{};  // odd run end: a padding row, masked out below
       __syncthreads();
       if (SIM) {  // run starts: rows whose length differs from the previous row's
         const bool start = tid < n_b && (tid == 0 || s_b[tid - 1].len != my_len);
@@ -703,6 +711,48 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
       // read once per chunk: `to.dense` changes only inside flushes, and a
       // chunk that starts sparse flushes only at its end
       const bool dense = dense_item || to.dense;
+      // Without flushes between steps an idle wave has nothing to do here.
+      if (!dense && !wave_active) continue;
+      // One step: the lanes' A rows against B rows jj .. jj+NB-1 of the
+      // chunk.  A pair is kept when its bound accumulator is <= 0 or its
+      // first 4 bytes are equal; the step's common case (no pair kept) is
+      // told by the minima over the step, one compare each.
+      auto step = [&](uint32_t jj, auto nb_tag) {
+        constexpr int NB = decltype(nb_tag)::value;
+        int32_t acc[NB][AR];
+        uint32_t x[NB][AR];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+          const BRow b = SIM ? s_b[jj + u] : BRow{};
+#pragma unroll
+          for (int r = 0; r < AR; r++) {
+            if (SIM) {
+              x[u][r] = apre[r] ^ b.pre;
+              acc[u][r] = bound_acc(ah[r], nal0[r], nal3[r], b.h, x[u][r]);
+            } else {
+              x[u][r] = 0u;
+              acc[u][r] = 0;
+            }
+          }
+        }
+        int32_t acc_min = acc[0][0];
+        uint32_t x_min = x[0][0];
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+#pragma unroll
+          for (int r = 0; r < AR; r++) {
+            acc_min = min(acc_min, acc[u][r]);
+            x_min = min(x_min, x[u][r]);
+          }
+        if (__builtin_amdgcn_ballot_w64(acc_min <= 0 || x_min == 0u)) {  // rare: a survivor
+#pragma unroll
+          for (int u = 0; u < NB; u++)
+#pragma unroll
+            for (int r = 0; r < AR; r++)
+              tile_append(a_valid[r] && (acc[u][r] <= 0 || x[u][r] == 0u), a_idx[r] - tc.a_base,
+                          cb - b_begin + jj + u, tc, to, out, bd.overflow);
+        }
+      };
       uint32_t run = 0;
       while (run < n_b) {  // wave-uniform
         uint32_t run_end = n_b;
@@ -716,10 +766,15 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
               break;
             }
           }
-          const uint32_t blen = s_b[run].len;
-          const float rlb = s_rlen[run];
+          if (wave_active) {
+            const uint32_t blen = s_b[run].len;
+            const float rlb = s_rlen[run];
 #pragma unroll
-          for (int r = 0; r < AR; r++) allowances(la[r], rla[r], blen, rlb, k0, k3, nal0[r], nal3[r]);
+            for (int r = 0; r < AR; r++) {
+              allowances(la[r], rla[r], blen, rlb, k0, k3, nal0[r], nal3[r]);
+              if (!a_valid[r]) nal0[r] = nal3[r] = kRejectAll;
+            }
+          }
         }
         for (uint32_t jj = run; jj < run_end; jj += BU) {
           // Where survivors are dense (no similarity test, a loose first
@@ -732,31 +787,11 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
             __syncthreads();
             if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
           }
-          const bool second = jj + 1 < run_end;
-          bool ok[BU][AR];
-#pragma unroll
-          for (int u = 0; u < BU; u++) {
-            const BRow b = SIM ? s_b[jj + u] : BRow{};
-#pragma unroll
-            for (int r = 0; r < AR; r++) {
-              bool o = a_valid[r] && (u == 0 || second);
-              if (SIM) o &= bound_keep(ah[r], apre[r], nal0[r], nal3[r], b.h, b.pre);
-              ok[u][r] = o;
-            }
-          }
-          bool any = false;
-#pragma unroll
-          for (int u = 0; u < BU; u++)
-#pragma unroll
-            for (int r = 0; r < AR; r++) any |= ok[u][r];
-          if (__builtin_amdgcn_ballot_w64(any)) {  // rare: a survivor in this step
-#pragma unroll
-            for (int u = 0; u < BU; u++)
-#pragma unroll
-              for (int r = 0; r < AR; r++)
-                tile_append(ok[u][r], a_idx[r] - tc.a_base, cb - b_begin + jj + u, tc, to, out,
-                            bd.overflow);
-          }
+          if (!wave_active) continue;
+          if (BU == 1 || jj + 1 == run_end)
+            step(jj, std::integral_constant<int, 1>{});
+          else
+            step(jj, std::integral_constant<int, BU>{});
         }
         run = run_end;
       }
@@ -829,7 +864,8 @@ __global__ __launch_bounds__(256) void k_filter_small(const BatchDev bd, const G
             cur_len = blen;
             allowances(la, rla, blen, sig_b[bj].rlen, k0, k3, nal0, nal3);
           }
-          ok = ok && bound_keep(ah, apre, nal0, nal3, bh, __builtin_bswap32(sig_b[bj].pre));
+          const uint32_t x = apre ^ __builtin_bswap32(sig_b[bj].pre);
+          ok = ok && (bound_acc(ah, nal0, nal3, bh, x) <= 0 || x == 0u);
         }
         if (__builtin_amdgcn_ballot_w64(ok)) {
           const uint32_t b_row = perm_b[bj];
