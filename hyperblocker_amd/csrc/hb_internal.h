@@ -77,6 +77,11 @@ struct DevColumn {
 struct SideDesc {
   const uint32_t *perm;
   const SigRow *jsig;      // signatures of the rule's first similarity column, or null
+  // The same rows ordered by (code, first byte, length): within a join group
+  // the rows are those of perm/jsig, ordered by their first byte.  Equal to
+  // perm/jsig when the rule has no similarity predicate.
+  const uint32_t *perm_fb;
+  const SigRow *jsig_fb;
   const uint32_t *run_code;
   const uint32_t *run_off;  // n_runs + 1 entries
   uint32_t n_runs;
@@ -203,6 +208,8 @@ void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long 
 // Join order of a table for one rule shape: keys (code << 8) | min(length,
 // 255) (code or length may be absent: null pointer), sorted with the row ids;
 // then the runs of equal code (key >> 8): flags, exclusive scan, run arrays.
+void launch_layout_keys_fb(const uint32_t *code, const SigRow *sig, const uint16_t *len,
+                           uint32_t n, uint64_t *keys, hipStream_t stream);
 void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,
                         hipStream_t stream);
 void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,

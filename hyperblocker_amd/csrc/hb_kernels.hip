@@ -411,6 +411,19 @@ __global__ void k_layout_keys(const uint32_t *__restrict__ code, const uint16_t 
   keys[i] = (c << 8) | l;
 }
 
+// Keys of the first-byte order: (code << 16) | (first byte << 8) |
+// min(length, 255).  Sorted, a join group's rows come ordered by first byte.
+__global__ void k_layout_keys_fb(const uint32_t *__restrict__ code, const SigRow *__restrict__ sig,
+                                 const uint16_t *__restrict__ len, uint32_t n,
+                                 uint64_t *__restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t c = code ? (uint64_t)code[i] : 0ull;
+  const uint32_t f = sig[i].pre & 0xffu;
+  const uint32_t l = min((uint32_t)len[i], 255u);
+  keys[i] = (c << 16) | (f << 8) | l;
+}
+
 __global__ void k_gather_sig(const uint32_t *__restrict__ perm, uint32_t n,
                              const SigRow *__restrict__ sig, SigRow *__restrict__ jsig) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -584,6 +597,15 @@ __device__ __forceinline__ void allowances(uint32_t la, float rla, uint32_t lb, 
   nal3 = neg_allowance(tmax, s, k3 * inv, k3);
 }
 
+// The accumulator chain: seed + sum of |a_i - b_i| over the 32 counters.
+__device__ __forceinline__ int32_t sad_chain(const uint32_t (&ah)[8], const uint32_t (&bh)[8],
+                                             uint32_t seed) {
+  uint32_t acc = seed;
+#pragma unroll
+  for (int k = 0; k < 8; k++) acc = __builtin_amdgcn_sad_u8(ah[k], bh[k], acc);
+  return (int32_t)acc;
+}
+
 // coef[0] applies when the first bytes differ, coef[3] (the smallest of
 // coef[1..3], so never stricter) when only a later one of the first 4 bytes
 // differs, and nothing is rejected when the first 4 bytes are equal.  With
@@ -621,6 +643,11 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
   __shared__ BRow s_b[SIM ? kChunk + 1 : 1];
   __shared__ unsigned long long s_starts[kChunk / 64];  // bit i: B row i starts a run
   __shared__ float s_rlen[SIM ? kChunk + 1 : 1];
+  // First-byte range of each slice (64 A rows: wave w, row r of its lanes),
+  // and per slice a bit per B row of the chunk whose first byte lies in it.
+  constexpr int kSlices = SIM ? 4 * AR : 1;
+  __shared__ uint32_t s_frange[kSlices];
+  __shared__ unsigned long long s_slow[kSlices][kChunk / 64];
   __shared__ TileOut to;
   const uint32_t tid = threadIdx.x;
   if (tid == 0) {
@@ -645,9 +672,10 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     const uint32_t a_cnt = a_is_l ? l_cnt : r_cnt, b_cnt = a_is_l ? r_cnt : l_cnt;
     const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
     const SideDesc &sa = d.s[a_is_l ? 0 : 1], &sb = d.s[a_is_l ? 1 : 0];
-    const uint32_t *perm_a = wave_ptr(sa.perm);
+    // A rows in first-byte order, B rows in length order (runs of equal length)
+    const uint32_t *perm_a = wave_ptr(sa.perm_fb);
     const uint32_t *perm_b = wave_ptr(sb.perm);
-    const SigRow *sig_a = wave_ptr(sa.jsig) + a_off;
+    const SigRow *sig_a = wave_ptr(sa.jsig_fb) + a_off;
     const SigRow *sig_b = wave_ptr(sb.jsig) + b_off;
     const uint64_t local = item - wave_u64(g.item_off[gi]);
     const uint32_t n_bseg = (b_cnt + tile_b - 1) / tile_b;
@@ -681,6 +709,21 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
         rla[r] = s.rlen;
       }
     }
+    if (SIM) {
+      // The slice's valid rows are consecutive in first-byte order, so their
+      // first bytes span [first lane's, last valid lane's]; an empty slice
+      // gets an empty range.  (The previous item's last reads of s_frange
+      // precede the barrier that ended it.)
+#pragma unroll
+      for (int r = 0; r < AR; r++) {
+        const uint32_t base = a_wave0 + r * 64;
+        const uint32_t nv = base < a_cnt ? min(64u, a_cnt - base) : 0u;
+        const uint32_t fb = apre[r] >> 24;
+        const uint32_t lo = nv ? (uint32_t)__builtin_amdgcn_readlane((int)fb, 0) : 256u;
+        const uint32_t hi = nv ? (uint32_t)__builtin_amdgcn_readlane((int)fb, (int)nv - 1) : 0u;
+        if (lane_id() == 0) s_frange[wave * AR + r] = lo | (hi << 16);
+      }
+    }
     const uint32_t b_begin = b_seg * tile_b;
     const uint32_t b_end = min(b_begin + tile_b, b_cnt);
     const TileCtx tc{perm_a + a_off, perm_b + b_off + b_begin, a_tile * 256 * AR, jr, a_is_l};
@@ -688,7 +731,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
       const uint32_t n_b = min(kChunk, b_end - cb);
       __syncthreads();
       if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);  // to.n is group-uniform here
-      uint32_t my_len = 0;
+      uint32_t my_len = 0, my_fb = 0;
       if (tid < n_b) {
         if (SIM) {
           const SigRow bs = sig_b[cb + tid];
@@ -697,6 +740,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
           for (int k = 0; k < 8; k++) br.h[k] = bs.h[k];
           br.pre = __builtin_bswap32(bs.pre);
           br.len = my_len = bs.len;
+          my_fb = bs.pre & 0xffu;
           s_b[tid] = br;
           s_rlen[tid] = bs.rlen;
         }
@@ -706,6 +750,13 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
         const bool start = tid < n_b && (tid == 0 || s_b[tid - 1].len != my_len);
         const unsigned long long m = __ballot(start);
         if (lane_id() == 0) s_starts[tid >> 6] = m;
+#pragma unroll
+        for (int sl = 0; sl < kSlices; sl++) {
+          const uint32_t fr = s_frange[sl];
+          const bool slow = tid < n_b && my_fb >= (fr & 0xffffu) && my_fb <= (fr >> 16);
+          const unsigned long long ms = __ballot(slow);
+          if (lane_id() == 0) s_slow[sl][tid >> 6] = ms;
+        }
         __syncthreads();
       }
       // read once per chunk: `to.dense` changes only inside flushes, and a
@@ -714,42 +765,54 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
       // Without flushes between steps an idle wave has nothing to do here.
       if (!dense && !wave_active) continue;
       // One step: the lanes' A rows against B rows jj .. jj+NB-1 of the
-      // chunk.  A pair is kept when its bound accumulator is <= 0 or its
-      // first 4 bytes are equal; the step's common case (no pair kept) is
-      // told by the minima over the step, one compare each.
-      auto step = [&](uint32_t jj, auto nb_tag) {
+      // chunk.  A pair is kept when its accumulator is <= 0.  Where a B
+      // row's first byte lies outside the first-byte range of a slice (bit
+      // clear in `slow`), every pair of the two differs in the first byte:
+      // the accumulator starts from nal0 with no per-pair class test.
+      // Otherwise the class is chosen per pair and a pair whose first 4
+      // bytes are equal is kept.  The step's common case (no pair kept) is
+      // told by the minimum over the step.
+      auto step = [&](uint32_t jj, auto nb_tag, const uint64_t (&slow_w)[AR]) {
         constexpr int NB = decltype(nb_tag)::value;
         int32_t acc[NB][AR];
-        uint32_t x[NB][AR];
+        uint64_t slow_any = 0;
 #pragma unroll
-        for (int u = 0; u < NB; u++) {
-          const BRow b = SIM ? s_b[jj + u] : BRow{};
+        for (int r = 0; r < AR; r++) slow_any |= (slow_w[r] >> (jj & 63u)) & ((1ull << NB) - 1);
+        if (!SIM) {
 #pragma unroll
-          for (int r = 0; r < AR; r++) {
-            if (SIM) {
-              x[u][r] = apre[r] ^ b.pre;
-              acc[u][r] = bound_acc(ah[r], nal0[r], nal3[r], b.h, x[u][r]);
-            } else {
-              x[u][r] = 0u;
-              acc[u][r] = 0;
+          for (int u = 0; u < NB; u++)
+#pragma unroll
+            for (int r = 0; r < AR; r++) acc[u][r] = 0;
+        } else if (slow_any == 0) {
+#pragma unroll
+          for (int u = 0; u < NB; u++) {
+            const BRow b = s_b[jj + u];
+#pragma unroll
+            for (int r = 0; r < AR; r++) acc[u][r] = sad_chain(ah[r], b.h, nal0[r]);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < NB; u++) {
+            const BRow b = s_b[jj + u];
+#pragma unroll
+            for (int r = 0; r < AR; r++) {
+              const uint32_t x = apre[r] ^ b.pre;
+              const int32_t a = sad_chain(ah[r], b.h, x >= 0x01000000u ? nal0[r] : nal3[r]);
+              acc[u][r] = x == 0u ? 0 : a;
             }
           }
         }
         int32_t acc_min = acc[0][0];
-        uint32_t x_min = x[0][0];
 #pragma unroll
         for (int u = 0; u < NB; u++)
 #pragma unroll
-          for (int r = 0; r < AR; r++) {
-            acc_min = min(acc_min, acc[u][r]);
-            x_min = min(x_min, x[u][r]);
-          }
-        if (__builtin_amdgcn_ballot_w64(acc_min <= 0 || x_min == 0u)) {  // rare: a survivor
+          for (int r = 0; r < AR; r++) acc_min = min(acc_min, acc[u][r]);
+        if (__builtin_amdgcn_ballot_w64(acc_min <= 0)) {  // rare: a survivor
 #pragma unroll
           for (int u = 0; u < NB; u++)
 #pragma unroll
             for (int r = 0; r < AR; r++)
-              tile_append(a_valid[r] && (acc[u][r] <= 0 || x[u][r] == 0u), a_idx[r] - tc.a_base,
+              tile_append(a_valid[r] && acc[u][r] <= 0, a_idx[r] - tc.a_base,
                           cb - b_begin + jj + u, tc, to, out, bd.overflow);
         }
       };
@@ -776,22 +839,30 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
             }
           }
         }
-        for (uint32_t jj = run; jj < run_end; jj += BU) {
-          // Where survivors are dense (no similarity test, a loose first
-          // threshold, or a large last flush) the buffer is also flushed
-          // between steps: a step adds at most 256 * AR * BU <= kTileOut / 2
-          // entries, so it never overflows.  Every wave of the group runs
-          // the same (run, jj) sequence and reads the same `dense` (written
-          // only between barriers), so all reach the barrier together.
-          if (dense && jj != run) {
-            __syncthreads();
-            if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
+        // The run in segments within one 64-row word of the slow masks.
+        for (uint32_t jj = run; jj < run_end;) {
+          const uint32_t seg_end = min(run_end, (jj & ~63u) + 64u);
+          uint64_t slow_w[AR];
+#pragma unroll
+          for (int r = 0; r < AR; r++)
+            slow_w[r] = SIM ? wave_u64(s_slow[SIM ? wave * AR + r : 0][jj >> 6]) : 0ull;
+          for (; jj < seg_end; jj += (BU == 1 || jj + 1 == seg_end) ? 1u : (uint32_t)BU) {
+            // Where survivors are dense (no similarity test, a loose first
+            // threshold, or a large last flush) the buffer is also flushed
+            // between steps: a step adds at most 256 * AR * BU <= kTileOut / 2
+            // entries, so it never overflows.  Every wave of the group runs
+            // the same (run, jj) sequence and reads the same `dense` (written
+            // only between barriers), so all reach the barrier together.
+            if (dense && jj != run) {
+              __syncthreads();
+              if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
+            }
+            if (!wave_active) continue;
+            if (BU == 1 || jj + 1 == seg_end)
+              step(jj, std::integral_constant<int, 1>{}, slow_w);
+            else
+              step(jj, std::integral_constant<int, BU>{}, slow_w);
           }
-          if (!wave_active) continue;
-          if (BU == 1 || jj + 1 == run_end)
-            step(jj, std::integral_constant<int, 1>{});
-          else
-            step(jj, std::integral_constant<int, BU>{});
         }
         run = run_end;
       }
@@ -1502,6 +1573,13 @@ void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, u
   if (!n) return;
   hipLaunchKernelGGL(k_layout_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, code, len, n,
                      keys);
+}
+
+void launch_layout_keys_fb(const uint32_t *code, const SigRow *sig, const uint16_t *len,
+                           uint32_t n, uint64_t *keys, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_layout_keys_fb, dim3(grid_for(n, 256)), dim3(256), 0, stream, code, sig,
+                     len, n, keys);
 }
 
 void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,

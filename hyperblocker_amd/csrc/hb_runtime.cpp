@@ -478,6 +478,8 @@ struct Layout {
   uint64_t *d_keys = nullptr;     // sorted (code << 8) | min(length, 255)
   uint32_t *d_perm = nullptr;     // row id of each join-order position
   hb::SigRow *d_jsig = nullptr;   // similarity signatures in join order
+  uint32_t *d_perm_fb = nullptr;   // rows by (code, first byte, length), with a similarity column
+  hb::SigRow *d_jsig_fb = nullptr;
   uint32_t *d_run_code = nullptr;
   uint32_t *d_run_off = nullptr;  // n_runs + 1
   uint32_t n_runs = 0;
@@ -931,7 +933,11 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_perm));
   HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_run_code));
   HB_TRY(t->dev_alloc((n1 + 1) * sizeof(uint32_t), (void **)&ly.d_run_off));
-  if (sim >= 0) HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig));
+  if (sim >= 0) {
+    HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig));
+    HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_perm_fb));
+    HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig_fb));
+  }
   if (n) {
     // The run count (distinct keys) is counted on the host from the codes it
     // holds, so the build is asynchronous: batches wait on its event, and its
@@ -956,19 +962,34 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     hb::launch_iota(iota, n, stream);
     int end_bit = 8;
     while (end_bit < 64 && (1ull << (end_bit - 8)) < n_codes) end_bit++;
-    size_t tb = 0, ts = 0;
+    const int end_bit_fb = std::min(64, end_bit + 8);
+    size_t tb = 0, ts = 0, tf = 0;
     HIP_OK(hb::sort_pairs_u64(nullptr, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit,
                               stream));
     HIP_OK(hb::exscan_u32(nullptr, ts, flags, run_id, n, stream));
+    uint64_t *keys_fb = nullptr;
+    if (sim >= 0) {
+      HIP_OK(hb::sort_pairs_u64(nullptr, tf, keys_in, keys_in, iota, ly.d_perm_fb, n, end_bit_fb,
+                                stream));
+      HB_TRY(temp(n1 * sizeof(uint64_t), (void **)&keys_fb));
+    }
     void *tmp = nullptr;
-    const size_t tmp_bytes = std::max<size_t>(std::max(tb, ts), 16);
+    const size_t tmp_bytes = std::max<size_t>(std::max(std::max(tb, ts), tf), 16);
     HB_TRY(temp(tmp_bytes, &tmp));
     HIP_OK(hb::sort_pairs_u64(tmp, tb, keys_in, ly.d_keys, iota, ly.d_perm, n, end_bit, stream));
     hb::launch_run_flags(ly.d_keys, n, flags, stream);
     HIP_OK(hb::exscan_u32(tmp, ts, flags, run_id, n, stream));
     hb::launch_layout_runs(ly.d_keys, flags, run_id, n, ly.d_run_code, ly.d_run_off, d_nruns,
                            stream);
-    if (sim >= 0) hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
+    if (sim >= 0) {
+      hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
+      // the first-byte order: keys_in is free again once the first sort has run
+      hb::launch_layout_keys_fb(key_codes, t->cols[sim].d_sig, t->cols[sim].d_len, n, keys_in,
+                                stream);
+      HIP_OK(hb::sort_pairs_u64(tmp, tf, keys_in, keys_fb, iota, ly.d_perm_fb, n, end_bit_fb,
+                                stream));
+      hb::launch_gather_sig(ly.d_perm_fb, n, t->cols[sim].d_sig, ly.d_jsig_fb, stream);
+    }
     HIP_OK(hipGetLastError());
     hipEvent_t e = nullptr;
     HB_TRY(t->ctx->events.get(&e));
@@ -1089,6 +1110,8 @@ hb_status enqueue_batch(Batch *b) {
       hb::SideDesc &sd = d.s[side];
       sd.perm = ly->d_perm;
       sd.jsig = ly->d_jsig;
+      sd.perm_fb = ly->d_perm_fb ? ly->d_perm_fb : ly->d_perm;
+      sd.jsig_fb = ly->d_jsig_fb ? ly->d_jsig_fb : ly->d_jsig;
       sd.run_code = ly->d_run_code;
       sd.run_off = ly->d_run_off;
       sd.n_runs = ly->n_runs;
