@@ -1203,9 +1203,16 @@ __device__ __forceinline__ int lane_prefix(const uint8_t *a, const uint8_t *b, i
 // strings of at most 32*NW bytes.  The inner string is held as 8 bit-planes
 // (plane k, word w: bit b set iff bit k of inner byte 32w+b is set), so the
 // positions equal to an outer byte c are the AND over k of
-// (plane k XNOR bit k of c).  `win` is the window [i-h, i+h] of outer
-// position i, advanced by a one-bit left shift per step; `avail` holds inner
-// positions not yet matched.  The lowest candidate is taken with x & -x.
+// (plane k XNOR bit k of c), one 3-input bit operation per plane.  `win` is
+// the window [i-h, i+h] of outer position i, advanced by a one-bit left
+// shift per step; `avail` holds inner positions not yet matched.  The lowest
+// candidate is x & -x over the NW-word concatenation (one borrow chain), and
+// the match count is S minus the positions left in `avail` at the end (each
+// match takes exactly one).  No per-step liveness test is needed: cells are
+// NUL padded and hold no NUL within their length, outer blocks past n_out
+// read as zero, and from i >= S + h on the window lies past the inner
+// string.  With HIGH false no inner byte of the wave has bit 7 set, so an
+// outer byte with bit 7 set matches nothing (its bit-7 mask clears `e`).
 template <int NW, bool HIGH>
 __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, int h, int n_out,
                                            const uint8_t *outer, int n_steps) {
@@ -1216,7 +1223,6 @@ __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, in
     avail[w] = S >= lo + 32 ? ~0u : (S > lo ? ((1u << (S - lo)) - 1u) : 0u);
     win[w] = h >= lo + 31 ? ~0u : (h >= lo ? (~0u >> (31 - (h - lo))) : 0u);
   }
-  int m = 0;
   const uint4 *o16 = reinterpret_cast<const uint4 *>(outer);
   // the next 16 outer bytes are loaded one block ahead, so the load overlaps
   // the 16 steps of the current block
@@ -1230,30 +1236,27 @@ __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, in
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int i = base + q;
-      const uint32_t c = (vw[q >> 2] >> (8 * (q & 3))) & 0xffu;
-      // -1 where this lane still has outer positions, else 0
-      const uint32_t live = (uint32_t)(-(int)(i < n_out));
+      const uint32_t c = __builtin_amdgcn_ubfe(vw[q >> 2], 8 * (q & 3), 8);
       uint32_t sb[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) sb[k] = (uint32_t)__builtin_amdgcn_sbfe((int)c, k, 1);
-      const uint32_t ascii = HIGH ? ~0u : (uint32_t)((int)(c >> 7) - 1);
       uint32_t cand[NW];
 #pragma unroll
       for (int w = 0; w < NW; w++) {
-        uint32_t e = avail[w] & win[w] & live & ascii;
+        // a & b & ~c (0x40) and a & (b XNOR c) (0x90), bit-wise
+        uint32_t e = HIGH ? (avail[w] & win[w]) : __builtin_amdgcn_bitop3_b32(avail[w], win[w], sb[7], 0x40);
 #pragma unroll
-        for (int k = 0; k < (HIGH ? 8 : 7); k++) e &= ~(P[w][k] ^ sb[k]);
+        for (int k = 0; k < (HIGH ? 8 : 7); k++) e = __builtin_amdgcn_bitop3_b32(e, P[w][k], sb[k], 0x90);
         cand[w] = e;
       }
-      // lowest set bit of the NW-word candidate mask
-      uint32_t found = 0;
+      unsigned borrow = 0;
 #pragma unroll
       for (int w = 0; w < NW; w++) {
-        const uint32_t low = found ? 0u : (cand[w] & (0u - cand[w]));
-        avail[w] ^= low;
-        found |= low;
+        unsigned bo;
+        const uint32_t neg = __builtin_subc(0u, cand[w], borrow, &bo);
+        borrow = bo;
+        avail[w] ^= cand[w] & neg;
       }
-      m += found != 0;
       // advance the window: positions shift up by one, and position 0 stays
       // inside while i + 1 <= h
 #pragma unroll
@@ -1261,7 +1264,10 @@ __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, in
       win[0] = (win[0] << 1) | (uint32_t)(i + 1 <= h);
     }
   }
-  return m;
+  int left = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) left += __builtin_popcount(avail[w]);
+  return S - left;
 }
 
 // Words w < nwc of the row's planes; words beyond the column's stored width
