@@ -1183,20 +1183,17 @@ __device__ __forceinline__ void stage_flush(const ExactArgs &a, StageOut &so) {
   __syncthreads();
 }
 
-// Common prefix length of two cells up to n bytes, one lane, 16 bytes at a
-// time (cells are 16-byte aligned and NUL padded to a multiple of 16).
+// Common prefix of a and b, capped at n and at 16 bytes (the cells' first
+// 16 bytes, which every cell has).  The cap changes no result: from p = 11
+// on, j + (1 - j) * p * 0.1 exceeds 1 for every Jaro value j < 1 (by
+// 0.1 * (1 - j) >= 0.1 * 2^-24, far above the rounding) and equals 1 at
+// j = 1, so Winkler's clamp yields exactly 1.0 either way.
 __device__ __forceinline__ int lane_prefix(const uint8_t *a, const uint8_t *b, int n) {
-  const uint4 *pa = reinterpret_cast<const uint4 *>(a);
-  const uint4 *pb = reinterpret_cast<const uint4 *>(b);
-  for (int base = 0; base < n; base += 16) {
-    const uint4 x = pa[base >> 4], y = pb[base >> 4];
-    const uint32_t d[4] = {x.x ^ y.x, x.y ^ y.y, x.z ^ y.z, x.w ^ y.w};
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      if (d[q]) return min(n, base + 4 * q + ((__ffs(d[q]) - 1) >> 3));
-    }
-  }
-  return n;
+  const uint4 x = *reinterpret_cast<const uint4 *>(a), y = *reinterpret_cast<const uint4 *>(b);
+  const uint64_t lo = ((uint64_t)(x.y ^ y.y) << 32) | (uint64_t)(x.x ^ y.x);
+  const uint64_t hi = ((uint64_t)(x.w ^ y.w) << 32) | (uint64_t)(x.z ^ y.z);
+  const int p = lo ? (__builtin_ctzll(lo) >> 3) : (hi ? 8 + (__builtin_ctzll(hi) >> 3) : 16);
+  return min(p, n);
 }
 
 // Greedy match count (device_func.cuh:81-111) for one pair per lane, inner
@@ -1367,8 +1364,22 @@ __global__ __launch_bounds__(256) void k_refine(const BatchDev b, const PairList
 
 // One pair per lane.  Pairs whose shorter string is longer than 256 bytes
 // are deferred to k_exact (one wave per pair, 257-slot semantics).
-__global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
+// A pair of k_exact_lanes between its load and its evaluation.
+struct ExactSlot {
+  PairEntry e;
+  uint32_t job, sim_off, la, lb;
+  int32_t n_sims;
+  uint32_t act;
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_exact_lanes(ExactArgs a) {
   __shared__ StageOut so;
+  // The pass's pairs are regrouped by n_out (the greedy loop's trip count)
+  // in 16 classes of 16 steps, so the waves of the group hold pairs of
+  // similar length: a wave runs to its longest pair.
+  constexpr int kClasses = 16;
+  __shared__ uint32_t s_ccnt[kClasses];
+  __shared__ ExactSlot s_slot[256];
   const unsigned long long n_in = *a.in.count;
   const unsigned long long n_use = n_in < a.in.cap ? n_in : a.in.cap;
   const uint32_t tid = threadIdx.x;
@@ -1380,34 +1391,51 @@ __global__ __launch_bounds__(256) void k_exact_lanes(ExactArgs a) {
       so.n_fin = 0;
       so.fin_job = kNoJob;
     }
+    if (tid < kClasses) s_ccnt[tid] = 0;
     __syncthreads();
-    const uint64_t idx = blk + tid;
-    bool act = idx < n_use;
-    PairEntry e{0u, 0u, 0u, 0u};
-    uint32_t job = 0, sim_off = 0;
-    int32_t n_sims = 0;
-    if (act) {
-      e = a.in.e[idx];
-      const JRDesc &d = a.b.jrs[e.jr];
-      job = d.job;
-      n_sims = d.n_sims;
-      sim_off = d.sim_off;
+    uint32_t cls = 0, rank = 0;
+    {
+      const uint64_t idx = blk + tid;
+      bool act = idx < n_use;
+      ExactSlot sl{};
+      if (act) {
+        sl.e = a.in.e[idx];
+        const JRDesc &d = a.b.jrs[sl.e.jr];
+        sl.job = d.job;
+        sl.n_sims = d.n_sims;
+        sl.sim_off = d.sim_off;
+      }
+      if (a.count_filtered) job_count(a.b, kJobFiltered, act, sl.job);
+      // a rule without similarity predicates is complete after the filter
+      stage_final(a, so, act && sl.n_sims == 0, sl.e.l, sl.e.r, sl.job);
+      act = act && sl.n_sims >= a.stage;
+      if (act) {
+        const SimDesc &sd = a.b.sims[sl.sim_off + a.stage - 1];
+        sl.la = sd.col_l.len[sl.e.l];
+        sl.lb = sd.col_r.len[sl.e.r];
+      }
+      const int S = (int)min(sl.la, sl.lb), T = (int)max(sl.la, sl.lb);
+      const bool deferred = act && S > 256;
+      list_append(deferred, sl.e, a.defer, a.b.overflow);
+      act = act && !deferred;
+      sl.act = act ? 1u : 0u;
+      if (act) cls = (uint32_t)min(min(T, S + ((S + 1) >> 1)) >> 4, kClasses - 1);
+      rank = atomicAdd(&s_ccnt[cls], 1u);
+      __syncthreads();
+      uint32_t off = 0;  // exclusive prefix over the classes below cls
+      for (uint32_t c = 0; c < cls; c++) off += s_ccnt[c];
+      s_slot[off + rank] = sl;
+      __syncthreads();
     }
-    if (a.count_filtered) job_count(a.b, kJobFiltered, act, job);
-    // a rule without similarity predicates is complete after the filter
-    stage_final(a, so, act && n_sims == 0, e.l, e.r, job);
-    act = act && n_sims >= a.stage;
+    const ExactSlot sl = s_slot[tid];
+    const PairEntry e = sl.e;
+    const uint32_t job = sl.job;
+    const int32_t n_sims = sl.n_sims;
+    const bool act = sl.act != 0;
     SimDesc sd{};
-    int la = 0, lb = 0;
-    if (act) {
-      sd = a.b.sims[sim_off + a.stage - 1];
-      la = sd.col_l.len[e.l];
-      lb = sd.col_r.len[e.r];
-    }
+    if (act) sd = a.b.sims[sl.sim_off + a.stage - 1];
+    const int la = (int)sl.la, lb = (int)sl.lb;
     const int S = min(la, lb), T = max(la, lb);
-    const bool deferred = act && S > 256;
-    list_append(deferred, e, a.defer, a.b.overflow);
-    act = act && !deferred;
     const bool swap = la > lb;  // device_func.cuh:71-79: left stays inner on ties
     const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
     const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
