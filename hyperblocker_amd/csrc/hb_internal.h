@@ -242,8 +242,12 @@ void launch_refine(BatchDev b, PairList in, PairList out, int n_cu, hipStream_t 
 // needed), the others to `next`.  Pairs with a shorter string above 256
 // bytes run one wave per pair (second kernel, via `defer`; not launched
 // when `may_defer` is false, i.e. no column pair can hold such a pair).
+// With `direct_pairs` set (a batch of one job) the final pairs go there as
+// (left, right) at their final-list positions instead, and audit entries go
+// to `direct_audit`: the job's output needs no scatter.
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
-                  PairList defer, AuditList audit, bool count_filtered, bool may_defer, int n_cu,
+                  PairList defer, AuditList audit, bool count_filtered, bool may_defer,
+                  uint2 *direct_pairs, hb_audit_entry *direct_audit, int n_cu,
                   hipStream_t stream);
 
 // Copies the final list and the audit list into per-job contiguous ranges:

@@ -1028,6 +1028,10 @@ struct ExactArgs {
   PairList final_;  // pairs that passed their JR's last predicate (tagged with the job)
   PairList defer;   // pairs whose shorter string exceeds 256 bytes
   AuditList audit;
+  // A batch of one job: final pairs and audit entries are written here, at
+  // the positions they take in final_ and audit (the same capacities).
+  uint2 *direct_pairs;
+  hb_audit_entry *direct_audit;
 };
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
@@ -1068,7 +1072,25 @@ __device__ bool set_insert(const BatchDev &b, uint32_t job, uint32_t l, uint32_t
 __device__ __forceinline__ void finalize(const ExactArgs &a, bool ok, uint32_t l, uint32_t r,
                                          uint32_t job) {
   const bool emit = ok && set_insert(a.b, job, l, r);
-  list_append(emit, PairEntry{l, r, job, 0u}, a.final_, a.b.overflow);
+  if (!a.direct_pairs) {
+    list_append(emit, PairEntry{l, r, job, 0u}, a.final_, a.b.overflow);
+  } else {
+    const uint64_t mask = __ballot(emit);
+    if (mask) {
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      unsigned long long base = 0;
+      if ((int)lane_id() == leader)
+        base = atomicAdd(a.final_.count, (unsigned long long)__popcll(mask));
+      base = readlane64(base, leader);
+      if (emit) {
+        const unsigned long long pos = base + rank_in(mask);
+        if (pos < a.final_.cap)
+          a.direct_pairs[pos] = make_uint2(l, r);
+        else
+          *a.b.overflow = 1ull;
+      }
+    }
+  }
   job_count(a.b, kJobMatches, emit, job);
 }
 
@@ -1086,14 +1108,20 @@ __device__ __forceinline__ bool decide(const ExactArgs &a, const PairEntry &e, u
     const double jf = winkler_fused(jaro, p);
     const unsigned long long pos = atomicAdd(a.audit.count, 1ull);
     if (pos < a.audit.cap) {
-      AuditRec &rec = a.audit.e[pos];
-      rec.e.l = e.l;
-      rec.e.r = e.r;
-      rec.e.slot = slot;
-      rec.e.flags = (((jw < thr) != (jf < thr)) ? 1 : 0) | (S >= 257 ? 2 : 0);
-      rec.e.jw = jw;
-      rec.job = job;
-      rec.pad = 0;
+      hb_audit_entry ae;
+      ae.l = e.l;
+      ae.r = e.r;
+      ae.slot = slot;
+      ae.flags = (((jw < thr) != (jf < thr)) ? 1 : 0) | (S >= 257 ? 2 : 0);
+      ae.jw = jw;
+      if (a.direct_audit) {
+        a.direct_audit[pos] = ae;
+      } else {
+        AuditRec &rec = a.audit.e[pos];
+        rec.e = ae;
+        rec.job = job;
+        rec.pad = 0;
+      }
     } else {
       *a.b.overflow = 1ull;
     }
@@ -1173,10 +1201,12 @@ __device__ __forceinline__ void stage_flush(const ExactArgs &a, StageOut &so) {
   if (tid < so.n_fin) {
     const PairEntry e = so.fin[tid];
     const unsigned long long pos = so.base_fin + tid;
-    if (pos < a.final_.cap)
-      a.final_.e[pos] = e;
-    else
+    if (pos >= a.final_.cap)
       *a.b.overflow = 1ull;
+    else if (a.direct_pairs)
+      a.direct_pairs[pos] = make_uint2(e.l, e.r);
+    else
+      a.final_.e[pos] = e;
     if (so.fin_job == kMixedJobs)  // the final entry's jr field holds its job
       atomicAdd(a.b.job_cnt + (size_t)kJobMatches * a.b.cnt_stride + e.jr, 1ull);
   }
@@ -1673,13 +1703,15 @@ void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, 
 }
 
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
-                  PairList defer, AuditList audit, bool count_filtered, bool may_defer, int n_cu,
+                  PairList defer, AuditList audit, bool count_filtered, bool may_defer,
+                  uint2 *direct_pairs, hb_audit_entry *direct_audit, int n_cu,
                   hipStream_t stream) {
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
-  ExactArgs a{b, in, stage, count_filtered ? 1 : 0, next, final_, defer, audit};
+  ExactArgs a{b,      in,    stage,        count_filtered ? 1 : 0, next, final_, defer, audit,
+              direct_pairs, direct_audit};
   hipLaunchKernelGGL(k_exact_lanes, dim3(cus * 8), dim3(256), 0, stream, a);
   if (!may_defer) return;
-  ExactArgs d{b, defer, stage, 0, next, final_, defer, audit};
+  ExactArgs d{b, defer, stage, 0, next, final_, defer, audit, direct_pairs, direct_audit};
   d.defer.cap = 0;  // deferred pairs are not deferred again
   hipLaunchKernelGGL(k_exact, dim3(cus * 4), dim3(256), 0, stream, d);
 }

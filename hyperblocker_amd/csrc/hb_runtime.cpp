@@ -1249,11 +1249,18 @@ hb_status enqueue_batch(Batch *b) {
   }
   HB_TRY(batch_alloc(b, b->defer_cap * sizeof(hb::PairEntry), &p));
   defer_buf = (hb::PairEntry *)p;
-  HB_TRY(batch_alloc(b, b->final_cap * sizeof(hb::PairEntry), &p));
-  final_buf = (hb::PairEntry *)p;
-  HB_TRY(batch_alloc(b, b->audit_cap * sizeof(hb::AuditRec), &p));
-  hb::AuditRec *audit_buf = (hb::AuditRec *)p;
-  // outputs in pinned memory, written by the scatter kernel
+  // A batch of one job writes its pairs and audit entries straight into the
+  // pinned outputs; others gather them per job with the scatter kernel.
+  const bool direct = n_jobs == 1;
+  final_buf = nullptr;
+  hb::AuditRec *audit_buf = nullptr;
+  if (!direct) {
+    HB_TRY(batch_alloc(b, b->final_cap * sizeof(hb::PairEntry), &p));
+    final_buf = (hb::PairEntry *)p;
+    HB_TRY(batch_alloc(b, b->audit_cap * sizeof(hb::AuditRec), &p));
+    audit_buf = (hb::AuditRec *)p;
+  }
+  // outputs in pinned memory
   void *dout = nullptr, *daud = nullptr;
   HB_TRY(batch_alloc_host(b, b->final_cap * 8, &hp, &dout));
   b->h_out = (uint32_t *)hp;
@@ -1299,14 +1306,15 @@ hb_status enqueue_batch(Batch *b) {
     const hb::PairList next{lists[(q + 1) & 1], list_cnt + q + 1, b->list_cap};
     const hb::PairList defer{defer_buf, defer_cnt + s, b->defer_cap};
     hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, may_defer,
-                     ctx->n_cu, st);
+                     direct ? (uint2 *)dout : nullptr,
+                     direct ? (hb_audit_entry *)daud : nullptr, ctx->n_cu, st);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b->marks[2], st));
-  // per-job output ranges
+  // per-job output ranges (a batch of one job: both offsets stay 0)
   uint64_t *out_off = (uint64_t *)(d_cnt + b->cnt_out_off);
   uint64_t *audit_off = (uint64_t *)(d_cnt + b->cnt_audit_off);
-  {
+  if (!direct) {
     size_t t = tmp_bytes;
     HIP_OK(hb::exscan_u64(tmp, t, (const uint64_t *)(bd.job_cnt + (size_t)hb::kJobMatches * stride),
                           out_off, stride, st));
@@ -1314,9 +1322,10 @@ hb_status enqueue_batch(Batch *b) {
     HIP_OK(hb::exscan_u64(tmp, t, (const uint64_t *)(bd.job_cnt + (size_t)hb::kJobAudit * stride),
                           audit_off, stride, st));
   }
-  hb::launch_scatter(bd, fin, out_off, d_cnt + fill_off, (uint32_t *)dout, b->final_cap, aud,
-                     audit_off, d_cnt + afill_off, (hb_audit_entry *)daud, b->audit_cap,
-                     ctx->n_cu, st);
+  if (!direct)
+    hb::launch_scatter(bd, fin, out_off, d_cnt + fill_off, (uint32_t *)dout, b->final_cap, aud,
+                       audit_off, d_cnt + afill_off, (hb_audit_entry *)daud, b->audit_cap,
+                       ctx->n_cu, st);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(b->h_cnt, d_cnt, b->cnt_words * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, st));
@@ -1393,6 +1402,10 @@ hb_status batch_complete(Batch *b) {
   // exact size, so a finished job holds only its own output until released.
   {
     const uint32_t n = b->n_jobs();
+    if (n == 1) {  // written in place (no scatter): the ranges end at the list counts
+      b->h_cnt[b->cnt_out_off + 1] = b->h_cnt[kCntFinal];
+      b->h_cnt[b->cnt_audit_off + 1] = b->h_cnt[kCntAudit];
+    }
     const unsigned long long n_pairs = b->h_cnt[b->cnt_out_off + n];
     const unsigned long long n_audit = b->h_cnt[b->cnt_audit_off + n];
     b->res_cnt.assign(b->h_cnt, b->h_cnt + b->cnt_words);
