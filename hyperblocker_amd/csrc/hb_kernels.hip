@@ -683,9 +683,9 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     // Rows of the A tile by wave: wave w holds rows [64*AR*w, 64*AR*(w+1)),
     // so a partly filled last tile leaves whole waves idle, and idle waves
     // skip the pair tests (they still take part in the group's barriers).
-    const uint32_t wave = tid >> 6;
+    const uint32_t wave = wave_u32(tid >> 6);
     const uint32_t a_wave0 = a_tile * 256 * AR + wave * 64 * AR;
-    const bool wave_active = a_wave0 < a_cnt;
+    const bool wave_active = a_wave0 < a_cnt;  // wave-uniform (scalar operands)
     bool a_valid[AR];
     uint32_t a_idx[AR], ah[AR][8], apre[AR], la[AR];
     uint32_t nal0[AR], nal3[AR];
@@ -761,7 +761,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
       }
       // read once per chunk: `to.dense` changes only inside flushes, and a
       // chunk that starts sparse flushes only at its end
-      const bool dense = dense_item || to.dense;
+      const bool dense = dense_item || wave_u32(to.dense) != 0;
       // Without flushes between steps an idle wave has nothing to do here.
       if (!dense && !wave_active) continue;
       // One step: the lanes' A rows against B rows jj .. jj+NB-1 of the
@@ -846,22 +846,34 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
 #pragma unroll
           for (int r = 0; r < AR; r++)
             slow_w[r] = SIM ? wave_u64(s_slow[SIM ? wave * AR + r : 0][jj >> 6]) : 0ull;
-          for (; jj < seg_end; jj += (BU == 1 || jj + 1 == seg_end) ? 1u : (uint32_t)BU) {
-            // Where survivors are dense (no similarity test, a loose first
-            // threshold, or a large last flush) the buffer is also flushed
-            // between steps: a step adds at most 256 * AR * BU <= kTileOut / 2
-            // entries, so it never overflows.  Every wave of the group runs
-            // the same (run, jj) sequence and reads the same `dense` (written
-            // only between barriers), so all reach the barrier together.
-            if (dense && jj != run) {
+          // Where survivors are dense (no similarity test, a loose first
+          // threshold, or a large last flush) the buffer is also flushed
+          // between steps: a step adds at most 256 * AR * BU <= kTileOut / 2
+          // entries, so it never overflows.  Every wave of the group runs
+          // the same (run, jj) sequence and reads the same `dense` (written
+          // only between barriers), so all reach the barrier together.
+          auto between = [&](uint32_t j) {
+            if (dense && j != run) {
               __syncthreads();
               if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
             }
-            if (!wave_active) continue;
-            if (BU == 1 || jj + 1 == seg_end)
+          };
+          if (!dense) {  // only active waves get here
+            for (; jj + BU <= seg_end; jj += BU) step(jj, std::integral_constant<int, BU>{}, slow_w);
+            if (BU > 1 && jj < seg_end) {  // one B row left in the segment
               step(jj, std::integral_constant<int, 1>{}, slow_w);
-            else
-              step(jj, std::integral_constant<int, BU>{}, slow_w);
+              jj++;
+            }
+          } else {
+            for (; jj + BU <= seg_end; jj += BU) {
+              between(jj);
+              if (wave_active) step(jj, std::integral_constant<int, BU>{}, slow_w);
+            }
+            if (BU > 1 && jj < seg_end) {
+              between(jj);
+              if (wave_active) step(jj, std::integral_constant<int, 1>{}, slow_w);
+              jj++;
+            }
           }
         }
         run = run_end;
