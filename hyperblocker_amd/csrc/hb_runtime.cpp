@@ -252,11 +252,13 @@ class EventPool {
 
 // Byte-string -> code dictionary shared by every table of a context, so that
 // codes from any two columns compare equal exactly when the strings do.
-// Dictionary codes of equality cells, shared by every table of a context so
-// that equal byte strings get equal codes across tables.  Sharded by hash,
-// each shard with its own lock and count, so that many threads code one
-// column at once without a shared write; code = shard + kShards * (index
-// within the shard).
+// Sharded by hash: each shard is an open-addressing table with its own lock
+// and count, and code = shard + kShards * (index within the shard).  A
+// column is coded in one batch (code_strings below): its values are hashed
+// and grouped by shard on all worker threads, then each shard's values are
+// looked up, and new ones inserted, by one thread that takes the shard's
+// lock once.  Codes are handed out in row order within a shard, so they do
+// not depend on thread timing.
 class Dictionary {
  public:
   static constexpr uint32_t kShards = 256;
@@ -269,25 +271,33 @@ class Dictionary {
     }
     return b;
   }
-  uint32_t code(std::string_view s) {
-    const size_t h = std::hash<std::string_view>{}(s);
-    Shard &sh = shards_[h % kShards];
-    std::lock_guard<std::mutex> lk(sh.mu);
-    auto it = sh.map.find(s);
-    if (it != sh.map.end()) return it->second;
-    char *dst = sh.store(s);
-    const uint32_t idx = (uint32_t)sh.map.size();
-    const uint32_t c = (uint32_t)(h % kShards) + kShards * idx;
-    sh.map.emplace(std::string_view(dst, s.size()), c);
-    sh.count.store(idx + 1, std::memory_order_release);
-    return c;
+  static uint64_t hash(std::string_view s) { return std::hash<std::string_view>{}(s); }
+  static uint32_t shard_of(uint64_t h) { return (uint32_t)(h % kShards); }
+  // Codes of n values of shard `sh` (rows[i] -> value get(rows[i]), hash
+  // hashes[rows[i]]), written to codes[rows[i]].
+  template <class Get>
+  void code_shard(uint32_t sh, const uint32_t *rows, size_t n, const Get &get,
+                  const uint64_t *hashes, uint32_t *codes) {
+    Shard &d = shards_[sh];
+    std::lock_guard<std::mutex> lk(d.mu);
+    for (size_t i = 0; i < n; i++) {
+      const uint32_t r = rows[i];
+      codes[r] = d.find_or_insert(get(r), hashes[r], sh);
+    }
   }
 
  private:
+  static constexpr uint32_t kEmpty = 0xffffffffu;
+  struct Slot {
+    uint64_t hash;
+    const char *ptr;
+    uint32_t len;
+    uint32_t code;  // kEmpty: free slot
+  };
   struct alignas(64) Shard {
     std::mutex mu;
     std::atomic<uint32_t> count{0};
-    std::unordered_map<std::string_view, uint32_t> map;
+    std::vector<Slot> slots;  // power-of-two size, at most half full
     std::vector<std::unique_ptr<char[]>> blocks;
     size_t used = 0;
     char *store(std::string_view s) {
@@ -299,6 +309,34 @@ class Dictionary {
       if (!s.empty()) std::memcpy(dst, s.data(), s.size());
       used += s.size();
       return dst;
+    }
+    void grow() {
+      std::vector<Slot> old(std::max<size_t>(64, slots.size() * 2), Slot{0, nullptr, 0, kEmpty});
+      old.swap(slots);
+      const size_t mask = slots.size() - 1;
+      for (const Slot &x : old) {
+        if (x.code == kEmpty) continue;
+        size_t i = (x.hash >> 8) & mask;
+        while (slots[i].code != kEmpty) i = (i + 1) & mask;
+        slots[i] = x;
+      }
+    }
+    uint32_t find_or_insert(std::string_view s, uint64_t h, uint32_t sh) {
+      if (slots.empty()) grow();
+      const size_t mask = slots.size() - 1;
+      size_t i = (h >> 8) & mask;  // the low bits chose the shard
+      for (;; i = (i + 1) & mask) {
+        const Slot &x = slots[i];
+        if (x.code == kEmpty) break;
+        if (x.hash == h && x.len == s.size() && std::memcmp(x.ptr, s.data(), s.size()) == 0)
+          return x.code;
+      }
+      const uint32_t idx = count.load(std::memory_order_relaxed);
+      const uint32_t c = sh + kShards * idx;
+      slots[i] = Slot{h, store(s), (uint32_t)s.size(), c};
+      count.store(idx + 1, std::memory_order_release);
+      if (2 * (size_t)(idx + 1) > slots.size()) grow();
+      return c;
     }
   };
   static constexpr size_t kBlock = 1 << 16;
@@ -400,6 +438,81 @@ void parallel_for(int64_t n, int64_t min_chunk, F fn) {
   pool.run((int)T, [&](int k) {
     const int64_t b = k * step, e = std::min(n, b + step);
     if (b < e) fn(b, e);
+  });
+}
+
+// codes[r] = dictionary code of get(r) for r in [0, n), on the worker pool.
+// A row equal to a recent row of its chunk (a small cache by hash) takes
+// that row's code afterwards, so columns with few distinct values send few
+// rows through the shards.
+template <class Get>
+void code_strings(Dictionary &d, size_t n, const Get &get, uint32_t *codes) {
+  constexpr uint32_t S = Dictionary::kShards;
+  if (n == 0) return;
+  std::vector<uint64_t> hashes(n);
+  std::vector<uint32_t> rows(n), rep(n);
+  const int T = (int)std::min<int64_t>(WorkerPool::get().size(),
+                                       std::max<int64_t>(1, (int64_t)n / 2048));
+  const size_t step = (n + T - 1) / T;
+  std::vector<uint32_t> hist((size_t)T * S, 0);  // rows of chunk k in shard s: hist[k * S + s]
+  auto run = [&](int n_tasks, const std::function<void(int)> &task) {
+    if (n_tasks <= 1) {
+      for (int k = 0; k < n_tasks; k++) task(k);
+    } else {
+      WorkerPool::get().run(n_tasks, task);
+    }
+  };
+  run(T, [&](int k) {
+    const size_t b = k * step, e = std::min(n, b + step);
+    uint32_t *hk = hist.data() + (size_t)k * S;
+    constexpr uint32_t kCache = 256;
+    uint32_t cache[kCache];  // row of a recent representative per hash slot
+    std::fill(cache, cache + kCache, 0xffffffffu);
+    for (size_t r = b; r < e; r++) {
+      const std::string_view v = get(r);
+      const uint64_t h = Dictionary::hash(v);
+      hashes[r] = h;
+      uint32_t &c = cache[(h >> 16) % kCache];
+      if (c != 0xffffffffu && hashes[c] == h && get(c) == v) {
+        rep[r] = c;
+        continue;
+      }
+      c = (uint32_t)r;
+      rep[r] = (uint32_t)r;
+      hk[Dictionary::shard_of(h)]++;
+    }
+  });
+  // offsets: shard-major, chunk-minor, so each shard's rows stay in row order
+  std::vector<size_t> shard_off(S + 1, 0);
+  std::vector<size_t> cur((size_t)T * S);
+  {
+    size_t o = 0;
+    for (uint32_t sh = 0; sh < S; sh++) {
+      shard_off[sh] = o;
+      for (int k = 0; k < T; k++) {
+        cur[(size_t)k * S + sh] = o;
+        o += hist[(size_t)k * S + sh];
+      }
+    }
+    shard_off[S] = o;
+  }
+  run(T, [&](int k) {
+    const size_t b = k * step, e = std::min(n, b + step);
+    size_t *ck = cur.data() + (size_t)k * S;
+    for (size_t r = b; r < e; r++)
+      if (rep[r] == r) rows[ck[Dictionary::shard_of(hashes[r])]++] = (uint32_t)r;
+  });
+  run(T > 1 ? (int)S : 1, [&](int k) {
+    const uint32_t lo = T > 1 ? (uint32_t)k : 0u, hi = T > 1 ? (uint32_t)k + 1 : S;
+    for (uint32_t sh = lo; sh < hi; sh++)
+      if (shard_off[sh + 1] > shard_off[sh])
+        d.code_shard(sh, rows.data() + shard_off[sh], shard_off[sh + 1] - shard_off[sh], get,
+                     hashes.data(), codes);
+  });
+  run(T, [&](int k) {  // a representative precedes its copies within the chunk
+    const size_t b = k * step, e = std::min(n, b + step);
+    for (size_t r = b; r < e; r++)
+      if (rep[r] != r) codes[r] = codes[rep[r]];
   });
 }
 
@@ -777,6 +890,28 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   return record_lazy(col.sig, stream);
 }
 
+// Host time of the phases of a call, printed to stderr when the environment
+// sets HB_HOST_TRACE (a diagnostic for host-bound pipelines).
+struct HostTrace {
+  const char *label;
+  bool on = std::getenv("HB_HOST_TRACE") != nullptr;
+  Clock::time_point t0 = Clock::now(), last = t0;
+  std::string line;
+  explicit HostTrace(const char *l) : label(l) {}
+  void mark(const char *what) {
+    if (!on) return;
+    const auto now = Clock::now();
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), " %s %.3f", what,
+                  std::chrono::duration<double, std::milli>(now - last).count());
+    line += buf;
+    last = now;
+  }
+  ~HostTrace() {
+    if (on) std::fprintf(stderr, "%s ms:%s\n", label, line.c_str());
+  }
+};
+
 hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   Column &col = t->cols[c];
   if (t->parent) {  // borrow the parent's codes at the slice's offset
@@ -795,27 +930,12 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   const size_t n = (size_t)t->n_rows;
   HB_TRY(t->host_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.h_code));
   {
-    Dictionary &d = t->ctx->dict;
     const char *bytes = col.h_bytes;
-    uint32_t *codes = col.h_code;
     const uint32_t w = col.width;
-    parallel_for((int64_t)n, 4096, [&](int64_t b, int64_t e) {
-      // a thread-local cache in front of the shared dictionary: columns with
-      // few distinct values would otherwise have every thread wait on the
-      // same few shard locks
-      std::unordered_map<std::string_view, uint32_t> local;
-      for (int64_t r = b; r < e; r++) {
-        const char *s = bytes + (size_t)r * w;
-        const std::string_view v(s, strnlen(s, w));
-        auto it = local.find(v);
-        if (it != local.end()) {
-          codes[r] = it->second;
-          continue;
-        }
-        codes[r] = d.code(v);
-        if (local.size() < 4096) local.emplace(v, codes[r]);
-      }
-    });
+    code_strings(t->ctx->dict, n, [&](size_t r) {
+      const char *s = bytes + r * w;
+      return std::string_view(s, strnlen(s, w));
+    }, col.h_code);
   }
   HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.d_code));
   if (n) HIP_OK(hipMemcpyAsync(col.d_code, col.h_code, n * sizeof(uint32_t),
@@ -852,27 +972,21 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
   const size_t n = (size_t)t->n_rows;
   HB_TRY(t->host_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.h_code));
   {
-    Dictionary &d = t->ctx->dict;
     std::vector<const uint32_t *> src;
     for (int c : cols) src.push_back(t->cols[c].h_code);
-    uint32_t *codes = cp.h_code;
-    const size_t k = cols.size();
+    const size_t k = cols.size(), kb = 1 + 4 * k;
+    // key bytes per row: a marker byte, then the column codes (4 bytes each)
+    std::vector<char> keys(n * kb);
     parallel_for((int64_t)n, 4096, [&](int64_t b, int64_t e) {
-      // key bytes: a marker byte, then the column codes (4 bytes each)
-      std::string key(1 + 4 * k, '\0');
-      key[0] = (char)(0xC0 + k);
-      std::unordered_map<std::string, uint32_t> local;  // as in ensure_code
       for (int64_t r = b; r < e; r++) {
-        for (size_t q = 0; q < k; q++) std::memcpy(&key[1 + 4 * q], &src[q][r], 4);
-        auto it = local.find(key);
-        if (it != local.end()) {
-          codes[r] = it->second;
-          continue;
-        }
-        codes[r] = d.code(key);
-        if (local.size() < 4096) local.emplace(key, codes[r]);
+        char *key = keys.data() + (size_t)r * kb;
+        key[0] = (char)(0xC0 + k);
+        for (size_t q = 0; q < k; q++) std::memcpy(key + 1 + 4 * q, &src[q][r], 4);
       }
     });
+    code_strings(t->ctx->dict, n, [&](size_t r) {
+      return std::string_view(keys.data() + r * kb, kb);
+    }, cp.h_code);
   }
   HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.d_code));
   if (n) HIP_OK(hipMemcpyAsync(cp.d_code, cp.h_code, n * sizeof(uint32_t),
@@ -916,11 +1030,14 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       return wait_lazy(it->second.lz, caller);
     }
   }
+  HostTrace trace("ensure_layout");
   const hipStream_t stream = t->ctx->build_stream;
   HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
   const uint32_t *key_codes = nullptr, *h_key = nullptr;
   if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes, &h_key));
+  trace.mark("key");
   if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
+  trace.mark("sig");
   uint64_t n_codes = 1;
   if (!eq.empty()) n_codes = t->ctx->dict.code_bound();
   std::lock_guard<std::mutex> lk(t->mu);
@@ -943,6 +1060,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     // holds, so the build is asynchronous: batches wait on its event, and its
     // temporaries return to the pool once the build stream passes them.
     ly.n_runs = eq.empty() ? 1u : count_distinct(h_key, n, n_codes);
+    trace.mark("distinct");
     DevicePool &pool = t->ctx->pool;
     hb_ctx::Grave temps;
     auto temp = [&](size_t bytes, void **p) -> hb_status {
@@ -999,6 +1117,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     t->ctx->graves.push_back(std::move(temps));
   }
   HB_TRY(record_lazy(ly.lz, stream));
+  trace.mark("launch");
   return wait_lazy(ly.lz, caller);
 }
 
@@ -1030,31 +1149,11 @@ unsigned long long pow2_at_least(unsigned long long x) {
   return p;
 }
 
-// Host time of the phases of enqueue_batch, printed to stderr when the
-// environment sets HB_HOST_TRACE (a diagnostic for host-bound pipelines).
-struct HostTrace {
-  bool on = std::getenv("HB_HOST_TRACE") != nullptr;
-  Clock::time_point t0 = Clock::now(), last = t0;
-  std::string line;
-  void mark(const char *what) {
-    if (!on) return;
-    const auto now = Clock::now();
-    char buf[64];
-    std::snprintf(buf, sizeof(buf), " %s %.3f", what,
-                  std::chrono::duration<double, std::milli>(now - last).count());
-    line += buf;
-    last = now;
-  }
-  ~HostTrace() {
-    if (on) std::fprintf(stderr, "enqueue_batch ms:%s\n", line.c_str());
-  }
-};
-
 hb_status enqueue_batch(Batch *b) {
   hb_ctx *ctx = b->ctx;
   hipStream_t st = b->stream;
   const uint32_t n_jobs = b->n_jobs();
-  HostTrace trace;
+  HostTrace trace("enqueue_batch");
 
   // Tables ready, lazy structures and layouts; job-rule descriptors.
   struct JR {
@@ -1777,6 +1876,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
     n_rows = v->n_rows;
   }
   HB_TRY(set_device(ctx));
+  HostTrace trace("table_upload");
   auto t = std::make_unique<hb_table>();
   t->ctx = ctx;
   t->n_rows = n_rows;
@@ -1805,6 +1905,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
     HB_TRY(t->dev_alloc(bytes, (void **)&col.d_bytes));
     if (n_rows == 0) std::memset(col.h_bytes, 0, bytes);
   }
+  trace.mark("widths+alloc");
   // pack (each cell NUL padded to its column's width)
   parallel_for(n_rows, 4096, [&](int64_t b, int64_t e) {
     for (int c = 0; c < nc; c++) {
@@ -1827,6 +1928,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
   }
   HIP_OK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
   HIP_OK(hipEventRecord(t->ready, ctx->copy_stream));
+  trace.mark("pack+copy");
   *out = t.release();
   return HB_OK;
 }
