@@ -554,6 +554,16 @@ namespace {
 struct Lazy {
   bool built = false;
   hipEvent_t ev = nullptr;
+  // ev has completed since it was last recorded: waiting on it is a no-op
+  mutable std::atomic<bool> done{false};
+  Lazy() = default;
+  Lazy(const Lazy &o) : built(o.built), ev(o.ev), done(o.done.load()) {}
+  Lazy &operator=(const Lazy &o) {
+    built = o.built;
+    ev = o.ev;
+    done = o.done.load();
+    return *this;
+  }
 };
 
 struct Column {
@@ -607,7 +617,8 @@ struct hb_table {
   std::vector<Column> cols;
   std::map<std::pair<std::vector<int>, int>, Layout> layouts;  // (equality columns, similarity column)
   std::map<std::vector<int>, Composite> composites;             // (equality columns)
-  hipEvent_t ready = nullptr;
+  hipEvent_t ready = nullptr;                // upload finished (a slice: its root's)
+  mutable std::atomic<bool> ready_done{false};  // `ready` seen complete
   std::mutex mu;  // guards lazy builds
   // A slice borrows rows [row_off, row_off + n_rows) of its parent's column
   // data (bytes, lengths, signatures, planes, codes); its layouts are its own.
@@ -846,13 +857,33 @@ void batch_free_memory(Batch *b) {
 }
 
 // Waits (on `stream`) for a lazy structure built by another stream.
+// Makes `stream` wait for the table's upload (nothing once it is seen done).
+hb_status wait_ready(const hb_table *t, hipStream_t stream) {
+  if (!t->ready || t->ready_done.load(std::memory_order_acquire)) return HB_OK;
+  if (hipEventQuery(t->ready) == hipSuccess) {
+    t->ready_done.store(true, std::memory_order_release);
+    return HB_OK;
+  }
+  HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
+  return HB_OK;
+}
+
+// Makes `stream` wait for the structure's build; once the build is seen
+// complete, later calls return at once (batches of thousands of jobs call
+// this for every layout they use).
 hb_status wait_lazy(const Lazy &lz, hipStream_t stream) {
-  if (lz.ev) HIP_OK(hipStreamWaitEvent(stream, lz.ev, 0));
+  if (!lz.ev || lz.done.load(std::memory_order_acquire)) return HB_OK;
+  if (hipEventQuery(lz.ev) == hipSuccess) {
+    lz.done.store(true, std::memory_order_release);
+    return HB_OK;
+  }
+  HIP_OK(hipStreamWaitEvent(stream, lz.ev, 0));
   return HB_OK;
 }
 
 hb_status record_lazy(Lazy &lz, hipStream_t stream) {
   if (!lz.ev) HIP_OK(hipEventCreateWithFlags(&lz.ev, hipEventDisableTiming));
+  lz.done.store(false, std::memory_order_release);
   HIP_OK(hipEventRecord(lz.ev, stream));
   lz.built = true;
   return HB_OK;
@@ -1032,7 +1063,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   }
   HostTrace trace("ensure_layout");
   const hipStream_t stream = t->ctx->build_stream;
-  HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
+  HB_TRY(wait_ready(t, stream));
   const uint32_t *key_codes = nullptr, *h_key = nullptr;
   if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes, &h_key));
   trace.mark("key");
@@ -1165,8 +1196,8 @@ hb_status enqueue_batch(Batch *b) {
   std::vector<JR> jrs;
   for (uint32_t j = 0; j < n_jobs; j++) {
     Batch::Item &it = b->items[j];
-    HIP_OK(hipStreamWaitEvent(st, it.L->ready, 0));
-    if (it.R != it.L) HIP_OK(hipStreamWaitEvent(st, it.R->ready, 0));
+    HB_TRY(wait_ready(it.L, st));
+    if (it.R != it.L) HB_TRY(wait_ready(it.R, st));
     for (int k = 0; k < (int)it.plan->rules.size(); k++) {
       const PlanRule &rule = it.plan->rules[k];
       for (const auto &s : rule.sims) {
@@ -1604,15 +1635,18 @@ hb_status register_jobs(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
     job->t_submit = now;
     jobs.push_back(std::move(job));
   }
+  {  // ids repeated within the batch
+    std::vector<uint64_t> sorted(ids, ids + n);
+    std::sort(sorted.begin(), sorted.end());
+    for (uint32_t j = 1; j < n; j++)
+      if (sorted[j] == sorted[j - 1])
+        return fail(HB_ERR_INVALID_ARG, "job id %llu repeated in the batch",
+                    (unsigned long long)sorted[j]);
+  }
   std::lock_guard<std::mutex> lk(ctx->mu);
-  for (uint32_t j = 0; j < n; j++) {
+  for (uint32_t j = 0; j < n; j++)
     if (ctx->jobs.count(ids[j]))
       return fail(HB_ERR_INVALID_ARG, "job id %llu already in use", (unsigned long long)ids[j]);
-    for (uint32_t k = 0; k < j; k++)
-      if (ids[k] == ids[j])
-        return fail(HB_ERR_INVALID_ARG, "job id %llu repeated in the batch",
-                    (unsigned long long)ids[j]);
-  }
   out.clear();
   for (auto &job : jobs) {
     out.push_back(job.get());
@@ -1713,9 +1747,13 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
 hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
                        const hb_table *const *ls, const hb_table *const *rs,
                        const uint64_t *ids) {
+  HostTrace trace("submit_batch");
   std::vector<Job *> jobs;
   HB_TRY(register_jobs(ctx, n, plans, ls, rs, ids, jobs));
-  return launch_batch(ctx, jobs);
+  trace.mark("register");
+  const hb_status st = launch_batch(ctx, jobs);
+  trace.mark("launch");
+  return st;
 }
 
 // The job's batch; waits for a queued job to be admitted when `block`.
