@@ -1922,15 +1922,22 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
   t->cols.resize(v->n_cols);
   auto src_row = [&](int64_t i) { return rows ? rows[i] : i; };
   const int nc = v->n_cols;
-  // cell widths: longest cell + NUL, rounded up to 16
+  // cell widths: longest cell + NUL, rounded up to 16.  The lengths are
+  // kept for the packing pass (up to 2^28 cells; beyond, measured again).
   std::vector<size_t> maxlen(nc, 0);
+  std::vector<uint32_t> lens;
+  if ((uint64_t)n_rows * (uint64_t)nc <= (1ull << 28)) lens.resize((size_t)n_rows * nc);
   std::mutex mm;
   parallel_for(n_rows, 4096, [&](int64_t b, int64_t e) {
     std::vector<size_t> ml(nc, 0);
     for (int c = 0; c < nc; c++) {
       const int64_t stride = v->col_stride[c];
-      for (int64_t i = b; i < e; i++)
-        ml[c] = std::max(ml[c], strnlen(v->col_base[c] + src_row(i) * stride, (size_t)stride));
+      uint32_t *lc = lens.empty() ? nullptr : lens.data() + (size_t)c * n_rows;
+      for (int64_t i = b; i < e; i++) {
+        const size_t len = strnlen(v->col_base[c] + src_row(i) * stride, (size_t)stride);
+        if (lc) lc[i] = (uint32_t)len;
+        ml[c] = std::max(ml[c], len);
+      }
     }
     std::lock_guard<std::mutex> lk(mm);
     for (int c = 0; c < nc; c++) maxlen[c] = std::max(maxlen[c], ml[c]);
@@ -1949,9 +1956,10 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
     for (int c = 0; c < nc; c++) {
       Column &col = t->cols[c];
       const int64_t stride = v->col_stride[c];
+      const uint32_t *lc = lens.empty() ? nullptr : lens.data() + (size_t)c * n_rows;
       for (int64_t i = b; i < e; i++) {
         const char *s = v->col_base[c] + src_row(i) * stride;
-        const size_t len = strnlen(s, (size_t)stride);
+        const size_t len = lc ? lc[i] : strnlen(s, (size_t)stride);
         char *dst = col.h_bytes + (size_t)i * col.width;
         std::memcpy(dst, s, len);
         std::memset(dst + len, 0, col.width - len);
