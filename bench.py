@@ -24,11 +24,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Algorithmic int32 lane-ops per candidate pair of the bound filter (the
-# dominant kernel): the test "SAD of the 32 byte-class counters > allowance
-# for (first-bytes class, S, T)" needs 8 v_sad_u8 (32 counters, 4 per
-# dword), 1 XOR of the 4-byte prefixes, 2 compares for the prefix class,
-# 1 select of the allowance and 1 final compare.
-FILTER_OPS_PER_CANDIDATE = 13
+# dominant kernel): the test "SAD of the 32 byte-class counters > allowance"
+# needs 8 v_sad_u8 (32 counters, 4 per dword) and 1 compare.  The allowance
+# depends on the pair's first-bytes class and lengths; it is computed per
+# run of equal B length, and A rows are ordered by first byte so that the
+# class is known per 64-row slice for most B rows, so no per-pair work is
+# counted for it (the per-pair class test, 4 more ops, runs only where a B
+# row's first byte falls in a slice's range).
+FILTER_OPS_PER_CANDIDATE = 9
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 int32 lanes/clk x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
 # Filter-stage HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE,
@@ -43,6 +46,18 @@ def measured_traffic(workload):
     for line in open(TRAFFIC_PROFILE):
         if line.startswith("traffic_bytes_per_launch"):
             return int(line.split("=")[1])
+    return None
+
+
+def measured_valu_per_candidate(workload):
+    """VALU lane-instructions per candidate of k_filter_tiles recorded by the
+    SQ counter run, or None."""
+    path = os.path.join(ROOT, "profiles", "k_filter_tiles_c2_sq_counters.txt")
+    if workload != "c2" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if "VALU lane-instructions per candidate" in line:
+            return float(line.rsplit("=", 1)[1])
     return None
 
 
@@ -201,6 +216,7 @@ def main():
                 "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
                 "ops_per_candidate": FILTER_OPS_PER_CANDIDATE, "candidates_per_launch": cand,
                 "avg_launch_ms": filt_ms,
+                "measured_valu_instr_per_candidate": measured_valu_per_candidate(args.workload),
                 "hbm_gbs": None if traffic is None else traffic / (filt_ms * 1e-3) / 1e9,
                 "hbm_peak_gbs": HBM_PEAK_GBS}
     cpu = None

@@ -636,7 +636,7 @@ struct __attribute__((aligned(8))) BRow {
 };
 
 template <bool SIM, int AR, int BU>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 6 : 8))) void
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 5 : (AR == 2 ? 7 : 8)))) void
 k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_t g_end,
                PairList out) {
   constexpr uint32_t kChunk = 256;
