@@ -248,6 +248,17 @@ class Context:
         check(lib().hb_batch_submit(self._h, n, plans, ls, rs, idarr))
         return [Job(self, i) for i in ids]
 
+    def batch_graph(self, jobs):
+        """jobs: list of (plan, l_table, r_table), run repeatedly as one
+        captured batch (hb_batch_graph_create).  Returns a BatchGraph."""
+        n = len(jobs)
+        plans = (C.c_void_p * n)(*[p._h.value for p, _, _ in jobs])
+        ls = (C.c_void_p * n)(*[l._h.value for _, l, _ in jobs])
+        rs = (C.c_void_p * n)(*[r._h.value for _, _, r in jobs])
+        h = C.c_void_p()
+        check(lib().hb_batch_graph_create(self._h, n, plans, ls, rs, C.byref(h)))
+        return BatchGraph(self, h, n)
+
     def batch_results(self, job):
         """All results of the batch `job` ran in (hb_batch_results): (job ids,
         pairs uint32[n, 2], pair offsets uint64[n_jobs + 1], audit entries,
@@ -382,6 +393,35 @@ class DevicePlan:
         if self._h and self._h.value and self.ctx._h:
             check(lib().hb_plan_release(self.ctx._h, self._h))
         self._h = None
+
+
+class BatchGraph:
+    """A job set captured as one HIP graph (hb_batch_graph_*); each run
+    replays it as a batch of new jobs."""
+
+    def __init__(self, ctx, handle, n_jobs):
+        self.ctx = ctx
+        self._h = handle
+        self.n_jobs = n_jobs
+
+    def run(self, job_ids=None):
+        """Replays the graph; returns the runs' Job handles (one per job, in
+        the order the graph was created with)."""
+        ids = ([next(_job_ids) for _ in range(self.n_jobs)] if job_ids is None
+               else list(job_ids))
+        check(lib().hb_batch_graph_run(self._h, (C.c_uint64 * self.n_jobs)(*ids)))
+        return [Job(self.ctx, i) for i in ids]
+
+    def release(self):
+        if self._h:
+            check(lib().hb_batch_graph_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 class Job:

@@ -32,7 +32,8 @@ EXPORTS = [
     "hb_table_upload", "hb_table_upload_serialized", "hb_table_upload_rows",
     "hb_table_slice", "hb_table_info", "hb_table_release",
     "hb_plan_create", "hb_plan_release",
-    "hb_job_submit", "hb_batch_submit", "hb_job_query", "hb_job_wait", "hb_job_results",
+    "hb_job_submit", "hb_batch_submit", "hb_batch_graph_create", "hb_batch_graph_run",
+    "hb_batch_graph_destroy", "hb_job_query", "hb_job_wait", "hb_job_results",
     "hb_job_stats_get", "hb_job_release", "hb_batch_results", "hb_jobs_release", "hb_block", "hb_free", "hb_autotune_filter_shape",
     "hb_server_create", "hb_server_submit", "hb_server_info", "hb_server_destroy",
     "hb_rules_load", "hb_rules_parse", "hb_host_plan_view", "hb_host_plan_free",
@@ -131,6 +132,10 @@ def lib():
         "hb_job_submit": ([VP, VP, VP, VP, C.c_uint64], st),
         "hb_batch_submit": ([VP, C.c_uint32, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
                              C.POINTER(C.c_uint64)], st),
+        "hb_batch_graph_create": ([VP, C.c_uint32, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
+                                   PVP], st),
+        "hb_batch_graph_run": ([VP, C.POINTER(C.c_uint64)], st),
+        "hb_batch_graph_destroy": ([VP], st),
         "hb_job_query": ([VP, C.c_uint64], st),
         "hb_job_wait": ([VP, C.c_uint64], st),
         "hb_job_results": ([VP, C.c_uint64, C.POINTER(C.POINTER(C.c_uint32)),

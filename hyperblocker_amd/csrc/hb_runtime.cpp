@@ -797,6 +797,15 @@ struct Batch {
   float join_ms = 0, filter_ms = 0, exact_ms = 0, device_ms = 0;
   std::chrono::steady_clock::time_point t_done;  // completion first observed
   size_t mem_estimate = 0;                       // working memory admitted for it
+  // Captured form (hb_batch_graph).  A template batch captures its launch
+  // sequence into graph_exec instead of launching it and keeps its memory;
+  // each run is a batch that borrows that memory (it owns none) and replays
+  // the graph.  A run that overflows reruns eagerly with memory of its own
+  // and marks the graph stale, so that the next run captures it again.
+  bool capture = false;
+  hipGraphExec_t graph_exec = nullptr;
+  bool borrowed = false;
+  std::shared_ptr<std::atomic<bool>> graph_stale;
 
   uint32_t n_jobs() const { return (uint32_t)items.size(); }
   uint32_t stride() const { return n_jobs() + 1; }
@@ -856,7 +865,6 @@ void batch_free_memory(Batch *b) {
   b->h_audit = nullptr;
 }
 
-// Waits (on `stream`) for a lazy structure built by another stream.
 // Makes `stream` wait for the table's upload (nothing once it is seen done).
 hb_status wait_ready(const hb_table *t, hipStream_t stream) {
   if (!t->ready || t->ready_done.load(std::memory_order_acquire)) return HB_OK;
@@ -1399,8 +1407,25 @@ hb_status enqueue_batch(Batch *b) {
   b->h_audit = (hb_audit_entry *)hp;
 
   trace.mark("alloc");
-  // Launch sequence.
-  HIP_OK(hipEventRecord(b->ev_start, st));
+  // Launch sequence.  A template batch captures it (after the waits above
+  // have completed, so the graph holds only this sequence) and records no
+  // events; a run replays it between its own start and end events.
+  const bool cap = b->capture;
+  struct CaptureGuard {  // ends an unfinished capture on an error return
+    hipStream_t s = nullptr;
+    ~CaptureGuard() {
+      if (!s) return;
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    }
+  } guard;
+  if (cap) {
+    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    guard.s = st;
+  } else {
+    HIP_OK(hipEventRecord(b->ev_start, st));
+  }
   HIP_OK(hipMemcpyAsync(ddesc, hdesc, o, hipMemcpyHostToDevice, st));
   hb::launch_fill_u64(d_cnt, b->cnt_words, 0ull, st);
   if (set_total) hb::launch_fill_u64(bd.sets.slots, set_total, ~0ull, st);
@@ -1409,7 +1434,7 @@ hb_status enqueue_batch(Batch *b) {
     size_t t = tmp_bytes;
     HIP_OK(hb::exscan_u64(tmp, t, g.items, g.item_off, n_groups + 1, st));
   }
-  HIP_OK(hipEventRecord(b->marks[0], st));
+  if (!cap) HIP_OK(hipEventRecord(b->marks[0], st));
   unsigned long long *list_cnt = d_cnt + b->cnt_stage_off;
   unsigned long long *defer_cnt = list_cnt + n_stage_slots;
   const hb::PairList list0{lists[0], list_cnt + 0, b->list_cap};
@@ -1422,7 +1447,7 @@ hb_status enqueue_batch(Batch *b) {
     q = e;
   }
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(b->marks[1], st));
+  if (!cap) HIP_OK(hipEventRecord(b->marks[1], st));
   const hb::PairList fin{final_buf, d_cnt + kCntFinal, b->final_cap};
   const hb::AuditList aud{audit_buf, d_cnt + kCntAudit, b->audit_cap};
   // list slots: 0 filter output, then (1 refined and) one per exact stage
@@ -1440,7 +1465,7 @@ hb_status enqueue_batch(Batch *b) {
                      direct ? (hb_audit_entry *)daud : nullptr, ctx->n_cu, st);
   }
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(b->marks[2], st));
+  if (!cap) HIP_OK(hipEventRecord(b->marks[2], st));
   // per-job output ranges (a batch of one job: both offsets stay 0)
   uint64_t *out_off = (uint64_t *)(d_cnt + b->cnt_out_off);
   uint64_t *audit_off = (uint64_t *)(d_cnt + b->cnt_audit_off);
@@ -1459,7 +1484,16 @@ hb_status enqueue_batch(Batch *b) {
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(b->h_cnt, d_cnt, b->cnt_words * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, st));
-  HIP_OK(hipEventRecord(b->ev_end, st));
+  if (cap) {
+    guard.s = nullptr;
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamEndCapture(st, &graph));
+    const hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_OK(e);
+  } else {
+    HIP_OK(hipEventRecord(b->ev_end, st));
+  }
   trace.mark("launch");
   return HB_OK;
 }
@@ -1519,14 +1553,22 @@ hb_status batch_complete(Batch *b) {
     if (b->retries > 12) return fail(HB_ERR_CAPACITY, "batch: output keeps overflowing");
     HIP_OK(hipStreamSynchronize(b->stream));
     batch_free_memory(b);
+    if (b->borrowed) {  // a graph run: rerun eagerly, and capture again next time
+      b->borrowed = false;
+      if (b->graph_stale) b->graph_stale->store(true);
+      for (auto &m : b->marks)
+        if (!m) HB_TRY(b->ctx->events.get(&m));
+    }
     HB_TRY(enqueue_batch(b));
     return HB_NOT_READY;
   }
   b->t_done = std::chrono::steady_clock::now();
   (void)hipEventElapsedTime(&b->device_ms, b->ev_start, b->ev_end);
-  (void)hipEventElapsedTime(&b->join_ms, b->ev_start, b->marks[0]);
-  (void)hipEventElapsedTime(&b->filter_ms, b->marks[0], b->marks[1]);
-  (void)hipEventElapsedTime(&b->exact_ms, b->marks[1], b->marks[2]);
+  if (b->marks[0] && b->marks[1] && b->marks[2]) {  // graph runs have no stage marks
+    (void)hipEventElapsedTime(&b->join_ms, b->ev_start, b->marks[0]);
+    (void)hipEventElapsedTime(&b->filter_ms, b->marks[0], b->marks[1]);
+    (void)hipEventElapsedTime(&b->exact_ms, b->marks[1], b->marks[2]);
+  }
   // The device buffers are idle now and go back to the pool at once; the
   // results move from the pinned staging buffers to heap memory of their
   // exact size, so a finished job holds only its own output until released.
@@ -1661,21 +1703,15 @@ size_t job_memory_estimate(const Job *j) {
   return (size_t)(j->L->n_rows + j->R->n_rows) * 160 + 4096;
 }
 
-// Runs registered jobs as one batch and assigns it to them.
-hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
-                       std::shared_ptr<Batch> *out = nullptr) {
-  HB_TRY(set_device(ctx));
-  HB_TRY(reap_graves(ctx, false));  // memory of released tables and finished layout builds
-  auto b = std::make_shared<Batch>();
-  b->ctx = ctx;
-  b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
-  const uint32_t n = (uint32_t)jobs.size();
+// Capacities of a batch whose items are set: pair lists, outputs and audit
+// entries from its plans' hints, dedupe sets, and its working memory.
+void size_batch(Batch *b) {
+  const uint32_t n = b->n_jobs();
   SizeHint want{0, 0, 0, 0, 0, 0};
   b->set_cap.assign(n, 0);
+  b->mem_estimate = 0;
   for (uint32_t j = 0; j < n; j++) {
-    Batch::Item it{jobs[j]->plan, jobs[j]->L, jobs[j]->R};
-    b->items.push_back(it);
-    b->job_ids.push_back(jobs[j]->id);
+    const Batch::Item &it = b->items[j];
     const unsigned long long jr = (unsigned long long)(it.L->n_rows + it.R->n_rows);
     const SizeHint h = it.plan->get_hint();
     want.list += h.list * jr;
@@ -1688,7 +1724,7 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
     // taken as the smaller side's rows (grown on overflow)
     if (it.plan->rules.size() > 1)
       b->set_cap[j] = pow2_at_least(4 * ((unsigned long long)std::min(it.L->n_rows, it.R->n_rows) + 256));
-    b->mem_estimate += job_memory_estimate(jobs[j]);
+    b->mem_estimate += (size_t)(it.L->n_rows + it.R->n_rows) * 160 + 4096;
   }
   // The row-proportional estimates are bounded (a batch of many jobs over
   // one large resident table would otherwise reserve for every job as if it
@@ -1703,6 +1739,22 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
       (unsigned long long)std::max((double)(1ull << 16), std::min(want.final_, final_max));
   b->audit_cap = (unsigned long long)std::max((double)(1ull << 12),
                                               std::min(want.audit, (double)(1ull << 24)));
+}
+
+// Runs registered jobs as one batch and assigns it to them.
+hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
+                       std::shared_ptr<Batch> *out = nullptr) {
+  HB_TRY(set_device(ctx));
+  HB_TRY(reap_graves(ctx, false));  // memory of released tables and finished layout builds
+  auto b = std::make_shared<Batch>();
+  b->ctx = ctx;
+  b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
+  const uint32_t n = (uint32_t)jobs.size();
+  for (uint32_t j = 0; j < n; j++) {
+    b->items.push_back(Batch::Item{jobs[j]->plan, jobs[j]->L, jobs[j]->R});
+    b->job_ids.push_back(jobs[j]->id);
+  }
+  size_batch(b.get());
   HB_TRY(ctx->events.get(&b->ev_start));
   HB_TRY(ctx->events.get(&b->ev_end));
   for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));
@@ -1765,10 +1817,135 @@ std::shared_ptr<Batch> job_batch(Job *j, bool block) {
 
 }  // namespace
 
+// A job set run repeatedly as one captured batch (hb_batch_graph_*).
+struct hb_batch_graph {
+  hb_ctx *ctx = nullptr;
+  std::vector<Batch::Item> items;
+  std::shared_ptr<Batch> tmpl;  // the captured batch: its memory and graph
+  std::shared_ptr<std::atomic<bool>> stale = std::make_shared<std::atomic<bool>>(false);
+  std::shared_ptr<Batch> last;  // the last run, completed before the next one starts
+  std::mutex mu;
+};
+
+namespace {
+
+// Prepares the graph's job set as a template batch and captures its launch
+// sequence (on a stream of its own, so that no other batch's work can enter
+// the capture).
+hb_status capture_graph(hb_batch_graph *g) {
+  hb_ctx *ctx = g->ctx;
+  HB_TRY(reap_graves(ctx, false));
+  auto b = std::make_shared<Batch>();
+  b->ctx = ctx;
+  b->items = g->items;
+  b->job_ids.assign(g->items.size(), 0);
+  size_batch(b.get());
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    b->shape = ctx->shape;
+  }
+  b->capture = true;
+  hipStream_t cs = nullptr;
+  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  b->stream = cs;
+  hb_status st;
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    st = enqueue_batch(b.get());
+  }
+  (void)hipStreamSynchronize(cs);
+  // the template's destructor synchronises a stream that outlives it
+  b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
+  (void)hipStreamDestroy(cs);
+  HB_TRY(st);
+  g->tmpl = b;
+  g->stale->store(false);
+  return HB_OK;
+}
+
+hb_status run_graph(hb_batch_graph *g, const uint64_t *ids) {
+  std::lock_guard<std::mutex> lk(g->mu);
+  hb_ctx *ctx = g->ctx;
+  HB_TRY(set_device(ctx));
+  if (g->last) {  // the previous run's results leave the shared buffers first
+    (void)batch_poll(g->last.get(), true);  // a failed run reports through its jobs
+    g->last.reset();
+  }
+  if (!g->tmpl || g->stale->load()) {
+    g->tmpl.reset();
+    HB_TRY(capture_graph(g));
+  }
+  const uint32_t n = (uint32_t)g->items.size();
+  std::vector<const hb_plan *> plans(n);
+  std::vector<const hb_table *> ls(n), rs(n);
+  for (uint32_t j = 0; j < n; j++) {
+    plans[j] = g->items[j].plan;
+    ls[j] = g->items[j].L;
+    rs[j] = g->items[j].R;
+  }
+  std::vector<Job *> jobs;
+  HB_TRY(register_jobs(ctx, n, plans.data(), ls.data(), rs.data(), ids, jobs));
+  const Batch &t = *g->tmpl;
+  auto r = std::make_shared<Batch>();
+  r->ctx = ctx;
+  r->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
+  r->items = t.items;
+  r->job_ids.assign(ids, ids + n);
+  r->shape = t.shape;
+  r->max_stage = t.max_stage;
+  r->list_cap = t.list_cap;
+  r->defer_cap = t.defer_cap;
+  r->final_cap = t.final_cap;
+  r->audit_cap = t.audit_cap;
+  r->set_cap = t.set_cap;
+  r->h_cnt = t.h_cnt;
+  r->cnt_words = t.cnt_words;
+  r->cnt_stage_off = t.cnt_stage_off;
+  r->cnt_job_off = t.cnt_job_off;
+  r->cnt_out_off = t.cnt_out_off;
+  r->cnt_audit_off = t.cnt_audit_off;
+  r->h_out = t.h_out;
+  r->h_audit = t.h_audit;
+  r->mem_estimate = t.mem_estimate;
+  r->borrowed = true;
+  r->graph_stale = g->stale;
+  hb_status st = HB_OK;
+  {
+    std::lock_guard<std::mutex> bl(r->mu);
+    auto launch = [&]() -> hb_status {
+      HB_TRY(ctx->events.get(&r->ev_start));
+      HB_TRY(ctx->events.get(&r->ev_end));
+      HIP_OK(hipEventRecord(r->ev_start, r->stream));
+      HIP_OK(hipGraphLaunch(t.graph_exec, r->stream));
+      HIP_OK(hipEventRecord(r->ev_end, r->stream));
+      return HB_OK;
+    };
+    st = launch();
+    if (st != HB_OK) {
+      r->state = BatchState::kFailed;
+      r->error = st;
+      r->error_text = g_last_error;
+    }
+  }
+  const auto now = Clock::now();
+  for (uint32_t j = 0; j < n; j++) {
+    std::lock_guard<std::mutex> jl(jobs[j]->mu);
+    jobs[j]->batch = r;
+    jobs[j]->index = j;
+    jobs[j]->t_admit = now;
+    jobs[j]->cv.notify_all();
+  }
+  g->last = r;
+  return st;
+}
+
+}  // namespace
+
 Batch::~Batch() {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (stream) (void)hipStreamSynchronize(stream);
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   batch_free_memory(this);
   ctx->events.put(ev_start);
   ctx->events.put(ev_end);
@@ -2240,6 +2417,46 @@ hb_status hb_batch_results(hb_ctx *ctx, uint64_t job_id, uint32_t *n_jobs,
 hb_status hb_jobs_release(hb_ctx *ctx, uint32_t n, const uint64_t *job_ids) {
   if (!ctx || (n && !job_ids)) return fail(HB_ERR_INVALID_ARG, "hb_jobs_release: null argument");
   for (uint32_t k = 0; k < n; k++) HB_TRY(hb_job_release(ctx, job_ids[k]));
+  return HB_OK;
+}
+
+hb_status hb_batch_graph_create(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
+                                const hb_table *const *l, const hb_table *const *r,
+                                hb_batch_graph **out) {
+  if (!out) return fail(HB_ERR_INVALID_ARG, "hb_batch_graph_create: out is null");
+  *out = nullptr;
+  if (!ctx || n_jobs == 0 || !plans || !l || !r)
+    return fail(HB_ERR_INVALID_ARG, "hb_batch_graph_create: no jobs or null arrays");
+  for (uint32_t j = 0; j < n_jobs; j++) HB_TRY(check_job_args(ctx, plans[j], l[j], r[j]));
+  HB_TRY(set_device(ctx));
+  auto g = std::make_unique<hb_batch_graph>();
+  g->ctx = ctx;
+  for (uint32_t j = 0; j < n_jobs; j++)
+    g->items.push_back(Batch::Item{plans[j], const_cast<hb_table *>(l[j]),
+                                   const_cast<hb_table *>(r[j])});
+  {
+    std::lock_guard<std::mutex> lk(g->mu);
+    HB_TRY(capture_graph(g.get()));
+  }
+  *out = g.release();
+  return HB_OK;
+}
+
+hb_status hb_batch_graph_run(hb_batch_graph *g, const uint64_t *job_ids) {
+  if (!g || !job_ids) return fail(HB_ERR_INVALID_ARG, "hb_batch_graph_run: null argument");
+  return run_graph(g, job_ids);
+}
+
+hb_status hb_batch_graph_destroy(hb_batch_graph *g) {
+  if (!g) return HB_OK;
+  {
+    std::lock_guard<std::mutex> lk(g->mu);
+    (void)set_device(g->ctx);
+    if (g->last) (void)batch_poll(g->last.get(), true);
+    g->last.reset();
+    g->tmpl.reset();
+  }
+  delete g;
   return HB_OK;
 }
 

@@ -202,6 +202,25 @@ hb_status hb_job_submit(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
 hb_status hb_batch_submit(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
                           const hb_table *const *l, const hb_table *const *r,
                           const uint64_t *job_ids);
+/* Captured batch: a job set run repeatedly (no counterpart in the
+ * reference).  hb_batch_graph_create prepares n_jobs jobs as for
+ * hb_batch_submit (per-table structures, descriptors, buffers sized from
+ * the plans' hints) and captures the batch's launch sequence into a HIP
+ * graph; it runs nothing.  Each hb_batch_graph_run replays the graph once
+ * as a batch whose jobs get the ids job_ids[0..n_jobs) (unused ids, read,
+ * waited for and released like those of hb_batch_submit; results equal the
+ * jobs submitted alone).  A run first completes the previous run of the
+ * graph (its buffers are the graph's); a run whose lists overflow reruns
+ * with larger ones like any batch, and the next run captures the graph
+ * again with those sizes.  Graph runs report no join/filter/exact stage
+ * times (0), only device_ms.  The plans and tables must outlive the graph.
+ * hb_batch_graph_destroy completes the last run and frees the graph. */
+typedef struct hb_batch_graph hb_batch_graph;
+hb_status hb_batch_graph_create(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
+                                const hb_table *const *l, const hb_table *const *r,
+                                hb_batch_graph **out);
+hb_status hb_batch_graph_run(hb_batch_graph *g, const uint64_t *job_ids);
+hb_status hb_batch_graph_destroy(hb_batch_graph *g);
 hb_status hb_job_query(hb_ctx *ctx, uint64_t job_id); /* HB_OK or HB_NOT_READY */
 hb_status hb_job_wait(hb_ctx *ctx, uint64_t job_id);
 /* (l_row, r_row) pairs as 2*n_pairs uint32, each matching pair exactly once,
