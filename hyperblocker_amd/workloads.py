@@ -131,6 +131,22 @@ def run_c4(ctx, n_jobs=4096, reps=2, inflight=4, autotune=True, log=print):
             matches = len(pairs)
             ctx.release_jobs(handles)
             batched.append(time.perf_counter() - t)
+        # captured: the same batch as one HIP graph, replayed
+        graph = ctx.batch_graph(jobs)
+        try:
+            graphed = []
+            gstats = None
+            for _ in range(reps + 1):
+                t = time.perf_counter()
+                handles = graph.run()
+                _, gpairs, _, _, _ = ctx.batch_results(handles[0])
+                gstats = handles[0].stats()
+                ctx.release_jobs(handles)
+                graphed.append(time.perf_counter() - t)
+                if len(gpairs) != matches:
+                    raise AssertionError("graph run: %d pairs, batch: %d" % (len(gpairs), matches))
+        finally:
+            graph.release()
         # plain: one job per submit, a few in flight
         plain = []
         for _ in range(reps):
@@ -142,11 +158,13 @@ def run_c4(ctx, n_jobs=4096, reps=2, inflight=4, autotune=True, log=print):
                     _drain([pending.pop(0)])
             _drain(pending)
             plain.append(time.perf_counter() - t)
-        bt, pt = min(batched), min(plain)
+        bt, pt, gt = min(batched), min(plain), min(graphed[1:])
         return {
             "jobs": n_jobs, "pairs": pairs_total, "matches": matches,
-            "batched_s": bt, "plain_s": pt,
+            "batched_s": bt, "plain_s": pt, "graph_s": gt,
             "batched_pairs_per_s": pairs_total / bt, "plain_pairs_per_s": pairs_total / pt,
+            "graph_pairs_per_s": pairs_total / gt, "graph_first_run_s": graphed[0],
+            "graph_device_ms": gstats["device_ms"],
             "speedup": pt / bt, "cold_s": cold_s,
             "batch_device_ms": stats["device_ms"], "batch_filter_ms": stats["filter_ms"],
             "batch_exact_ms": stats["exact_ms"], "batch_join_ms": stats["join_ms"],
