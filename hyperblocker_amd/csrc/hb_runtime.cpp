@@ -1944,7 +1944,10 @@ hb_status run_graph(hb_batch_graph *g, const uint64_t *ids) {
 Batch::~Batch() {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  if (stream) (void)hipStreamSynchronize(stream);
+  // A completed batch has no work left on the device (its end event was
+  // seen); otherwise its kernels may still use the memory freed below.
+  if (stream && (state != BatchState::kDone || !allocs.empty() || graph_exec))
+    (void)hipStreamSynchronize(stream);
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   batch_free_memory(this);
   ctx->events.put(ev_start);
