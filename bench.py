@@ -136,8 +136,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--inflight", type=int, default=2,
-                    help="jobs outstanding at once in the timed loop")
+    ap.add_argument("--inflight", type=int, default=4,
+                    help="jobs outstanding at once in the timed loop (a context has 4 "
+                         "batch streams)")
     ap.add_argument("--cpu-rows", type=int, default=20000,
                     help="left rows in the CPU baseline sample (0 = skip)")
     ap.add_argument("--jobs", type=int, default=0,
