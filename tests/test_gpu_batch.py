@@ -239,3 +239,27 @@ def test_batch_results_block_equals_each_job(ctx):
             tr.release()
         for p in plans.values():
             p.release()
+
+
+def test_release_before_completion(ctx):
+    """Jobs released while still running (never waited for) give their
+    memory back only once the device is done with it: the jobs submitted
+    around them, and later ones reusing that memory, stay exact."""
+    L, R, _, cfg = datagen.make("c1")
+    plan = hb.HostPlan.from_yaml(cfg["rules"])
+    dp = ctx.plan(plan)
+    tl, tr = ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))
+    try:
+        want = _results(ctx.submit(dp, tl, tr))
+        for _ in range(3):
+            jobs = [ctx.submit(dp, tl, tr) for _ in range(6)]
+            for j in jobs[1::2]:
+                j.release()  # still running, most likely
+            for j in jobs[0::2]:
+                got = _results(j)
+                H.assert_same_pairs(got[0], want[0], "neighbour of a released job")
+                H.assert_same_audit(got[1], want[1], "neighbour of a released job")
+    finally:
+        tl.release()
+        tr.release()
+        dp.release()
