@@ -151,7 +151,8 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: only --gpus 1 is supported; this build measures one MI355X "
-              "(see DESIGN.md for how the job shards across GPUs)", file=sys.stderr)
+              "(across GPUs the work splits by partitions or jobs, with no exchange step)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
