@@ -167,7 +167,7 @@ struct GroupArrays {
 // Launch shape of the filter stage (chosen per device by the autotuner).
 struct FilterShape {
   int rows_per_lane;   // 1, 2 or 4
-  uint32_t tile_b;     // multiple of 256
+  uint32_t tile_b;     // multiple of 64
   unsigned grid_per_cu;
   uint32_t small_pairs_max = 2048;  // groups up to this many pairs run one wave per group
 };

@@ -1170,8 +1170,8 @@ float filter_coef(double thr, int p) {
 }
 
 bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
-  return (rows_per_lane == 1 || rows_per_lane == 2 || rows_per_lane == 4) && tile_b >= 256 &&
-         tile_b <= 65536 && tile_b % 256 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
+  return (rows_per_lane == 1 || rows_per_lane == 2 || rows_per_lane == 4) && tile_b >= 64 &&
+         tile_b <= 65536 && tile_b % 64 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
 }
 
 // First similarity thresholds below this get the refine pass: there the
@@ -2644,8 +2644,8 @@ hb_status hb_autotune_filter_shape(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *
     }
   } else {
     for (int rpl : {1, 2, 4})
-      for (int tb : {256, 512})
-        for (int gpc : {8, 16, 32})
+      for (int tb : {128, 256, 512})
+        for (int gpc : {16, 32})
           cands.push_back(
               hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc, ctx->shape.small_pairs_max});
   }

@@ -121,7 +121,8 @@ def test_adversarial_parity(ctx, name, seed, alphabet, max_len, long_frac):
     H.assert_same_audit(audit, want_audit, name)
 
 
-FILTER_SHAPES = [(1, 256, 4), (2, 256, 16), (2, 512, 12), (4, 256, 8), (4, 1024, 6)]
+FILTER_SHAPES = [(1, 256, 4), (2, 256, 16), (2, 512, 12), (4, 256, 8), (4, 1024, 6), (2, 64, 64),
+                 (4, 192, 32)]
 
 
 @pytest.mark.parametrize("shape", FILTER_SHAPES)
@@ -145,7 +146,7 @@ def test_filter_shapes_give_identical_results(ctx, shape):
 
 
 def test_filter_shape_validation(ctx):
-    for bad in [(3, 256, 8), (2, 100, 8), (2, 256, 0), (2, 256, 65), (1, 131072, 4)]:
+    for bad in [(3, 256, 8), (2, 100, 8), (2, 32, 16), (2, 256, 0), (2, 256, 65), (1, 131072, 4)]:
         with pytest.raises(capi.HBError) as e:
             ctx.filter_shape = bad
         assert e.value.status == capi.HB_ERR_INVALID_ARG
