@@ -271,8 +271,39 @@ class Dictionary {
     }
     return b;
   }
-  static uint64_t hash(std::string_view s) { return std::hash<std::string_view>{}(s); }
+  // The dictionary's own hash (the partition rule's std::hash is separate):
+  // 8 bytes at a time, multiply-xor, then a 64-bit finalizer (MurmurHash3's
+  // fmix64); equality columns hold short strings.
+  static uint64_t hash(std::string_view s) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)s.size();
+    const char *p = s.data();
+    size_t n = s.size();
+    for (; n >= 8; p += 8, n -= 8) {
+      uint64_t w;
+      std::memcpy(&w, p, 8);
+      h = (h ^ w) * 0xff51afd7ed558ccdull;
+      h ^= h >> 32;
+    }
+    if (n) {
+      uint64_t w = 0;
+      std::memcpy(&w, p, n);
+      h = (h ^ w) * 0xc4ceb9fe1a85ec53ull;
+      h ^= h >> 29;
+    }
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+  }
   static uint32_t shard_of(uint64_t h) { return (uint32_t)(h % kShards); }
+  // Code of one value (hash h): for columns with few distinct values.
+  uint32_t code_one(std::string_view v, uint64_t h) {
+    const uint32_t sh = shard_of(h);
+    std::lock_guard<std::mutex> lk(shards_[sh].mu);
+    return shards_[sh].find_or_insert(v, h, sh);
+  }
   // Codes of n values of shard `sh` (rows[i] -> value get(rows[i]), hash
   // hashes[rows[i]]), written to codes[rows[i]].
   template <class Get>
@@ -465,50 +496,66 @@ void code_strings(Dictionary &d, size_t n, const Get &get, uint32_t *codes) {
   run(T, [&](int k) {
     const size_t b = k * step, e = std::min(n, b + step);
     uint32_t *hk = hist.data() + (size_t)k * S;
-    constexpr uint32_t kCache = 256;
-    uint32_t cache[kCache];  // row of a recent representative per hash slot
-    std::fill(cache, cache + kCache, 0xffffffffu);
+    // rows of recent representatives, two per hash slot (most recent first)
+    constexpr uint32_t kCache = 512, kNone = 0xffffffffu;
+    uint32_t cache[kCache][2];
+    for (auto &w : cache) w[0] = w[1] = kNone;
     for (size_t r = b; r < e; r++) {
       const std::string_view v = get(r);
       const uint64_t h = Dictionary::hash(v);
       hashes[r] = h;
-      uint32_t &c = cache[(h >> 16) % kCache];
-      if (c != 0xffffffffu && hashes[c] == h && get(c) == v) {
-        rep[r] = c;
+      uint32_t *w = cache[(h >> 16) % kCache];
+      const uint32_t c0 = w[0], c1 = w[1];
+      if (c0 != kNone && hashes[c0] == h && get(c0) == v) {
+        rep[r] = c0;
         continue;
       }
-      c = (uint32_t)r;
+      if (c1 != kNone && hashes[c1] == h && get(c1) == v) {
+        rep[r] = c1;
+        w[1] = c0;
+        w[0] = c1;
+        continue;
+      }
+      w[1] = c0;
+      w[0] = (uint32_t)r;
       rep[r] = (uint32_t)r;
       hk[Dictionary::shard_of(h)]++;
     }
   });
-  // offsets: shard-major, chunk-minor, so each shard's rows stay in row order
-  std::vector<size_t> shard_off(S + 1, 0);
-  std::vector<size_t> cur((size_t)T * S);
-  {
-    size_t o = 0;
-    for (uint32_t sh = 0; sh < S; sh++) {
-      shard_off[sh] = o;
-      for (int k = 0; k < T; k++) {
-        cur[(size_t)k * S + sh] = o;
-        o += hist[(size_t)k * S + sh];
+  size_t n_reps = 0;
+  for (uint32_t x : hist) n_reps += x;
+  if (n_reps <= 4096) {  // few distinct values: code them here, in row order
+    for (size_t r = 0; r < n; r++)
+      if (rep[r] == r) codes[r] = d.code_one(get(r), hashes[r]);
+  } else {
+    // offsets: shard-major, chunk-minor, so each shard's rows stay in row order
+    std::vector<size_t> shard_off(S + 1, 0);
+    std::vector<size_t> cur((size_t)T * S);
+    {
+      size_t o = 0;
+      for (uint32_t sh = 0; sh < S; sh++) {
+        shard_off[sh] = o;
+        for (int k = 0; k < T; k++) {
+          cur[(size_t)k * S + sh] = o;
+          o += hist[(size_t)k * S + sh];
+        }
       }
+      shard_off[S] = o;
     }
-    shard_off[S] = o;
+    run(T, [&](int k) {
+      const size_t b = k * step, e = std::min(n, b + step);
+      size_t *ck = cur.data() + (size_t)k * S;
+      for (size_t r = b; r < e; r++)
+        if (rep[r] == r) rows[ck[Dictionary::shard_of(hashes[r])]++] = (uint32_t)r;
+    });
+    run(T > 1 ? (int)S : 1, [&](int k) {
+      const uint32_t lo = T > 1 ? (uint32_t)k : 0u, hi = T > 1 ? (uint32_t)k + 1 : S;
+      for (uint32_t sh = lo; sh < hi; sh++)
+        if (shard_off[sh + 1] > shard_off[sh])
+          d.code_shard(sh, rows.data() + shard_off[sh], shard_off[sh + 1] - shard_off[sh],
+                       get, hashes.data(), codes);
+    });
   }
-  run(T, [&](int k) {
-    const size_t b = k * step, e = std::min(n, b + step);
-    size_t *ck = cur.data() + (size_t)k * S;
-    for (size_t r = b; r < e; r++)
-      if (rep[r] == r) rows[ck[Dictionary::shard_of(hashes[r])]++] = (uint32_t)r;
-  });
-  run(T > 1 ? (int)S : 1, [&](int k) {
-    const uint32_t lo = T > 1 ? (uint32_t)k : 0u, hi = T > 1 ? (uint32_t)k + 1 : S;
-    for (uint32_t sh = lo; sh < hi; sh++)
-      if (shard_off[sh + 1] > shard_off[sh])
-        d.code_shard(sh, rows.data() + shard_off[sh], shard_off[sh + 1] - shard_off[sh], get,
-                     hashes.data(), codes);
-  });
   run(T, [&](int k) {  // a representative precedes its copies within the chunk
     const size_t b = k * step, e = std::min(n, b + step);
     for (size_t r = b; r < e; r++)
@@ -2102,6 +2149,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
   t->cols.resize(v->n_cols);
   auto src_row = [&](int64_t i) { return rows ? rows[i] : i; };
   const int nc = v->n_cols;
+  constexpr int64_t kAhead = 16;  // rows prefetched ahead in the gathers below
   // cell widths: longest cell + NUL, rounded up to 16.  The lengths are
   // kept for the packing pass (up to 2^28 cells; beyond, measured again).
   std::vector<size_t> maxlen(nc, 0);
@@ -2114,6 +2162,8 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
       const int64_t stride = v->col_stride[c];
       uint32_t *lc = lens.empty() ? nullptr : lens.data() + (size_t)c * n_rows;
       for (int64_t i = b; i < e; i++) {
+        // rows of a partition are scattered over the source: fetch ahead
+        if (i + kAhead < e) __builtin_prefetch(v->col_base[c] + src_row(i + kAhead) * stride);
         const size_t len = strnlen(v->col_base[c] + src_row(i) * stride, (size_t)stride);
         if (lc) lc[i] = (uint32_t)len;
         ml[c] = std::max(ml[c], len);
@@ -2138,6 +2188,7 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
       const int64_t stride = v->col_stride[c];
       const uint32_t *lc = lens.empty() ? nullptr : lens.data() + (size_t)c * n_rows;
       for (int64_t i = b; i < e; i++) {
+        if (i + kAhead < e) __builtin_prefetch(v->col_base[c] + src_row(i + kAhead) * stride);
         const char *s = v->col_base[c] + src_row(i) * stride;
         const size_t len = lc ? lc[i] : strnlen(s, (size_t)stride);
         char *dst = col.h_bytes + (size_t)i * col.width;
