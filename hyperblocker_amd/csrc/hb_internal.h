@@ -96,8 +96,15 @@ struct JRDesc {
   uint32_t sim_off;      // first SimDesc of this JR
   uint32_t multi;        // the job has several rules: its final pairs are deduplicated
   uint32_t refine;       // the refine pass tests the first similarity predicate
-  uint32_t pad[3];
+  uint32_t pad;
+  // Allowance table of (2*c0, 2*c3), or null: entry (lb, la) at
+  // allow_tab[2 * (lb * kAllowCols + la)] holds the negated allowances
+  // (class 0, class 3) for an A row of length la and a B row of length lb
+  // (lengths 1..255; column 0 for an empty or over-long cell, column 256
+  // for a padding A row).
+  const uint32_t *allow_tab;
 };
+constexpr uint32_t kAllowCols = 257, kAllowRows = 256;
 
 struct SimDesc {
   DevColumn col_l, col_r;
@@ -205,6 +212,9 @@ void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long long v,
                      hipStream_t stream);
 
+// Fills an allowance table (JRDesc::allow_tab) for coefficients c0, c3.
+void launch_allow_table(float c0, float c3, uint32_t *tab, hipStream_t stream);
+
 // Join order of a table for one rule shape: keys (code << 8) | min(length,
 // 255) (code or length may be absent: null pointer), sorted with the row ids;
 // then the runs of equal code (key >> 8): flags, exclusive scan, run arrays.
@@ -224,10 +234,14 @@ void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
 // Groups of every JR of a batch (one per left run), with their work.
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream);
 
-// Stage 0 over groups [g_begin, g_end) (JRs with, or JRs without, a
-// similarity predicate): bound filter of the first similarity predicate (or
-// every pair); kept pairs are appended to `out` tagged with their JR.
-void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, bool has_sim,
+// Filter variants: JRs without a similarity predicate; with one and an
+// allowance table; with one and allowances computed in the kernel.
+enum : int { kFilterEq = 0, kFilterSimTable = 1, kFilterSim = 2 };
+
+// Stage 0 over groups [g_begin, g_end) (JRs of one filter variant): bound
+// filter of the first similarity predicate (or every pair); kept pairs are
+// appended to `out` tagged with their JR.
+void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, int variant,
                    PairList out, FilterShape shape, int n_cu, hipStream_t stream);
 
 // Second bound on the filter's survivors (`in`): the filter's JW bound for

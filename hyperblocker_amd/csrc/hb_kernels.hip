@@ -597,6 +597,24 @@ __device__ __forceinline__ void allowances(uint32_t la, float rla, uint32_t lb, 
   nal3 = neg_allowance(tmax, s, k3 * inv, k3);
 }
 
+// The allowances of every (A length, B length) pair for one threshold, so
+// that the filter reads them per run instead of computing them.  Entry
+// (lb, la) is what `allowances` gives for signatures of those lengths (the
+// reciprocal formed as in k_build_sig), so the table changes no decision.
+__global__ __launch_bounds__(256) void k_allow_table(float c0, float c3, uint32_t *__restrict__ tab) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= kAllowRows * kAllowCols) return;
+  const uint32_t lb = i / kAllowCols, la = i % kAllowCols;
+  uint32_t n0 = kRejectAll, n3 = kRejectAll;
+  if (la < 256) {
+    const float rla = (la >= 1u && la <= 255u) ? 1.0f / (float)la : INFINITY;
+    const float rlb = (lb >= 1u && lb <= 255u) ? 1.0f / (float)lb : INFINITY;
+    allowances(la, rla, lb, rlb, 2.0f * c0, 2.0f * c3, n0, n3);
+  }
+  tab[2 * i] = n0;
+  tab[2 * i + 1] = n3;
+}
+
 // The accumulator chain: seed + sum of |a_i - b_i| over the 32 counters.
 __device__ __forceinline__ int32_t sad_chain(const uint32_t (&ah)[8], const uint32_t (&bh)[8],
                                              uint32_t seed) {
@@ -635,7 +653,7 @@ struct __attribute__((aligned(8))) BRow {
   uint32_t len;
 };
 
-template <bool SIM, int AR, int BU>
+template <bool SIM, bool TAB, int AR, int BU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 5 : (AR == 2 ? 7 : 8)))) void
 k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_t g_end,
                PairList out) {
@@ -665,6 +683,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     const uint32_t jr = wave_u32(g.jr[gi]);
     const JRDesc &d = bd.jrs[jr];
     const float k0 = wave_f32(2.0f * d.c0), k3 = wave_f32(2.0f * d.c3);
+    const uint32_t *atab = TAB ? wave_ptr(d.allow_tab) : nullptr;
     const bool dense_item = !SIM || wave_u32(d.refine) != 0;
     const uint32_t l_off = wave_u32(g.l_off[gi]), l_cnt = wave_u32(g.l_cnt[gi]);
     const uint32_t r_off = wave_u32(g.r_off[gi]), r_cnt = wave_u32(g.r_cnt[gi]);
@@ -705,8 +724,12 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
 #pragma unroll
         for (int k = 0; k < 8; k++) ah[r][k] = s.h[k];
         apre[r] = __builtin_bswap32(s.pre);
-        la[r] = s.len;
+        la[r] = s.len & 0xffffu;
         rla[r] = s.rlen;
+      }
+      if (TAB) {  // bits 16 and up: byte offset of the row's allowance-table column
+        const uint32_t col = !a_valid[r] ? 256u : (rla[r] < INFINITY ? la[r] : 0u);
+        la[r] |= (col * 8u) << 16;
       }
     }
     if (SIM) {
@@ -820,6 +843,26 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
       while (run < n_b) {  // wave-uniform
         uint32_t run_end = n_b;
         if (SIM) {
+          if (wave_active) {
+            const uint32_t blen = wave_u32(s_b[run].len);
+            if (TAB) {
+              const uint32_t bcol = blen - 1u < 255u ? blen : 0u;
+              const char *row = (const char *)(atab + 2u * kAllowCols * bcol);
+#pragma unroll
+              for (int r = 0; r < AR; r++) {
+                const uint2 e = *(const uint2 *)(row + (la[r] >> 16));
+                nal0[r] = e.x;
+                nal3[r] = e.y;
+              }
+            } else {
+              const float rlb = s_rlen[run];
+#pragma unroll
+              for (int r = 0; r < AR; r++) {
+                allowances(la[r], rla[r], blen, rlb, k0, k3, nal0[r], nal3[r]);
+                if (!a_valid[r]) nal0[r] = nal3[r] = kRejectAll;
+              }
+            }
+          }
           // next set bit of s_starts after `run`
           for (uint32_t w = (run + 1) >> 6; w < kChunk / 64; w++) {
             unsigned long long m = wave_u64(s_starts[w]);
@@ -827,15 +870,6 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
             if (m) {
               run_end = min(n_b, (w << 6) + (uint32_t)__builtin_ctzll(m));
               break;
-            }
-          }
-          if (wave_active) {
-            const uint32_t blen = s_b[run].len;
-            const float rlb = s_rlen[run];
-#pragma unroll
-            for (int r = 0; r < AR; r++) {
-              allowances(la[r], rla[r], blen, rlb, k0, k3, nal0[r], nal3[r]);
-              if (!a_valid[r]) nal0[r] = nal3[r] = kRejectAll;
             }
           }
         }
@@ -1651,6 +1685,11 @@ void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, u
                      keys);
 }
 
+void launch_allow_table(float c0, float c3, uint32_t *tab, hipStream_t stream) {
+  hipLaunchKernelGGL(k_allow_table, dim3(grid_for(kAllowRows * kAllowCols, 256)), dim3(256), 0,
+                     stream, c0, c3, tab);
+}
+
 void launch_layout_keys_fb(const uint32_t *code, const SigRow *sig, const uint16_t *len,
                            uint32_t n, uint64_t *keys, hipStream_t stream) {
   if (!n) return;
@@ -1685,7 +1724,7 @@ void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream) {
                      b, g);
 }
 
-void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, bool has_sim,
+void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, int variant,
                    PairList out, FilterShape shape, int n_cu, hipStream_t stream) {
   if (g_end <= g_begin) return;
   // The tile grid launches grid_per_cu work-groups per CU (16 by default,
@@ -1693,24 +1732,26 @@ void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, 
   // grid-stride loop; surplus groups find no item and exit.
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
   const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 8;
-#define HB_FILTER(SIM)                                                                       \
+#define HB_FILTER(SIM, TAB)                                                                  \
   do {                                                                                       \
     if (shape.rows_per_lane == 4)                                                            \
-      hipLaunchKernelGGL((k_filter_tiles<SIM, 4, 1>), dim3(grid), dim3(256), 0, stream, b, g, \
-                         g_begin, g_end, out);                                               \
+      hipLaunchKernelGGL((k_filter_tiles<SIM, TAB, 4, 1>), dim3(grid), dim3(256), 0, stream, \
+                         b, g, g_begin, g_end, out);                                         \
     else if (shape.rows_per_lane == 2)                                                       \
-      hipLaunchKernelGGL((k_filter_tiles<SIM, 2, 2>), dim3(grid), dim3(256), 0, stream, b, g, \
-                         g_begin, g_end, out);                                               \
+      hipLaunchKernelGGL((k_filter_tiles<SIM, TAB, 2, 2>), dim3(grid), dim3(256), 0, stream, \
+                         b, g, g_begin, g_end, out);                                         \
     else                                                                                     \
-      hipLaunchKernelGGL((k_filter_tiles<SIM, 1, 2>), dim3(grid), dim3(256), 0, stream, b, g, \
-                         g_begin, g_end, out);                                               \
+      hipLaunchKernelGGL((k_filter_tiles<SIM, TAB, 1, 2>), dim3(grid), dim3(256), 0, stream, \
+                         b, g, g_begin, g_end, out);                                         \
     hipLaunchKernelGGL((k_filter_small<SIM>), dim3(grid_small), dim3(256), 0, stream, b, g,   \
                        g_begin, g_end, out);                                                 \
   } while (0)
-  if (has_sim)
-    HB_FILTER(true);
+  if (variant == kFilterSimTable)
+    HB_FILTER(true, true);
+  else if (variant == kFilterSim)
+    HB_FILTER(true, false);
   else
-    HB_FILTER(false);
+    HB_FILTER(false, false);
 #undef HB_FILTER
 }
 

@@ -593,6 +593,11 @@ struct hb_ctx {
   };
   std::mutex grave_mu;
   std::deque<Grave> graves;
+  // Filter allowance tables by (c0, c3) bit patterns, kept for the context's
+  // lifetime; at most kMaxAllowTables (JRs of further thresholds compute
+  // their allowances in the kernel).
+  std::mutex allow_mu;
+  std::unordered_map<uint64_t, uint32_t *> allow_tabs;
 };
 
 namespace {
@@ -1216,6 +1221,39 @@ float filter_coef(double thr, int p) {
   return cf;
 }
 
+constexpr size_t kMaxAllowTables = 64;
+
+// The filter's allowance table for bound coefficients (c0, c3), 514 KiB:
+// built once per context on the build stream, which the host waits for.
+// Null once the context holds kMaxAllowTables; the filter then computes
+// those JRs' allowances itself.
+hb_status allow_table(hb_ctx *ctx, float c0, float c3, const uint32_t **out) {
+  uint32_t b0, b3;
+  std::memcpy(&b0, &c0, sizeof(b0));
+  std::memcpy(&b3, &c3, sizeof(b3));
+  const uint64_t key = ((uint64_t)b0 << 32) | b3;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->allow_mu);
+  auto it = ctx->allow_tabs.find(key);
+  if (it != ctx->allow_tabs.end()) {
+    *out = it->second;
+    return HB_OK;
+  }
+  if (ctx->allow_tabs.size() >= kMaxAllowTables) return HB_OK;
+  uint32_t *tab = nullptr;
+  HIP_OK(hipMalloc(&tab, sizeof(uint32_t) * 2 * hb::kAllowRows * hb::kAllowCols));
+  hb::launch_allow_table(c0, c3, tab, ctx->build_stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->build_stream);
+  if (e != hipSuccess) {
+    (void)hipFree(tab);
+    return fail(HB_ERR_HIP, "allowance table: %s", hipGetErrorString(e));
+  }
+  ctx->allow_tabs.emplace(key, tab);
+  *out = tab;
+  return HB_OK;
+}
+
 bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
   return (rows_per_lane == 1 || rows_per_lane == 2 || rows_per_lane == 4) && tile_b >= 64 &&
          tile_b <= 65536 && tile_b % 64 == 0 && groups_per_cu >= 1 && groups_per_cu <= 64;
@@ -1224,10 +1262,6 @@ bool valid_filter_shape(int rows_per_lane, int tile_b, int groups_per_cu) {
 // First similarity thresholds below this get the refine pass: there the
 // 32-class filter bound keeps many pairs whose JW is far below threshold.
 constexpr float kRefineBelow = 0.85f;
-
-// Filter variant of a JR: the kernel template it needs (with or without a
-// similarity predicate; every equality is part of the join key).
-int filter_variant(const PlanRule &rule) { return rule.sims.empty() ? 0 : 1; }
 
 unsigned long long pow2_at_least(unsigned long long x) {
   unsigned long long p = 1024;
@@ -1242,13 +1276,19 @@ hb_status enqueue_batch(Batch *b) {
   HostTrace trace("enqueue_batch");
 
   // Tables ready, lazy structures and layouts; job-rule descriptors.
+  // The filter variant of a JR is the kernel template it needs: with or
+  // without a similarity predicate (every equality is part of the join
+  // key), and with or without an allowance table.
   struct JR {
     uint32_t job;
     int rule;
     int variant;
     Layout *ly[2];
+    const uint32_t *tab;
   };
   std::vector<JR> jrs;
+  float memo_thr = -1.0f;  // threshold of memo_tab
+  const uint32_t *memo_tab = nullptr;
   for (uint32_t j = 0; j < n_jobs; j++) {
     Batch::Item &it = b->items[j];
     HB_TRY(wait_ready(it.L, st));
@@ -1260,7 +1300,16 @@ hb_status enqueue_batch(Batch *b) {
         HB_TRY(ensure_sig(it.R, s.cr, st));
       }
       trace.mark("sig");
-      JR jr{j, k, filter_variant(rule), {nullptr, nullptr}};
+      JR jr{j, k, hb::kFilterEq, {nullptr, nullptr}, nullptr};
+      if (!rule.sims.empty()) {
+        const float thr = rule.sims[0].thr;
+        if (thr != memo_thr) {
+          HB_TRY(allow_table(ctx, filter_coef(thr, 0), filter_coef(thr, 3), &memo_tab));
+          memo_thr = thr;
+        }
+        jr.tab = memo_tab;
+        jr.variant = jr.tab ? hb::kFilterSimTable : hb::kFilterSim;
+      }
       std::vector<int> el, er;
       for (const auto &e : rule.eq) {
         el.push_back(e.first);
@@ -1307,6 +1356,7 @@ hb_status enqueue_batch(Batch *b) {
       d.c1 = filter_coef(thr, 1);
       d.c2 = filter_coef(thr, 2);
       d.c3 = filter_coef(thr, 3);
+      d.allow_tab = jr.tab;
     }
     d.job = jr.job;
     d.n_sims = (int32_t)rule.sims.size();
@@ -1489,7 +1539,7 @@ hb_status enqueue_batch(Batch *b) {
     uint32_t e = q;
     while (e < n_jr && jrs[e].variant == jrs[q].variant) e++;
     const int variant = jrs[q].variant;
-    hb::launch_filter(bd, g, jr_group_off[q], jr_group_off[e], variant != 0, list0, b->shape,
+    hb::launch_filter(bd, g, jr_group_off[q], jr_group_off[e], variant, list0, b->shape,
                       ctx->n_cu, st);
     q = e;
   }
@@ -2115,6 +2165,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
     ctx->jobs.clear();  // each batch returns its memory to the pools with its last job
   }
   (void)reap_graves(ctx, true);
+  for (auto &kv : ctx->allow_tabs) (void)hipFree(kv.second);
   ctx->pool.release_all();
   ctx->hpool.release_all();
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);

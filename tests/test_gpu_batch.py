@@ -263,3 +263,50 @@ def test_release_before_completion(ctx):
         tl.release()
         tr.release()
         dp.release()
+
+
+def test_allowance_tables_and_computed_allowances_agree():
+    """The tile filter reads its length allowances from a table per first
+    threshold, built once per context; a context holding 64 tables filters
+    further thresholds with allowances computed in the kernel.  The same
+    jobs in a context that has filled its tables (computed allowances, beside
+    a job of a tabled threshold in the same batch) and in a fresh context
+    (tables) keep the same number of candidates in the filter and give the
+    oracle's pairs and audit entries."""
+    L, R = _adversarial_tables(91, 700, 600, b"abcdefgh", 40, n_keys=1)
+    thr_new = 0.77
+    plan_new = hb.HostPlan.from_yaml([datagen.rule_yaml([], [(1, thr_new)])])
+    plan_old = hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(2, 0.5)])])
+    want_new = H.run_oracle(plan_new, L, R)
+    want_old = H.run_oracle(plan_old, L, R)
+    tiny = H.columns_from_rows([(b"1", b"ab", b"ab", b"0", b"0")])
+    full, fresh = hb.Context(0), hb.Context(0)
+    try:
+        got = {}
+        for name, c in (("full", full), ("fresh", fresh)):
+            if name == "full":
+                tl, tr = c.upload(hb.ColumnTable(tiny)), c.upload(hb.ColumnTable(tiny))
+                for i in range(64):  # 0.5 first, as plan_old's threshold
+                    p = c.plan(hb.HostPlan.from_yaml([datagen.rule_yaml([], [(1, 0.5 + 0.004 * i)])]))
+                    _results(c.submit(p, tl, tr))
+                    p.release()
+                tl.release()
+                tr.release()
+            tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
+            pn, po = c.plan(plan_new), c.plan(plan_old)
+            got[name] = [_results(j) for j in c.submit_batch([(pn, tl, tr), (po, tl, tr)])]
+            for p in (pn, po):
+                p.release()
+            tl.release()
+            tr.release()
+        for name in got:
+            for (pairs, audit, stats), want, what in zip(got[name], (want_new, want_old),
+                                                         ("thr 0.77", "thr 0.5")):
+                H.assert_same_pairs(pairs, want[0], "%s context, %s" % (name, what))
+                H.assert_same_audit(audit, want[1], "%s context, %s" % (name, what))
+        for k in range(2):
+            assert got["full"][k][2]["filtered"] == got["fresh"][k][2]["filtered"]
+        assert got["fresh"][0][2]["filtered"] < got["fresh"][0][2]["candidates"]
+    finally:
+        full.close()
+        fresh.close()
