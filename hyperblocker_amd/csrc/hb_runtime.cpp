@@ -573,7 +573,7 @@ struct Batch;
 struct hb_ctx {
   int device = 0;
   int n_cu = 256;
-  hb::FilterShape shape{2, 256, 16};  // filter launch shape; HB_FILTER_SHAPE overrides
+  hb::FilterShape shape{2, 192, 16};  // filter launch shape; HB_FILTER_SHAPE overrides
   hipStream_t copy_stream = nullptr;
   hipStream_t build_stream = nullptr;  // one-off per-table builds that the host waits for
   std::vector<hipStream_t> streams;
@@ -822,7 +822,7 @@ struct Batch {
   };
   std::vector<Item> items;
   std::vector<uint64_t> job_ids;  // id of each job of the batch
-  hb::FilterShape shape{2, 256, 16};
+  hb::FilterShape shape{2, 192, 16};
   int max_stage = 1;             // exact stages launched (>= 1)
   // capacities (entries)
   unsigned long long list_cap = 0, defer_cap = 0, final_cap = 0, audit_cap = 0;
@@ -2746,7 +2746,7 @@ hb_status hb_autotune_filter_shape(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *
     }
   } else {
     for (int rpl : {1, 2, 4})
-      for (int tb : {128, 256, 512})
+      for (int tb : {128, 192, 256, 512})
         for (int gpc : {16, 32})
           cands.push_back(
               hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc, ctx->shape.small_pairs_max});

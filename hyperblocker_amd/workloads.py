@@ -115,8 +115,8 @@ def run_c4(ctx, n_jobs=4096, reps=2, inflight=4, autotune=True, log=print):
         if autotune:
             t = time.perf_counter()
             sample = jobs[:: max(1, len(jobs) // 512)]
-            cands = [(1, 256, 16), (2, 128, 32), (2, 256, 16), (2, 256, 32), (2, 512, 16), (4, 256, 12),
-                     (4, 512, 8)]
+            cands = [(1, 256, 16), (2, 128, 32), (2, 192, 16), (2, 192, 32), (2, 256, 16), (2, 256, 32),
+                     (2, 512, 16), (4, 256, 12), (4, 512, 8)]
             tuned = ctx.autotune(sample, cands)
             log("c4: autotuned shape %s (filter %.3f ms on %d jobs) in %.1f s"
                 % (tuned[0], tuned[1], len(sample), time.perf_counter() - t))

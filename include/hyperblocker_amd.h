@@ -281,7 +281,7 @@ hb_status hb_server_destroy(hb_server *s);
  * build their per-table structures, then twice per candidate shape, and
  * sets the shape with the smallest filter-stage time as the context's.
  * candidates holds n_candidates triples (rows per lane, tile_b, groups per
- * CU); NULL tries 18 built-in shapes.  Writes the chosen triple and its
+ * CU); NULL tries 24 built-in shapes.  Writes the chosen triple and its
  * time.  No counterpart in the reference (its launch is fixed). */
 hb_status hb_autotune_filter_shape(hb_ctx *ctx, uint32_t n_jobs, const hb_plan *const *plans,
                                    const hb_table *const *l, const hb_table *const *r,
