@@ -76,6 +76,29 @@ def cpu_baseline(cfg, L, R, plan, budget_rows):
                       % (step, (cfg["n_l"] + step - 1) // step, cfg["n_r"], cnt["pairs"], dt)}
 
 
+def end_to_end_cli(L, R, rules):
+    """The drop-in CLI (run_hyperblocker_exec) on the same tables written as
+    CSV files with the rule as YAML, one partition: the whole run in a fresh
+    process (CSV load, plan, partitioning, context set-up, upload, blocking,
+    output writing), with the CLI's own phase times."""
+    import subprocess
+    import tempfile
+    from hyperblocker_amd import datagen
+    exe = os.path.join(ROOT, "hyperblocker_amd", "bin", "run_hyperblocker_exec")
+    with tempfile.TemporaryDirectory() as d:
+        datagen.write_dataset(d, L, R, rules)
+        t = time.perf_counter()
+        subprocess.run([exe, "-data_l", d + "/tableA.csv", "-data_r", d + "/tableB.csv",
+                        "-rule_dir", d + "/rules", "-o", d + "/out.csv", "-n_partitions", "1",
+                        "-metrics", d + "/m.json"], check=True, stdout=subprocess.DEVNULL,
+                       timeout=300)
+        process_s = time.perf_counter() - t
+        m = json.load(open(d + "/m.json"))
+    return {"pairs_per_s": m["pairs"] / m["total_s"], "total_s": m["total_s"],
+            "load_s": m["load_s"], "partition_s": m["partition_s"],
+            "blocking_s": m["blocking_s"], "process_s": process_s, "matches": m["matches"]}
+
+
 def copy_calibration(n_bytes):
     """Ten device-to-device copies of n_bytes: under rocprofv3 --pmc, the
     FETCH_SIZE / WRITE_SIZE this copy kernel reports per dispatch against its
@@ -141,6 +164,8 @@ def main():
                          "batch streams)")
     ap.add_argument("--cpu-rows", type=int, default=20000,
                     help="left rows in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-cli", action="store_true",
+                    help="skip the end-to-end run of the CLI on CSV files")
     ap.add_argument("--jobs", type=int, default=0,
                     help="jobs of the c4/c5 job-set workloads (0 = their default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -233,6 +258,7 @@ def main():
     cpu = None
     if args.cpu_rows > 0:
         cpu = cpu_baseline(cfg, L, R, plan, args.cpu_rows)
+    e2e = None if args.no_cli else end_to_end_cli(L, R, cfg["rules"])
     out = {
         "metric": "rule-evaluated pairs/sec on 1 MI355X",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
@@ -255,6 +281,8 @@ def main():
         "job_alone_device_ms": float(np.mean([s["device_ms"] for s in alone])),
         "filtered": stats[-1]["filtered"], "jw_exact": stats[-1]["jw_exact"],
         "roofline": roofline, "cpu_baseline": cpu,
+        # run_hyperblocker_exec on the same tables as CSV files (fresh process)
+        "end_to_end_cli": e2e,
     }
     print(json.dumps(out))
     dp.release()
