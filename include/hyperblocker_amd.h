@@ -141,7 +141,8 @@ hb_status hb_ctx_synchronize(hb_ctx *ctx);
 /* Launch shape of the bound-filter kernel on this context: rows of the
  * larger side per lane (1, 2 or 4), rows of the other side per work item
  * (a multiple of 64, at most 65536) and work-groups per compute unit
- * (1..64).  Applies to jobs submitted afterwards; results do not depend on
+ * (1..64); (2, 192, 16) by default, or HB_FILTER_SHAPE="r,t,g" at context
+ * creation.  Applies to jobs submitted afterwards; results do not depend on
  * it.  No counterpart in the reference (its launch is fixed at 64 x 64
  * threads, host_producer.h:114-115); used by the autotuner. */
 hb_status hb_ctx_set_filter_shape(hb_ctx *ctx, int32_t rows_per_lane, int32_t tile_b,
