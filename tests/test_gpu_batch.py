@@ -265,14 +265,15 @@ def test_release_before_completion(ctx):
         dp.release()
 
 
-def test_allowance_tables_and_computed_allowances_agree():
+@pytest.mark.parametrize("shape", [(2, 192, 16), (1, 256, 4), (4, 512, 8)])
+def test_allowance_tables_and_computed_allowances_agree(shape):
     """The tile filter reads its length allowances from a table per first
     threshold, built once per context; a context holding 64 tables filters
     further thresholds with allowances computed in the kernel.  The same
     jobs in a context that has filled its tables (computed allowances, beside
     a job of a tabled threshold in the same batch) and in a fresh context
     (tables) keep the same number of candidates in the filter and give the
-    oracle's pairs and audit entries."""
+    oracle's pairs and audit entries, for each number of rows per lane."""
     L, R = _adversarial_tables(91, 700, 600, b"abcdefgh", 40, n_keys=1)
     thr_new = 0.77
     plan_new = hb.HostPlan.from_yaml([datagen.rule_yaml([], [(1, thr_new)])])
@@ -292,6 +293,7 @@ def test_allowance_tables_and_computed_allowances_agree():
                     p.release()
                 tl.release()
                 tr.release()
+            c.filter_shape = shape
             tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
             pn, po = c.plan(plan_new), c.plan(plan_old)
             got[name] = [_results(j) for j in c.submit_batch([(pn, tl, tr), (po, tl, tr)])]
