@@ -5,9 +5,10 @@ inputs, execution-plan encoding and left_id,right_id output, evaluated by
 HIP kernels through the C ABI in include/hyperblocker_amd.h.
 """
 from .blocker import (ArrayPlan, ColumnTable, Context, DevicePlan, DeviceTable, HostPlan, Server,
-                      HostTable, Job, filter_coefs, partition, sort_pairs, write_pairs)
+                      HostTable, Job, SerializedTable, filter_coefs, partition, sort_pairs,
+                      write_pairs)
 from .capi import HBError
 
 __all__ = ["ArrayPlan", "ColumnTable", "Context", "DevicePlan", "DeviceTable", "HostPlan", "Server",
-           "HostTable", "Job", "HBError", "filter_coefs", "partition", "sort_pairs",
+           "HostTable", "Job", "HBError", "SerializedTable", "filter_coefs", "partition", "sort_pairs",
            "write_pairs"]

@@ -139,6 +139,37 @@ class ColumnTable:
         return bytes(self.columns[col][row])
 
 
+class SerializedTable:
+    """A table in the reference's row-major layout (SerializedTable,
+    core/data_structures/table.h:16-51, as Serialize2PinnedMemory fills it,
+    :75-128): col_size[c] = longest cell + 1, col_offset[c] = sum of the
+    earlier col_size (col_offset[0] = 0), aligned_tuple_size =
+    (((sum col_size + 1) >> 6) << 6) + 64, cell (r, c) at
+    data[r * aligned_tuple_size + col_offset[c]], zero padded.  Built from
+    numpy bytes ('S') columns; `data` is a uint8 array."""
+
+    def __init__(self, columns):
+        cols = [np.ascontiguousarray(c) for c in columns]
+        rows = {len(c) for c in cols}
+        if len(rows) != 1:
+            raise ValueError("ragged columns")
+        self.n_rows = rows.pop()
+        self.n_cols = len(cols)
+        self.col_size = [int(np.char.str_len(c).max()) + 1 if self.n_rows else 1 for c in cols]
+        self.col_offset = [int(x) for x in np.concatenate([[0], np.cumsum(self.col_size)[:-1]])]
+        self.aligned_tuple_size = (((sum(self.col_size) + 1) >> 6) << 6) + 64
+        ats = self.aligned_tuple_size
+        self.data = np.zeros(max(self.n_rows, 1) * ats, dtype=np.uint8)
+        rowview = self.data[: self.n_rows * ats].reshape(self.n_rows, ats)
+        for c, off, size in zip(cols, self.col_offset, self.col_size):
+            cells = np.frombuffer(c.astype("S%d" % size).tobytes(), dtype=np.uint8)
+            rowview[:, off:off + size] = cells.reshape(self.n_rows, size)
+        self._size = (C.c_size_t * self.n_cols)(*self.col_size)
+        self._offset = (C.c_size_t * self.n_cols)(*self.col_offset)
+        self.view = capi.SerializedTable(self.n_rows, self.n_cols, ats, self._size,
+                                         self._offset, self.data.ctypes.data)
+
+
 def partition(l_table, r_table, plan, n_partitions, prefix_idx=capi.CHECK_POINT):
     """DataPartitioning (core/components/data_mngr.h:81-222) of two HostTables
     or two ColumnTables.  Returns a list of (l_rows, r_rows) int64 arrays,
@@ -227,6 +258,13 @@ class Context:
 
     def upload(self, table, rows=None):
         return DeviceTable(self, table, rows)
+
+    def upload_serialized(self, table):
+        """hb_table_upload_serialized of a SerializedTable (the reference's
+        row-major handoff)."""
+        h = C.c_void_p()
+        check(lib().hb_table_upload_serialized(self._h, C.byref(table.view), C.byref(h)))
+        return DeviceTable(self, table, _handle=h, _n_rows=table.n_rows)
 
     def plan(self, host_plan):
         return DevicePlan(self, host_plan)

@@ -34,6 +34,8 @@
 
 #include "../../include/hyperblocker_amd.h"
 #include "hb_internal.h"
+#define HB_ALLOW_FN __device__ __forceinline__
+#include "hb_allow.h"
 
 #pragma clang fp contract(off)
 
@@ -553,48 +555,17 @@ __device__ __forceinline__ uint32_t find_group(const uint64_t *off, uint32_t n, 
 // Signatures arrive in join order (the JR's layouts): element i of jsig
 // belongs to row perm[i], so the rows of a join group are contiguous.
 
-// The bound test of may_pass in allowance form, for a lane-held A row and a
-// B row shared by the wave.  may_pass rejects when
-//   t * (1/la + 1/lb) < 2 * coef,   t = la + lb - SAD = 2 * sum of min counts
-// (m_ub * (S + T) < coef * S * T divided by S * T / 2), evaluated in float as
-// (float)t * (rla + rlb) < k.  For fixed lengths the left side does not
-// decrease with t, so the rejected t are exactly those below
-//   T* = min { t : !((float)t * (rla + rlb) < k) },
-// i.e. the pair is rejected iff SAD > allow = la + lb - T*.  neg_allowance
-// returns -allow (two's complement) found by that same float comparison, so
-// decisions equal the direct form bit for bit; seeding the SAD chain with it
-// leaves SAD - allow, positive iff rejected.  An empty or over-long cell has
-// rlen = +inf and is never rejected, nor is any pair when k <= 0.
-constexpr int32_t kPassAllow = 1 << 30;
-// Negated allowance of a padding A row: the accumulator stays > 0.
-constexpr uint32_t kRejectAll = 1u << 30;
-
-__device__ __forceinline__ uint32_t neg_allowance(int32_t tmax, float s, float q, float k) {
-  // t * s crosses k within 1 of ceil(q), q ~ k / s: |k / s| <= 511 and q and
-  // every float operation are off by at most 2^-22 relative, so one step
-  // either way lands on T* (clamped to [1, tmax + 1]; T* >= 1 as k > 0).
-  if (!(k > 0.0f)) return (uint32_t)(-kPassAllow);
-  int32_t t = (int32_t)ceilf(q);
-  t = max(1, min(t, tmax + 1));
-  if (!((float)(t - 1) * s < k))
-    t -= 1;
-  else if (t <= tmax && (float)t * s < k)
-    t += 1;
-  return (uint32_t)(t - tmax);
-}
+// The bound test of may_pass in allowance form (hb_allow.h): for a
+// lane-held A row and a B row shared by the wave, the pair is rejected iff
+// SAD > allow(la, lb), found by the same float comparison, so decisions
+// equal the direct form bit for bit.
 
 // Negated allowances of one A row for one B length, both prefix classes.
 __device__ __forceinline__ void allowances(uint32_t la, float rla, uint32_t lb, float rlb,
                                            float k0, float k3, uint32_t &nal0, uint32_t &nal3) {
   const float s = rla + rlb;
-  if (!(s < INFINITY)) {  // an empty or over-long cell: never rejected
-    nal0 = nal3 = (uint32_t)(-kPassAllow);
-    return;
-  }
-  const float inv = __builtin_amdgcn_rcpf(s);
-  const int32_t tmax = (int32_t)((la & 0xffffu) + (lb & 0xffffu));
-  nal0 = neg_allowance(tmax, s, k0 * inv, k0);
-  nal3 = neg_allowance(tmax, s, k3 * inv, k3);
+  const float inv = s < INFINITY ? __builtin_amdgcn_rcpf(s) : 0.0f;
+  allowances_inv(la, rla, lb, rlb, k0, k3, inv, nal0, nal3);
 }
 
 // The allowances of every (A length, B length) pair for one threshold, so
@@ -885,11 +856,16 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
           // between steps: a step adds at most 256 * AR * BU <= kTileOut / 2
           // entries, so it never overflows.  Every wave of the group runs
           // the same (run, jj) sequence and reads the same `dense` (written
-          // only between barriers), so all reach the barrier together.
+          // only between barriers), so all reach the barrier together.  The
+          // count is read between two barriers: a wave that goes on to its
+          // next step appends (changing to.n) only after every wave has
+          // read it, so all waves take the same flush decision.
           auto between = [&](uint32_t j) {
             if (dense && j != run) {
               __syncthreads();
-              if (to.n >= kTileOut / 2) tile_flush(to, tc, out, bd.overflow);
+              const bool flush = to.n >= kTileOut / 2;
+              __syncthreads();
+              if (flush) tile_flush(to, tc, out, bd.overflow);
             }
           };
           if (!dense) {  // only active waves get here

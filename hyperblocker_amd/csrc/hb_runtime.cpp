@@ -598,6 +598,9 @@ struct hb_ctx {
   // their allowances in the kernel).
   std::mutex allow_mu;
   std::unordered_map<uint64_t, uint32_t *> allow_tabs;
+  // Test hook: this many batch launches fail before any device work
+  // (HB_INJECT_LAUNCH_FAILURES at creation), to exercise the error path.
+  std::atomic<int> inject_launch_failures{0};
 };
 
 namespace {
@@ -1838,31 +1841,39 @@ void size_batch(Batch *b) {
                                               std::min(want.audit, (double)(1ull << 24)));
 }
 
-// Runs registered jobs as one batch and assigns it to them.
+// Runs registered jobs as one batch and assigns it to them.  The batch is
+// created first, so that every job gets one: a launch that fails at any
+// step leaves a failed batch whose error each job's wait and results
+// report (and *out is never null).
 hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
                        std::shared_ptr<Batch> *out = nullptr) {
-  HB_TRY(set_device(ctx));
-  HB_TRY(reap_graves(ctx, false));  // memory of released tables and finished layout builds
   auto b = std::make_shared<Batch>();
   b->ctx = ctx;
-  b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
   const uint32_t n = (uint32_t)jobs.size();
   for (uint32_t j = 0; j < n; j++) {
     b->items.push_back(Batch::Item{jobs[j]->plan, jobs[j]->L, jobs[j]->R});
     b->job_ids.push_back(jobs[j]->id);
   }
-  size_batch(b.get());
-  HB_TRY(ctx->events.get(&b->ev_start));
-  HB_TRY(ctx->events.get(&b->ev_end));
-  for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));
+  auto prepare = [&]() -> hb_status {
+    HB_TRY(set_device(ctx));
+    HB_TRY(reap_graves(ctx, false));  // memory of released tables and finished layout builds
+    if (ctx->inject_launch_failures.load() > 0 && ctx->inject_launch_failures.fetch_sub(1) > 0)
+      return fail(HB_ERR_HIP, "launch failure injected (HB_INJECT_LAUNCH_FAILURES)");
+    b->stream = ctx->streams[ctx->next_stream++ % ctx->streams.size()];
+    size_batch(b.get());
+    HB_TRY(ctx->events.get(&b->ev_start));
+    HB_TRY(ctx->events.get(&b->ev_end));
+    for (auto &m : b->marks) HB_TRY(ctx->events.get(&m));
+    return HB_OK;
+  };
+  hb_status st = prepare();
   {
     std::lock_guard<std::mutex> lk(ctx->mu);
     b->shape = ctx->shape;
   }
-  hb_status st;
   {
     std::lock_guard<std::mutex> lk(b->mu);
-    st = enqueue_batch(b.get());
+    if (st == HB_OK) st = enqueue_batch(b.get());
     if (st != HB_OK) {
       b->state = BatchState::kFailed;
       b->error = st;
@@ -1890,7 +1901,7 @@ hb_status launch_batch(hb_ctx *ctx, const std::vector<Job *> &jobs,
     jobs[j]->cv.notify_all();
   }
   if (out) *out = b;
-  return st;
+  return st == HB_OK ? HB_OK : fail(st, "%s", b->error_text.c_str());
 }
 
 hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
@@ -2093,6 +2104,8 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
     if (std::sscanf(env, "%d,%d,%d", &rpl, &tb, &gpc) == 3 && valid_filter_shape(rpl, tb, gpc))
       ctx->shape = hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc};
   }
+  if (const char *env = std::getenv("HB_INJECT_LAUNCH_FAILURES"))
+    ctx->inject_launch_failures.store(std::max(0, std::atoi(env)));
   if (const char *env = std::getenv("HB_SMALL_PAIRS")) {
     const long v = std::atol(env);
     if (v >= 0 && v <= (1l << 24)) ctx->shape.small_pairs_max = (uint32_t)v;
@@ -2649,8 +2662,10 @@ void server_loop(hb_server *s) {
       std::shared_ptr<Batch> b;
       (void)launch_batch(s->ctx, take, &b);  // a failure is reported through the jobs
       lk.lock();
-      s->mem_in_use += b->mem_estimate;
-      s->inflight.push_back(std::move(b));
+      if (b && b->state != BatchState::kFailed) {
+        s->mem_in_use += b->mem_estimate;
+        s->inflight.push_back(std::move(b));
+      }
       s->batches++;
       s->jobs_run += take.size();
       admitted = true;

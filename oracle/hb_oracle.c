@@ -484,4 +484,41 @@ int64_t hbo_audit(const hbo_plan *plan, const hbo_table *L,
                  sizeof(hbo_audit_entry), NULL);
 }
 
+/* hbo_eval_pair over a list of pairs, on n_threads threads. */
+typedef struct {
+  const hbo_plan *plan;
+  const hbo_table *L, *R;
+  const uint32_t *pairs;
+  int64_t n, next; /* next: atomic chunk cursor */
+  uint8_t *out;
+} eval_job_t;
+
+static void *eval_worker(void *arg) {
+  eval_job_t *job = (eval_job_t *)arg;
+  for (;;) {
+    int64_t b = __atomic_fetch_add(&job->next, 4096, __ATOMIC_RELAXED);
+    if (b >= job->n) break;
+    int64_t e = b + 4096 < job->n ? b + 4096 : job->n;
+    for (int64_t i = b; i < e; i++)
+      job->out[i] = (uint8_t)hbo_eval_pair(job->plan, job->L, job->pairs[2 * i], job->R,
+                                           job->pairs[2 * i + 1], NULL);
+  }
+  return NULL;
+}
+
+int hbo_eval_pairs(const hbo_plan *plan, const hbo_table *L, const hbo_table *R,
+                   const uint32_t *pairs, int64_t n, int n_threads, uint8_t *out) {
+  if (hbo_plan_check(plan)) return -1;
+  for (int64_t i = 0; i < n; i++)
+    if ((int64_t)pairs[2 * i] >= L->n_rows || (int64_t)pairs[2 * i + 1] >= R->n_rows) return -2;
+  eval_job_t job = {plan, L, R, pairs, n, 0, out};
+  if (n_threads < 1) n_threads = 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)n_threads);
+  if (!th) return -3;
+  for (int t = 0; t < n_threads; t++) pthread_create(&th[t], NULL, eval_worker, &job);
+  for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
+  free(th);
+  return 0;
+}
+
 void hbo_free(void *p) { free(p); }

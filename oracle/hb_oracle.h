@@ -118,6 +118,12 @@ int64_t hbo_audit(const hbo_plan *plan, const hbo_table *L,
                   const hbo_table *R, int64_t l_begin, int64_t l_step,
                   int n_threads, hbo_audit_entry **out);
 
+/* hbo_eval_pair for each of n (l, r) pairs (2*n uint32), on n_threads
+ * threads: out[i] = 1 if pair i is emitted.  Returns 0, or a negative error
+ * (malformed plan, row out of range). */
+int hbo_eval_pairs(const hbo_plan *plan, const hbo_table *L, const hbo_table *R,
+                   const uint32_t *pairs, int64_t n, int n_threads, uint8_t *out);
+
 void hbo_free(void *p);
 
 #ifdef __cplusplus

@@ -91,6 +91,10 @@ def lib():
                                 C.c_int64, C.c_int64, C.c_int,
                                 C.POINTER(C.POINTER(_Audit))]
         L.hbo_audit.restype = C.c_int64
+        L.hbo_eval_pairs.argtypes = [C.POINTER(_Plan), C.POINTER(_Table), C.POINTER(_Table),
+                                     C.POINTER(C.c_uint32), C.c_int64, C.c_int,
+                                     C.POINTER(C.c_uint8)]
+        L.hbo_eval_pairs.restype = C.c_int
         L.hbo_free.argtypes = [C.c_void_p]
         L.hbo_free.restype = None
         _lib = L
@@ -191,6 +195,20 @@ def block(plan, L, R, l_begin=0, l_step=1, n_threads=None):
     arr = np.ctypeslib.as_array(out, shape=(max(n, 1) * 2,))[: 2 * n].copy().reshape(n, 2)
     lib().hbo_free(out)
     return arr, cnt.as_dict()
+
+
+def eval_pairs(plan, L, R, pairs, n_threads=None):
+    """hbo_eval_pair of every (l, r) row of `pairs` (uint32 [n, 2]): bool [n]."""
+    if n_threads is None:
+        n_threads = os.cpu_count() or 1
+    p = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint32).reshape(-1, 2))
+    out = np.zeros(len(p), np.uint8)
+    rc = lib().hbo_eval_pairs(C.byref(plan.view), C.byref(L.view), C.byref(R.view),
+                              p.ctypes.data_as(C.POINTER(C.c_uint32)), len(p), n_threads,
+                              out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != 0:
+        raise ValueError("oracle hbo_eval_pairs failed: %d" % rc)
+    return out.astype(bool)
 
 
 def audit(plan, L, R, l_begin=0, l_step=1, n_threads=None):

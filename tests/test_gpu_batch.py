@@ -312,3 +312,67 @@ def test_allowance_tables_and_computed_allowances_agree(shape):
     finally:
         full.close()
         fresh.close()
+
+
+@pytest.mark.parametrize("shape", [(1, 64, 4), (1, 256, 4), (2, 192, 16), (2, 512, 8),
+                                   (4, 256, 8), (4, 1024, 6)])
+def test_dense_flushes_with_partial_tiles(ctx, shape):
+    """The filter's dense path (flushes of the work-group's survivor buffer
+    between steps) for an equality-only rule and a JW >= 0.5 rule on two
+    letters, with join groups of 700 x 600 rows, so that every group ends
+    in a partly filled A tile (some waves idle) while the buffer fills and
+    is flushed many times: pairs and audit entries equal the oracle's at
+    each launch shape."""
+    L, R = _adversarial_tables(131, 1400, 1200, b"ab", 24, n_keys=2)
+    old = ctx.filter_shape
+    ctx.filter_shape = shape
+    plans = [hb.HostPlan.from_yaml([datagen.rule_yaml([3], [])]),
+             hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.5)])]),
+             hb.HostPlan.from_yaml([datagen.rule_yaml([3], []), datagen.rule_yaml([4], [(2, 0.5)])])]
+    dps = [ctx.plan(p) for p in plans]
+    tl, tr = ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))
+    try:
+        jobs = ctx.submit_batch([(dp, tl, tr) for dp in dps])
+        for k, (plan, job) in enumerate(zip(plans, jobs)):
+            got = _results(job)
+            want, want_audit, _ = H.run_oracle(plan, L, R)
+            H.assert_same_pairs(got[0], want, "plan %d, shape %s" % (k, shape))
+            H.assert_same_audit(got[1], want_audit, "plan %d, shape %s" % (k, shape))
+            assert got[2]["filtered"] > 0.3 * got[2]["candidates"]
+    finally:
+        ctx.filter_shape = old
+        tl.release()
+        tr.release()
+        for dp in dps:
+            dp.release()
+
+
+def test_autotuner_picks_a_candidate_and_changes_no_result(ctx):
+    """hb_autotune_filter_shape times the given shapes on a job sample and
+    makes the fastest the context's: the chosen shape is one of the
+    candidates, its time is positive, and jobs run afterwards still equal
+    the oracle."""
+    L, R = _adversarial_tables(141, 2500, 2200, b"abcdefgh", 40, n_keys=3)
+    plan = hb.HostPlan.from_yaml(PLANS["two_eq_two_sim"])
+    plan2 = hb.HostPlan.from_yaml(PLANS["eq_sim"])
+    dp, dp2 = ctx.plan(plan), ctx.plan(plan2)
+    tl, tr = ctx.upload(hb.ColumnTable(L)), ctx.upload(hb.ColumnTable(R))
+    old = ctx.filter_shape
+    cands = [(1, 256, 4), (2, 192, 16), (2, 512, 8), (4, 256, 12)]
+    try:
+        best, ms = ctx.autotune([(dp, tl, tr), (dp2, tl, tr)], cands)
+        assert best in cands and ms > 0
+        assert ctx.filter_shape == best
+        for p, d in ((plan, dp), (plan2, dp2)):
+            got = _results(ctx.submit(d, tl, tr))
+            want, want_audit, _ = H.run_oracle(p, L, R)
+            H.assert_same_pairs(got[0], want, "after autotune %s" % (best,))
+            H.assert_same_audit(got[1], want_audit, "after autotune %s" % (best,))
+        best2, _ = ctx.autotune([(dp2, tl, tr)])  # the built-in candidate list
+        assert ctx.filter_shape == best2
+    finally:
+        ctx.filter_shape = old
+        tl.release()
+        tr.release()
+        dp.release()
+        dp2.release()

@@ -267,3 +267,26 @@ def test_cli_on_reference_dataset(tmp_path):
     Lh, Rh, _ = H.load_fixture("dblp_acm")
     want = sorted(Lh.cell(l, 0) + b"," + Rh.cell(r, 0) for l, r in g["pairs"])
     assert sorted(open(out, "rb").read().splitlines()) == want
+
+
+def test_reference_layout_handoff(ctx):
+    """The reference's row-major SerializedTable (table.h:75-128: col_size =
+    longest cell + 1, aligned_tuple_size = (((sum + 1) >> 6) << 6) + 64,
+    zero padding) of the dblp-acm tables, passed through
+    hb_table_upload_serialized: pairs and audit entries equal the oracle's
+    committed results for that data set."""
+    g = json.load(open(os.path.join(H.GOLDEN, "dblp_acm.json")))
+    Lh, Rh, plan = H.load_fixture("dblp_acm")
+    sl, sr = hb.SerializedTable(H.host_columns(Lh)), hb.SerializedTable(H.host_columns(Rh))
+    assert sl.col_offset[0] == 0 and sl.aligned_tuple_size % 64 == 0
+    tl, tr = ctx.upload_serialized(sl), ctx.upload_serialized(sr)
+    dp = ctx.plan(plan)
+    try:
+        got, audit, stats = ctx.block(dp, tl, tr)
+    finally:
+        dp.release()
+        tl.release()
+        tr.release()
+    assert hb.sort_pairs(got).tolist() == g["pairs"]
+    H.assert_same_audit(audit, H.audit_from_json(g["audit_entries"]), "serialized dblp_acm")
+    assert stats["pairs"] == g["counters"]["pairs"]

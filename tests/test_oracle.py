@@ -323,6 +323,166 @@ def test_block_threads_and_sampling():
     assert want == {tuple(x) for x in p1.tolist() if x[0] % 13 == 0}
 
 
+def test_eval_pairs_agrees_with_block():
+    """The batched per-pair evaluator gives exactly hbo_block's pair set on
+    every pair of a small job."""
+    rng = random.Random(9)
+    rows = lambda n: [(str(i).encode(), bytes(rng.choice(b"abc") for _ in range(rng.randint(0, 9))),
+                       str(rng.randrange(3)).encode()) for i in range(n)]
+    L, R = H.columns_from_rows(rows(90)), H.columns_from_rows(rows(80))
+    plan = O.Plan(2, [2, 2, 1, 1, CP, 1, 1, CP], b"eess ss ", [1, 1, .7, .7, 1, .9, .9, 1])
+    tl, tr = O.Table(L), O.Table(R)
+    pairs, _ = O.block(plan, tl, tr, n_threads=2)
+    ll, rr = np.meshgrid(np.arange(90), np.arange(80), indexing="ij")
+    every = np.stack([ll.ravel(), rr.ravel()], 1).astype(np.uint32)
+    ok = O.eval_pairs(plan, tl, tr, every, n_threads=3)
+    assert {tuple(x) for x in every[ok].tolist()} == {tuple(x) for x in pairs.tolist()}
+
+
+# ---- equality and the audit band, pinned from first principles ----------------
+
+@pytest.mark.parametrize("a,b,want", [
+    (b"1", b"10", False),            # a prefix is not equal
+    (b"10", b"1", False),
+    (b"", b"", True),                # two empty cells
+    (b"", b"0", False),
+    (b"abc", b"abc", True),
+    (b"abc", b"abd", False),
+    (b"\x80\xff", b"\x80\xff", True),  # bytes above 127 compare as bytes
+    (b"\x80\xff", b"\x80\xfe", False),
+    (b"\xe9t\xe9", b"\xc3\xa9t\xc3\xa9", False),
+])
+def test_equal_direct_cases(a, b, want):
+    """equalities_kernel (device_func.cuh:39-52): byte equality up to the
+    first NUL, nothing more."""
+    assert O.equal(a, b) is want
+    assert O.equal(b, a) is want
+
+
+def test_equal_stops_at_nul():
+    """Cells are read up to their first NUL: bytes after it never count."""
+    import ctypes as C
+    a = C.create_string_buffer(b"ab\x00xy", 6)
+    b = C.create_string_buffer(b"ab\x00zw", 6)
+    assert O.lib().hbo_equal(a, b) == 1
+    c = C.create_string_buffer(b"ab\x00", 4)
+    d = C.create_string_buffer(b"abc", 4)
+    assert O.lib().hbo_equal(c, d) == 0
+
+
+def _plan_rules(plan):
+    """The oracle plan's rules as lists of (slot, type, threshold)."""
+    idx = list(plan.pred_index)
+    typ = plan.pred_type.decode("latin-1")
+    thr = list(plan.pred_threshold)
+    rules, cur, i = [], [], 0
+    while i < len(idx):
+        if idx[i] == CP:
+            rules.append(cur)
+            cur = []
+            i += 1
+        else:
+            cur.append((i, idx[i], idx[i + 1], typ[i], thr[i]))
+            i += 2
+    return rules
+
+
+def _audit_brute_force(plan, L, R):
+    """The audit list written out from its definition: for every pair and
+    every rule on its own, predicates in plan order until the first false
+    one (the reference's checkpoint skip, global_func.cuh:152-159); each
+    similarity predicate evaluated there whose JW lies within 1e-6 of its
+    threshold (compared as double, global_func.cuh:139-145) is an entry.
+    Equality is Python byte equality; JW and its fused variant come from the
+    oracle's functions pinned above; bit0 = the fused tail decides the
+    threshold test the other way, bit1 = the shorter cell has >= 257 bytes."""
+    out = []
+    for l in range(len(L[0])):
+        for r in range(len(R[0])):
+            for rule in _plan_rules(plan):
+                for slot, cl, cr, t, thr in rule:
+                    a, b = bytes(L[cl][l]), bytes(R[cr][r])
+                    if t == "e":
+                        alive = a == b
+                    else:
+                        thr_d = float(np.float32(thr))
+                        jw = O.jw(a, b)
+                        if abs(jw - thr_d) < 1e-6:
+                            flags = 1 if (jw < thr_d) != (O.jw_fused(a, b) < thr_d) else 0
+                            if min(len(a), len(b)) >= 257:
+                                flags |= 2
+                            out.append((l, r, slot, flags, jw))
+                        alive = not (jw < thr_d)
+                    if not alive:
+                        break
+    out.sort()
+    arr = np.zeros(len(out), dtype=O.AUDIT_DTYPE)
+    for k, e in enumerate(out):
+        arr[k] = e
+    return arr
+
+
+def _band_tables(seed, n, alphabet, max_len, long_len=0):
+    rng = random.Random(seed)
+
+    def s(m):
+        return bytes(rng.choice(alphabet) for _ in range(rng.randint(0, m)))
+
+    rows_l = [(str(i).encode(), s(max_len), s(max_len // 2 + 1), str(rng.randrange(2)).encode())
+              for i in range(n)]
+    rows_r = []
+    for i in range(n):
+        src = rows_l[rng.randrange(n)]
+        t = bytearray(src[1])
+        if t:
+            t[rng.randrange(len(t))] = rng.choice(alphabet)
+        rows_r.append((str(i).encode(), bytes(t) if rng.random() < 0.6 else s(max_len),
+                       src[2] if rng.random() < 0.5 else s(max_len // 2 + 1),
+                       str(rng.randrange(2)).encode()))
+    if long_len:
+        base = bytes(rng.choice(alphabet) for _ in range(long_len))
+        rows_l[0] = (rows_l[0][0], base, rows_l[0][2], b"0")
+        rows_r[0] = (rows_r[0][0], base[:-3] + b"xyz", rows_r[0][2], b"0")
+        rows_r[1] = (rows_r[1][0], base[5:] + b"q" * 9, rows_r[1][2], b"0")
+    return H.columns_from_rows(rows_l), H.columns_from_rows(rows_r)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_audit_list_brute_force(case):
+    """hbo_audit against the audit definition written out in Python, on
+    small tables whose thresholds are JW values of actual pairs (so the band
+    holds entries), for plans of 1-3 rules mixing equalities and 1-2
+    similarity predicates; one case holds 300-byte cells (flag bit1)."""
+    alphabet = [b"ab", b"abc", b"abcd", bytes(range(1, 256)), b"ab", b"abc"][case]
+    L, R = _band_tables(40 + case, 24, alphabet, 14, long_len=300 if case == 5 else 0)
+    rng = random.Random(case)
+    jws1 = sorted({O.jw(L[1][i], R[1][j]) for i in range(24) for j in range(24)} - {0.0, 1.0})
+    jws2 = sorted({O.jw(L[2][i], R[2][j]) for i in range(24) for j in range(24)} - {0.0, 1.0})
+    t1 = float(np.float32(jws1[rng.randrange(len(jws1))]))
+    t2 = float(np.float32(jws2[rng.randrange(len(jws2))]))
+    t3 = float(np.float32(jws1[len(jws1) // 3]))
+    if case == 5:
+        t1 = float(np.float32(O.jw(L[1][0], R[1][0])))
+        t3 = float(np.float32(O.jw(L[1][0], R[1][1])))
+    plans = [
+        O.Plan(1, [1, 1, CP], b"ss ", [t1, t1, 1]),
+        O.Plan(1, [3, 3, 1, 1, 2, 2, CP], b"eessss ", [1, 1, t1, t1, t2, t2, 1]),
+        O.Plan(3, [3, 3, 2, 2, CP, 1, 1, CP, 2, 2, 1, 1, CP], b"eess ss ssss ",
+               [1, 1, t2, t2, 1, t3, t3, 1, t2, t2, t1, t1, 1]),
+    ]
+    found = 0
+    for plan in plans:
+        assert plan.check() == 0
+        want = _audit_brute_force(plan, L, R)
+        got = O.audit(plan, O.Table(L), O.Table(R), n_threads=2)
+        got = got[np.lexsort((got["slot"], got["r"], got["l"]))]
+        H.assert_same_audit(got, want, "case %d" % case)
+        found += len(want)
+        if case == 5:
+            assert np.any(want["flags"] & 2), "long cells give bit1 entries"
+    assert found > 0
+
+
 # ---- golden fixtures ------------------------------------------------------------
 
 def _golden(name):
