@@ -3,15 +3,20 @@
 
 Workload c2: 100,000 x 100,000 synthetic rows (5 columns), one rule
 year equal AND JW(title) >= 0.89 AND JW(author) >= 0.85, one partition.
-A step = one blocking job over the resident tables: submit, run every
-kernel, hand the matched pairs back to the caller (copied into a numpy
-array).  Steps are issued with up to --inflight jobs outstanding, as the
-partition pipeline issues partitions; each job still does all of its work.
-Inputs are uploaded before the timed region; per-table device structures
-(signatures, join index) are built by the warmup jobs, and the cold first
-job and the latency of a job alone are reported separately.
+The inputs are both tables in the reference's row-major layout
+(SerializedTable), resident in device memory before the timed region, as
+blocking_kernel receives them.  A step = one fresh blocking job: two new
+tables unpacked from those rows (hb_table_from_device), every per-table
+structure built (dictionary codes, signatures, join layouts), every kernel
+of the job, and the matched pairs handed back to the caller (copied into a
+numpy array).  Steps are issued with up to --inflight jobs outstanding, as
+the partition pipeline issues partitions.
 
-Prints one JSON line.
+Also reported: the same steps with the rows handed over from host memory
+(PCIe-inclusive), the repeated-job rate of one pair of warm resident tables,
+per-stage times of jobs alone, the served-vs-plain trace of the c5 workload
+(p50/p99 latency), the CPU oracle on a sample, and the drop-in CLI end to
+end.  Prints one JSON line.
 """
 import argparse
 import collections
@@ -193,6 +198,10 @@ def main():
                     help="left rows in the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-cli", action="store_true",
                     help="skip the end-to-end run of the CLI on CSV files")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served-vs-plain trace (c5) in the default run")
+    ap.add_argument("--serve-jobs", type=int, default=1500,
+                    help="jobs of the served trace in the default run")
     ap.add_argument("--jobs", type=int, default=0,
                     help="jobs of the c4/c5 job-set workloads (0 = their default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -216,56 +225,80 @@ def main():
         return
     L, R, dup, cfg = datagen.make(args.workload)
     plan = hb.HostPlan.from_yaml(cfg["rules"])
+    n_pairs = cfg["n_l"] * cfg["n_r"]
     ctx = hb.Context(0)
     ctx.synchronize()
-    t = time.perf_counter()
-    tl = ctx.upload(hb.ColumnTable(L))
-    tr = ctx.upload(hb.ColumnTable(R))
-    ctx.synchronize()
-    upload_s = time.perf_counter() - t
     dp = ctx.plan(plan)
+    # The inputs: both tables in the reference's row-major layout
+    # (SerializedTable), resident in device memory before timing, as
+    # blocking_kernel receives them.
+    sl, sr = hb.SerializedTable(L), hb.SerializedTable(R)
+    dl, dr = hb.DeviceSerializedTable(sl), hb.DeviceSerializedTable(sr)
 
-    def finish(job):
+    def finish(item):
+        job, tables = item
         pairs, audit, stats = job.results()
         job.release()
+        for t in tables:
+            t.release()
         return pairs, stats
 
-    def one():
-        return finish(ctx.submit(dp, tl, tr))
+    def fresh():
+        """One step: new tables unpacked from the device rows (every
+        per-table structure is built again), the job, its results."""
+        tl, tr = ctx.from_device(dl), ctx.from_device(dr)
+        return ctx.submit(dp, tl, tr), (tl, tr)
+
+    def run_steps(make, steps, inflight):
+        stats, pending, last = [], collections.deque(), None
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pending.append(make())
+            if len(pending) >= max(1, inflight):
+                last, st = finish(pending.popleft())
+                stats.append(st)
+        while pending:
+            last, st = finish(pending.popleft())
+            stats.append(st)
+        ctx.synchronize()
+        return time.perf_counter() - t0, stats, last
 
     t = time.perf_counter()
-    pairs, cold = one()
+    pairs, cold = finish(fresh())
     cold_s = time.perf_counter() - t
-    for _ in range(max(0, args.warmup - 1)):
-        one()
-    ctx.synchronize()
-    stats = []
-    pending = collections.deque()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        pending.append(ctx.submit(dp, tl, tr))
-        if len(pending) >= max(1, args.inflight):
-            pairs, st = finish(pending.popleft())
-            stats.append(st)
-    while pending:
-        pairs, st = finish(pending.popleft())
-        stats.append(st)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    # Jobs alone, one at a time: their latency, and the kernels' durations
-    # undisturbed.  In the timed loop jobs in flight run on different
-    # streams and overlap on the GPU, which stretches each other's kernels,
-    # so the roofline is taken from these runs.
+    run_steps(fresh, max(0, args.warmup - 1), args.inflight)
+    dt, stats, pairs = run_steps(fresh, args.steps, args.inflight)
+    ms_per_step = dt * 1e3 / args.steps
+    value = n_pairs * args.steps / dt
+
+    # PCIe-inclusive: the same steps with the rows handed over from host
+    # memory (hb_table_upload_serialized: one copy in, unpacked on the device).
+    dt_pcie, _, _ = run_steps(lambda: (lambda tl, tr: (ctx.submit(dp, tl, tr), (tl, tr)))(
+        ctx.upload_serialized(sl), ctx.upload_serialized(sr)), args.steps, args.inflight)
+    # Resident and warm: one pair of tables kept across jobs, so per-table
+    # structures are built once (the repeated-job rate of a resident table).
+    tl, tr = ctx.from_device(dl), ctx.from_device(dr)
+    for _ in range(2):
+        finish((ctx.submit(dp, tl, tr), ()))
+    dt_warm, stats_warm, _ = run_steps(lambda: (ctx.submit(dp, tl, tr), ()),
+                                       max(args.steps, 20), args.inflight)
+    # Jobs alone, one at a time, on the warm tables: their latency, and the
+    # kernels' durations undisturbed (jobs in flight overlap on the GPU and
+    # stretch each other's kernels), so the roofline is taken from these.
     lat, alone = [], []
     for _ in range(max(5, args.steps // 2)):
         t = time.perf_counter()
-        _, st_alone = one()
+        _, st_alone = finish((ctx.submit(dp, tl, tr), ()))
         lat.append((time.perf_counter() - t) * 1e3)
         alone.append(st_alone)
+    # A fresh job alone: its latency from the device rows to the results.
+    lat_fresh, fresh_dev = [], []
+    for _ in range(5):
+        t = time.perf_counter()
+        _, st = finish(fresh())
+        lat_fresh.append((time.perf_counter() - t) * 1e3)
+        fresh_dev.append(st["device_ms"])
 
-    n_pairs = cfg["n_l"] * cfg["n_r"]
-    ms_per_step = dt * 1e3 / args.steps
-    value = n_pairs * args.steps / dt
     filt_ms = float(np.mean([s["filter_ms"] for s in alone]))
     filt_ms_timed = float(np.mean([s["filter_ms"] for s in stats]))
     cand = stats[-1]["candidates"]
@@ -276,12 +309,17 @@ def main():
                 "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
                 "ops_per_candidate": FILTER_OPS_PER_CANDIDATE, "candidates_per_launch": cand,
                 "avg_launch_ms": filt_ms,
-                "launch_ms_from": "jobs run alone after the timed loop (HIP events on the job's "
-                                  "stream); with %d jobs in flight the filter stage took %.4f ms"
-                                  % (args.inflight, filt_ms_timed),
+                "launch_ms_from": "jobs run alone on warm tables (HIP events on the job's "
+                                  "stream); with %d fresh jobs in flight the filter stage took "
+                                  "%.4f ms" % (args.inflight, filt_ms_timed),
                 "measured_valu_instr_per_candidate": measured_valu_per_candidate(args.workload),
                 "hbm_gbs": None if traffic is None else traffic / (filt_ms * 1e-3) / 1e9,
                 "hbm_peak_gbs": HBM_PEAK_GBS}
+    served = None
+    if not args.no_serve:
+        from hyperblocker_amd import workloads
+        served = workloads.run_c5(ctx, n_jobs=args.serve_jobs,
+                                  log=lambda m: print(m, file=sys.stderr, flush=True))
     cpu = None
     if args.cpu_rows > 0:
         cpu = cpu_baseline(cfg, L, R, plan, args.cpu_rows)
@@ -293,21 +331,36 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic (seeded generator, workload %s)" % args.workload,
         "config": {"workload": "%s: %d x %d rows, 1 rule (year eq, JW title>=0.89, "
-                               "JW author>=0.85), n_partitions=1"
-                               % (args.workload, cfg["n_l"], cfg["n_r"]),
-                   "pairs_per_step": n_pairs, "matches": int(len(pairs))},
-        # per job, jobs alone; in the timed loop (jobs overlapping) below
+                               "JW author>=0.85), n_partitions=1; each step a fresh job from "
+                               "the reference's row-major tables resident in HBM (per-table "
+                               "builds, kernels and the copy of the pairs to the host inside "
+                               "the step)" % (args.workload, cfg["n_l"], cfg["n_r"]),
+                   "pairs_per_step": n_pairs, "matches": int(len(pairs)),
+                   "inflight": args.inflight},
+        "fresh_job_alone_ms": float(np.median(lat_fresh)),
+        "fresh_job_alone_device_ms": float(np.median(fresh_dev)),
+        "pcie_inclusive_pairs_per_s": n_pairs * args.steps / dt_pcie,
+        "pcie_inclusive_ms_per_step": dt_pcie * 1e3 / args.steps,
+        "resident_warm_pairs_per_s": n_pairs * max(args.steps, 20) / dt_warm,
+        "resident_warm_ms_per_step": dt_warm * 1e3 / max(args.steps, 20),
+        "cold_first_job_ms": cold_s * 1e3,
+        # per job on warm tables, alone; fresh jobs in the timed loop (overlapping)
         "stage_ms": {k: float(np.mean([s[k] for s in alone]))
                      for k in ("device_ms", "join_ms", "filter_ms", "exact_ms")},
         "stage_ms_timed": {k: float(np.mean([s[k] for s in stats]))
                            for k in ("device_ms", "join_ms", "filter_ms", "exact_ms")},
-        "cold_first_job_ms": cold_s * 1e3,
-        "upload_ms": upload_s * 1e3,
-        "pcie_inclusive_pairs_per_s": n_pairs / (upload_s + ms_per_step * 1e-3),
-        "inflight": args.inflight, "job_alone_ms": float(np.median(lat)),
-        "job_alone_device_ms": float(np.mean([s["device_ms"] for s in alone])),
+        "job_alone_ms": float(np.median(lat)),
         "filtered": stats[-1]["filtered"], "jw_exact": stats[-1]["jw_exact"],
         "roofline": roofline, "cpu_baseline": cpu,
+        # c5: served (continuous batching) against plain launches on one trace
+        "served": None if served is None else {
+            k: served[k] for k in ("jobs", "right_rows", "arrival_rate_per_s", "offered_load",
+                                   "plain_service_ms", "served_pairs_per_s", "served_p50_ms",
+                                   "served_p99_ms", "plain_pairs_per_s", "plain_p50_ms",
+                                   "plain_p99_ms", "saturated_served_pairs_per_s",
+                                   "saturated_served_p50_ms", "saturated_served_p99_ms",
+                                   "saturated_plain_pairs_per_s", "saturated_plain_p50_ms",
+                                   "saturated_plain_p99_ms", "served_batches")},
         # run_hyperblocker_exec on the same tables as CSV files (fresh process)
         "end_to_end_cli": e2e,
     }
@@ -315,6 +368,8 @@ def main():
     dp.release()
     tl.release()
     tr.release()
+    dl.free()
+    dr.free()
     ctx.close()
 
 

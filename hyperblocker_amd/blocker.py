@@ -170,6 +170,50 @@ class SerializedTable:
                                          self._offset, self.data.ctypes.data)
 
 
+class DeviceBytes:
+    """A buffer of device memory holding a copy of host bytes (hipMalloc +
+    hipMemcpy through libamdhip64), for callers that hand the library data
+    already resident on the GPU (hb_table_from_device)."""
+
+    _hip = None
+
+    def __init__(self, host_bytes, device=0):
+        if DeviceBytes._hip is None:
+            DeviceBytes._hip = C.CDLL("libamdhip64.so")
+        hip = DeviceBytes._hip
+        buf = np.ascontiguousarray(host_bytes).view(np.uint8)
+        self.n_bytes = max(int(buf.nbytes), 1)
+        self.ptr = C.c_void_p()
+        if hip.hipSetDevice(device) != 0 or hip.hipMalloc(C.byref(self.ptr),
+                                                          C.c_size_t(self.n_bytes)) != 0:
+            raise MemoryError("hipMalloc of %d bytes failed" % self.n_bytes)
+        if buf.nbytes and hip.hipMemcpy(self.ptr, C.c_void_p(buf.ctypes.data),
+                                        C.c_size_t(buf.nbytes), 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            DeviceBytes._hip.hipDeviceSynchronize()
+            DeviceBytes._hip.hipFree(self.ptr)
+        self.ptr = None
+
+
+class DeviceSerializedTable:
+    """A SerializedTable whose rows sit in device memory: the reference's
+    d_tb_data with its col_size / col_offset (global_func.cuh:104-110)."""
+
+    def __init__(self, table, device=0):
+        self.host = table
+        self.n_rows = table.n_rows
+        self.buffer = DeviceBytes(table.data[: max(table.n_rows * table.aligned_tuple_size, 1)],
+                                  device)
+        self.view = capi.SerializedTable(table.n_rows, table.n_cols, table.aligned_tuple_size,
+                                         table._size, table._offset, self.buffer.ptr.value)
+
+    def free(self):
+        self.buffer.free()
+
+
 def partition(l_table, r_table, plan, n_partitions, prefix_idx=capi.CHECK_POINT):
     """DataPartitioning (core/components/data_mngr.h:81-222) of two HostTables
     or two ColumnTables.  Returns a list of (l_rows, r_rows) int64 arrays,
@@ -258,6 +302,13 @@ class Context:
 
     def upload(self, table, rows=None):
         return DeviceTable(self, table, rows)
+
+    def from_device(self, table):
+        """hb_table_from_device of a DeviceSerializedTable (rows already in
+        device memory; keep it alive until jobs on the table are done)."""
+        h = C.c_void_p()
+        check(lib().hb_table_from_device(self._h, C.byref(table.view), C.byref(h)))
+        return DeviceTable(self, table, _handle=h, _n_rows=table.n_rows)
 
     def upload_serialized(self, table):
         """hb_table_upload_serialized of a SerializedTable (the reference's

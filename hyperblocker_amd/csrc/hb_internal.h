@@ -201,7 +201,65 @@ struct AuditList {
   unsigned long long cap;
 };
 
+// Dictionary of a context in device memory: equality columns (and the code
+// tuples of composite keys) are coded here, so that equal codes mean equal
+// byte strings across every table of the context.  Open addressing keyed by
+// a 64-bit hash of the cell; the code of a value is its slot index.  The
+// first row to claim a slot copies its bytes to the arena as the slot's
+// representative, and every row is then compared byte for byte with the
+// representative of its slot: a row that differs (a hash collision), or that
+// found no slot within max_probe probes, or whose slot has no representative
+// (arena full), gets no code here and is coded by the host's exact dictionary
+// instead (codes from kFallbackCodeBase up).
+struct DictDev {
+  unsigned long long *keys;        // slots: hash, or kDictEmpty
+  unsigned long long *reps;        // slots: arena offset << 16 | chunks, or kDictNoRep
+  uint8_t *arena;
+  unsigned long long *arena_used;  // bytes handed out (may exceed arena_cap)
+  unsigned long long arena_cap;
+  uint32_t mask;                   // slots - 1 (slots a power of two, <= 2^30)
+  uint32_t max_probe;
+  uint32_t hash_bits;              // 64; fewer only to force collisions in tests
+  uint32_t pad;
+};
+constexpr unsigned long long kDictEmpty = ~0ull;
+constexpr unsigned long long kDictNoRep = ~0ull;
+constexpr uint32_t kNoCode = 0xffffffffu;
+constexpr uint32_t kFallbackCodeBase = 0x80000000u;
+constexpr int kMaxKeyColumns = 16;  // equality columns of one composite key
+
+struct KeyColumns {
+  const uint32_t *code[kMaxKeyColumns];
+  uint32_t k;
+};
+
+// One column of a row-major table (the reference's SerializedTable) to
+// unpack into a column of NUL-padded cells.
+struct UnpackColumn {
+  uint8_t *out;      // n_rows x width bytes
+  uint64_t offset;   // byte offset of the cell in a row
+  uint32_t size;     // bytes of the cell in a row (col_size)
+  uint32_t width;    // multiple of 16, > size
+};
+
 // Kernel launch wrappers (hb_kernels.hip).  All are asynchronous on stream.
+
+// Dictionary codes of n cells (each `width` bytes, 16-byte aligned, NUL
+// padded: a cell ends at its first NUL, or has fixed_len bytes when
+// fixed_len > 0).  Rows left to the host get kNoCode in `code` and are listed
+// in fb_rows[0 .. *fb_count) (fb_rows holds n entries; *fb_count must be 0).
+void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, uint32_t n,
+                      DictDev d, uint32_t *code, uint32_t *fb_rows,
+                      unsigned long long *fb_count, hipStream_t stream);
+// Key cells of a composite key: byte 0xC0 + k, then the k column codes
+// (little endian), zero padded to width (>= 1 + 4k, multiple of 16).
+void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t width,
+                         hipStream_t stream);
+// Unpacks n_rows rows of row_bytes bytes (the reference's row-major table on
+// the device) into the columns' NUL-padded cells: each cell is copied up to
+// its first NUL or `size` bytes, the rest of its width zeroed.
+void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,
+                        const UnpackColumn *cols, uint32_t n_cols, hipStream_t stream);
 void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint16_t *len,
                       SigRow *sig, hipStream_t stream);
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,

@@ -30,6 +30,7 @@
 // rounds once, in the reference's order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "../../include/hyperblocker_amd.h"
@@ -398,6 +399,200 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long n, unsigned
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (unsigned long long)gridDim.x * blockDim.x)
     p[i] = v;
+}
+
+// ---- device dictionary ---------------------------------------------------
+
+// 64-bit hash of a cell's 16-byte chunks (NUL padded, so equal strings give
+// equal chunks): multiply-xor per 8-byte word, then MurmurHash3's 64-bit
+// finalizer over the hash and the length.
+__device__ __forceinline__ uint64_t dict_mix(uint64_t h, uint64_t w) {
+  h = (h ^ w) * 0xff51afd7ed558ccdull;
+  return h ^ (h >> 32);
+}
+__device__ __forceinline__ uint64_t dict_final(uint64_t h, uint32_t len) {
+  h ^= (uint64_t)len * 0x9E3779B97F4A7C15ull;
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return h;
+}
+
+// Position of the first zero byte of a dword (4 if none).
+__device__ __forceinline__ uint32_t first_zero_byte(uint32_t x) {
+  const uint32_t z = (x - 0x01010101u) & ~x & 0x80808080u;
+  return z ? (uint32_t)(__builtin_ctz(z) >> 3) : 4u;
+}
+
+// Length and number of 16-byte chunks of a cell (the chunks hold the cell's
+// bytes and, in NUL mode, its terminating NUL), and its hash.
+__device__ __forceinline__ uint64_t dict_hash_cell(const uint4 *cell, uint32_t width,
+                                                   uint32_t fixed_len, uint32_t hash_bits,
+                                                   uint32_t &len, uint32_t &chunks) {
+  uint64_t h = 0x9E3779B97F4A7C15ull;
+  const uint32_t n16 = width >> 4;
+  len = fixed_len;
+  chunks = fixed_len ? (fixed_len + 15) >> 4 : n16;
+  for (uint32_t k = 0; k < chunks; k++) {
+    const uint4 v = cell[k];
+    h = dict_mix(h, ((uint64_t)v.y << 32) | v.x);
+    h = dict_mix(h, ((uint64_t)v.w << 32) | v.z);
+    if (!fixed_len) {
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      uint32_t pos = 16;
+#pragma unroll
+      for (int q = 3; q >= 0; q--) {
+        const uint32_t z = first_zero_byte(w[q]);
+        if (z < 4) pos = 4 * q + z;
+      }
+      if (pos < 16) {
+        len = 16 * k + pos;
+        chunks = k + 1;
+        break;
+      }
+    }
+  }
+  h = dict_final(h, len);
+  if (hash_bits < 64) h &= (1ull << hash_bits) - 1ull;
+  return h == kDictEmpty ? kDictEmpty - 1ull : h;
+}
+
+// One thread per row: find or claim the slot of the cell's hash.
+__global__ void k_dict_insert(const uint8_t *__restrict__ cells, uint32_t width,
+                              uint32_t fixed_len, uint32_t n, DictDev d,
+                              uint32_t *__restrict__ code) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint4 *cell = reinterpret_cast<const uint4 *>(cells + (size_t)r * width);
+  uint32_t len, chunks;
+  const uint64_t h = dict_hash_cell(cell, width, fixed_len, d.hash_bits, len, chunks);
+  uint32_t slot = (uint32_t)(h ^ (h >> 31)) & d.mask;
+  uint32_t c = kNoCode;
+  for (uint32_t p = 0; p < d.max_probe; p++) {
+    unsigned long long k = __hip_atomic_load(d.keys + slot, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    if (k == kDictEmpty) {
+      k = atomicCAS(d.keys + slot, kDictEmpty, (unsigned long long)h);
+      if (k == kDictEmpty) {  // this row's cell becomes the slot's representative
+        const unsigned long long bytes = (unsigned long long)chunks * 16;
+        const unsigned long long off = atomicAdd(d.arena_used, bytes);
+        unsigned long long rep = kDictNoRep;
+        if (off + bytes <= d.arena_cap && chunks < 65536) {
+          uint4 *dst = reinterpret_cast<uint4 *>(d.arena + off);
+          for (uint32_t q = 0; q < chunks; q++) dst[q] = cell[q];
+          rep = (off << 16) | chunks;
+        }
+        d.reps[slot] = rep;
+        c = slot;
+        break;
+      }
+    }
+    if (k == h) {
+      c = slot;
+      break;
+    }
+    slot = (slot + 1) & d.mask;
+  }
+  code[r] = c;
+}
+
+// One thread per row: the cell must equal its slot's representative; rows
+// that do not (or have no slot) are listed for the host.
+__global__ void k_dict_verify(const uint8_t *__restrict__ cells, uint32_t width,
+                              uint32_t fixed_len, uint32_t n, DictDev d,
+                              uint32_t *__restrict__ code, uint32_t *__restrict__ fb_rows,
+                              unsigned long long *fb_count) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t c = code[r];
+  bool ok = c != kNoCode;
+  if (ok) {
+    const unsigned long long rep = d.reps[c];
+    const uint4 *cell = reinterpret_cast<const uint4 *>(cells + (size_t)r * width);
+    uint32_t chunks = fixed_len ? (fixed_len + 15) >> 4 : 0;
+    if (!fixed_len) {  // the cell's chunks up to the one holding its NUL
+      for (uint32_t k = 0; k < (width >> 4); k++) {
+        const uint4 v = cell[k];
+        chunks = k + 1;
+        if (first_zero_byte(v.x) < 4 || first_zero_byte(v.y) < 4 || first_zero_byte(v.z) < 4 ||
+            first_zero_byte(v.w) < 4)
+          break;
+      }
+    }
+    ok = rep != kDictNoRep && (uint32_t)(rep & 0xffffu) == chunks;
+    if (ok) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(d.arena + (rep >> 16));
+      for (uint32_t q = 0; q < chunks && ok; q++) {
+        const uint4 a = cell[q], b = src[q];
+        ok = a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+      }
+    }
+  }
+  if (!ok) {
+    code[r] = kNoCode;
+    fb_rows[atomicAdd(fb_count, 1ull)] = r;
+  }
+}
+
+// Key cells of a composite key: 0xC0 + k, then the k codes.
+__global__ void k_compose_keys(KeyColumns cols, uint32_t n, uint8_t *__restrict__ cells,
+                               uint32_t width) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t *dst = reinterpret_cast<uint32_t *>(cells + (size_t)r * width);
+  // byte 0 = marker, bytes 1 + 4q .. 4 + 4q = code q: assembled as dwords
+  uint32_t carry = 0xC0u + cols.k;
+  uint32_t w = 0;
+  for (uint32_t q = 0; q < cols.k; q++) {
+    const uint32_t v = cols.code[q][r];
+    dst[w++] = carry | (v << 8);
+    carry = v >> 24;
+  }
+  dst[w++] = carry;
+  for (; w < (width >> 2); w++) dst[w] = 0u;
+}
+
+// One wave per row (grid-stride): the row's cells are read byte by byte
+// (cells of the reference's layout are not aligned), lane j holding bytes
+// 4j .. 4j+3 of a 256-byte block of the cell, and written as aligned dwords
+// with every byte from the first NUL on set to zero.
+__global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__ rows,
+                                                    uint64_t row_bytes, uint32_t n_rows,
+                                                    const UnpackColumn *__restrict__ cols,
+                                                    uint32_t n_cols) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t r = wave; r < n_rows; r += n_waves) {
+    const uint8_t *row = rows + (size_t)r * row_bytes;
+    for (uint32_t c = 0; c < n_cols; c++) {
+      const UnpackColumn col = cols[c];
+      const uint8_t *src = row + col.offset;
+      uint32_t *dst = reinterpret_cast<uint32_t *>(col.out + (size_t)r * col.width);
+      bool ended = false;  // wave-uniform: a NUL was seen in an earlier block
+      for (uint32_t base = 0; base < col.width; base += 256) {
+        const uint32_t p0 = base + 4 * lane;
+        uint32_t v = 0;
+        if (!ended) {
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (p0 + q < col.size) v |= (uint32_t)src[p0 + q] << (8 * q);
+        }
+        // bytes past `size` read as 0, so a cell of exactly `size` bytes ends there
+        const uint32_t z = first_zero_byte(v);
+        const uint64_t zmask = __ballot(z < 4 && p0 < col.width);
+        if (zmask) {
+          const uint32_t first = (uint32_t)__builtin_ctzll(zmask);
+          if (lane > first) v = 0;
+          else if (lane == first) v &= z == 0 ? 0u : (0xffffffffu >> (32 - 8 * z));
+          ended = true;
+        }
+        if (p0 < col.width) dst[p0 >> 2] = v;
+      }
+    }
+  }
 }
 
 // Join-order keys: (dictionary code << 8) | min(length, 255) of the rule's
@@ -1652,6 +1847,29 @@ void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long 
   unsigned g = grid_for(n, 256);
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(256), 0, stream, p, n, v);
+}
+
+void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, uint32_t n,
+                      DictDev d, uint32_t *code, uint32_t *fb_rows,
+                      unsigned long long *fb_count, hipStream_t stream) {
+  if (!n) return;
+  const uint32_t blocks = (n + 255) / 256;
+  k_dict_insert<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code);
+  k_dict_verify<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code, fb_rows,
+                                            fb_count);
+}
+
+void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t width,
+                         hipStream_t stream) {
+  if (!n) return;
+  k_compose_keys<<<(n + 255) / 256, 256, 0, stream>>>(cols, n, cells, width);
+}
+
+void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,
+                        const UnpackColumn *cols, uint32_t n_cols, hipStream_t stream) {
+  if (!n_rows || !n_cols) return;
+  const uint32_t blocks = std::min<uint32_t>((n_rows + 3) / 4, 8192);
+  k_unpack_rows<<<blocks, 256, 0, stream>>>(rows, row_bytes, n_rows, cols, n_cols);
 }
 
 void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,

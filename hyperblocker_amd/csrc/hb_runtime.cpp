@@ -601,6 +601,21 @@ struct hb_ctx {
   // Test hook: this many batch launches fail before any device work
   // (HB_INJECT_LAUNCH_FAILURES at creation), to exercise the error path.
   std::atomic<int> inject_launch_failures{0};
+  // Device dictionary of equality codes (hb::DictDev), created on first use;
+  // its kernels run on the build stream, one column at a time.  `dict`
+  // above codes the rows it leaves to the host.
+  struct DeviceDict {
+    std::mutex mu;
+    bool created = false;
+    hb::DictDev dev{};
+    void *mem = nullptr;
+    uint64_t slots = 1ull << 23;
+    uint64_t arena_bytes = 256ull << 20;
+    uint32_t max_probe = 512;
+    uint32_t hash_bits = 64;
+    std::atomic<bool> fallback_used{false};  // some code came from the host dictionary
+    std::atomic<uint64_t> fallback_rows{0};
+  } ddict;
 };
 
 namespace {
@@ -624,7 +639,6 @@ struct Lazy {
 struct Column {
   uint32_t width = 16;
   uint8_t *d_bytes = nullptr;
-  char *h_bytes = nullptr;  // pinned staging, kept for dictionary coding
   // similarity predicates
   Lazy sig;
   uint16_t *d_len = nullptr;
@@ -635,7 +649,6 @@ struct Column {
   // equality predicates
   Lazy code;
   uint32_t *d_code = nullptr;
-  uint32_t *h_code = nullptr;  // pinned
 };
 
 // Join key of a table for a list of two or more equality columns: one
@@ -643,7 +656,6 @@ struct Column {
 struct Composite {
   Lazy code;
   uint32_t *d_code = nullptr;
-  uint32_t *h_code = nullptr;  // pinned
 };
 
 // The rows of a table in join order for one rule shape: sorted by the join
@@ -660,7 +672,10 @@ struct Layout {
   hb::SigRow *d_jsig_fb = nullptr;
   uint32_t *d_run_code = nullptr;
   uint32_t *d_run_off = nullptr;  // n_runs + 1
-  uint32_t n_runs = 0;
+  // Number of runs: counted on the device by the build and copied to
+  // h_nruns (pinned); -1 until the host has read it there.
+  std::atomic<int64_t> n_runs{-1};
+  uint32_t *h_nruns = nullptr;
 };
 
 }  // namespace
@@ -1006,6 +1021,96 @@ struct HostTrace {
   }
 };
 
+// Creates the context's device dictionary on first use (sizes from
+// HB_DICT_SLOTS, HB_DICT_ARENA_MB, HB_DICT_MAX_PROBE and HB_DICT_HASH_BITS at
+// context creation).
+hb_status dict_create(hb_ctx *ctx, hipStream_t stream) {
+  hb_ctx::DeviceDict &dd = ctx->ddict;
+  std::lock_guard<std::mutex> lk(dd.mu);
+  if (dd.created) return HB_OK;
+  const size_t slot_bytes = dd.slots * 8;
+  const size_t bytes = 2 * slot_bytes + dd.arena_bytes + 256;
+  HIP_OK(hipMalloc(&dd.mem, bytes));
+  char *p = (char *)dd.mem;
+  dd.dev.keys = (unsigned long long *)p;
+  dd.dev.reps = (unsigned long long *)(p + slot_bytes);
+  dd.dev.arena_used = (unsigned long long *)(p + 2 * slot_bytes);
+  dd.dev.arena = (uint8_t *)(p + 2 * slot_bytes + 256);
+  dd.dev.arena_cap = dd.arena_bytes;
+  dd.dev.mask = (uint32_t)(dd.slots - 1);
+  dd.dev.max_probe = dd.max_probe;
+  dd.dev.hash_bits = dd.hash_bits;
+  hb::launch_fill_u64(dd.dev.keys, dd.slots, hb::kDictEmpty, stream);
+  hb::launch_fill_u64(dd.dev.arena_used, 1, 0ull, stream);
+  HIP_OK(hipGetLastError());
+  dd.created = true;
+  return HB_OK;
+}
+
+// Dictionary codes of n cells (device memory, `width` bytes each; a cell
+// ends at its first NUL, or has fixed_len bytes) into d_code, on `stream`.
+// Waits for the device: rows the device dictionary could not code (hash
+// collisions, a full table or arena) are coded by the host dictionary, from
+// kFallbackCodeBase up, and written back before this returns.
+hb_status dict_code(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t fixed_len,
+                    size_t n, uint32_t *d_code, hipStream_t stream) {
+  if (n == 0) return HB_OK;
+  HB_TRY(dict_create(ctx, stream));
+  DevicePool &pool = ctx->pool;
+  uint32_t *fb_rows = nullptr;
+  unsigned long long *fb_count = nullptr, *h_count = nullptr;
+  HB_TRY(pool.alloc(n * sizeof(uint32_t), (void **)&fb_rows));
+  hb_status st = pool.alloc(sizeof(unsigned long long), (void **)&fb_count);
+  if (st == HB_OK) st = ctx->hpool.alloc(sizeof(unsigned long long), (void **)&h_count, nullptr);
+  hipEvent_t ev = nullptr;
+  if (st == HB_OK) st = ctx->events.get(&ev);
+  auto done = [&](hb_status r) {
+    if (ev) ctx->events.put(ev);
+    if (h_count) ctx->hpool.free(h_count, sizeof(unsigned long long));
+    if (fb_count) pool.free(fb_count, sizeof(unsigned long long));
+    pool.free(fb_rows, n * sizeof(uint32_t));
+    return r;
+  };
+  if (st != HB_OK) return done(st);
+  hb::launch_fill_u64(fb_count, 1, 0ull, stream);
+  hb::launch_dict_code(cells, width, fixed_len, (uint32_t)n, ctx->ddict.dev, d_code, fb_rows,
+                       fb_count, stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_count, fb_count, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       stream);
+  if (e == hipSuccess) e = hipEventRecord(ev, stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev);
+  if (e != hipSuccess) return done(fail(HB_ERR_HIP, "dictionary coding: %s", hipGetErrorString(e)));
+  const unsigned long long n_fb = *h_count;
+  if (n_fb) {  // rare: code those rows on the host (exact byte keys)
+    std::vector<uint32_t> rows(n_fb), codes(n);
+    std::vector<char> host_cells(n * (size_t)width);
+    e = hipMemcpy(rows.data(), fb_rows, n_fb * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(host_cells.data(), cells, host_cells.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(codes.data(), d_code, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+      return done(fail(HB_ERR_HIP, "dictionary fallback: %s", hipGetErrorString(e)));
+    for (uint32_t r : rows) {
+      const char *cell = host_cells.data() + (size_t)r * width;
+      const std::string_view v(cell, fixed_len ? fixed_len : strnlen(cell, width));
+      const uint32_t c = ctx->dict.code_one(v, Dictionary::hash(v));
+      if (c >= hb::kFallbackCodeBase)
+        return done(fail(HB_ERR_CAPACITY, "dictionary: more than 2^31 distinct values"));
+      codes[r] = hb::kFallbackCodeBase | c;
+    }
+    e = hipMemcpy(d_code, codes.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return done(fail(HB_ERR_HIP, "dictionary fallback: %s", hipGetErrorString(e)));
+    ctx->ddict.fallback_used.store(true);
+    ctx->ddict.fallback_rows.fetch_add(n_fb);
+  }
+  return done(HB_OK);
+}
+
+// Dictionary codes of column c (on first use; a slice borrows its root's).
 hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   Column &col = t->cols[c];
   if (t->parent) {  // borrow the parent's codes at the slice's offset
@@ -1014,7 +1119,6 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(t->mu);
     if (!col.code.built) {
       col.d_code = pc.d_code + t->row_off;
-      col.h_code = pc.h_code + t->row_off;
       col.code.built = true;
     }
     return HB_OK;
@@ -1022,37 +1126,28 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
   std::lock_guard<std::mutex> lk(t->mu);
   if (col.code.built) return wait_lazy(col.code, stream);
   const size_t n = (size_t)t->n_rows;
-  HB_TRY(t->host_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.h_code));
-  {
-    const char *bytes = col.h_bytes;
-    const uint32_t w = col.width;
-    code_strings(t->ctx->dict, n, [&](size_t r) {
-      const char *s = bytes + r * w;
-      return std::string_view(s, strnlen(s, w));
-    }, col.h_code);
-  }
   HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.d_code));
-  if (n) HIP_OK(hipMemcpyAsync(col.d_code, col.h_code, n * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, stream));
+  HB_TRY(dict_code(t->ctx, col.d_bytes, col.width, 0, n, col.d_code, stream));
   return record_lazy(col.code, stream);
 }
 
-// Join key codes of the equality columns `cols` (one column: its codes;
-// several: the composite codes, built from the columns' codes on the host,
-// shared by every table of the context through the dictionary).
+// Join key codes of the equality columns `cols`: one column's codes, or for
+// several the composite codes (the dictionary codes of each row's tuple of
+// column codes), so that equal keys mean every listed cell equal.
 hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stream,
-                     const uint32_t **out, const uint32_t **h_out) {
+                     const uint32_t **out) {
   if (cols.size() == 1) {
     HB_TRY(ensure_code(t, cols[0], stream));
     *out = t->cols[cols[0]].d_code;
-    *h_out = t->cols[cols[0]].h_code;
     return HB_OK;
   }
+  if (cols.size() > (size_t)hb::kMaxKeyColumns)
+    return fail(HB_ERR_PLAN, "a rule has %zu equality predicates; at most %d are supported",
+                cols.size(), hb::kMaxKeyColumns);
   if (t->parent) {  // borrow the parent's composite codes at the slice's offset
-    const uint32_t *pc = nullptr, *ph = nullptr;
-    HB_TRY(ensure_key(t->parent, cols, stream, &pc, &ph));
+    const uint32_t *pc = nullptr;
+    HB_TRY(ensure_key(t->parent, cols, stream, &pc));
     *out = pc + t->row_off;
-    *h_out = ph + t->row_off;
     return HB_OK;
   }
   for (int c : cols) HB_TRY(ensure_code(t, c, stream));
@@ -1060,49 +1155,37 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
   Composite &cp = t->composites[cols];
   if (cp.code.built) {
     *out = cp.d_code;
-    *h_out = cp.h_code;
     return wait_lazy(cp.code, stream);
   }
   const size_t n = (size_t)t->n_rows;
-  HB_TRY(t->host_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.h_code));
-  {
-    std::vector<const uint32_t *> src;
-    for (int c : cols) src.push_back(t->cols[c].h_code);
-    const size_t k = cols.size(), kb = 1 + 4 * k;
-    // key bytes per row: a marker byte, then the column codes (4 bytes each)
-    std::vector<char> keys(n * kb);
-    parallel_for((int64_t)n, 4096, [&](int64_t b, int64_t e) {
-      for (int64_t r = b; r < e; r++) {
-        char *key = keys.data() + (size_t)r * kb;
-        key[0] = (char)(0xC0 + k);
-        for (size_t q = 0; q < k; q++) std::memcpy(key + 1 + 4 * q, &src[q][r], 4);
-      }
-    });
-    code_strings(t->ctx->dict, n, [&](size_t r) {
-      return std::string_view(keys.data() + r * kb, kb);
-    }, cp.h_code);
-  }
   HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.d_code));
-  if (n) HIP_OK(hipMemcpyAsync(cp.d_code, cp.h_code, n * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, stream));
+  if (n) {
+    const uint32_t key_len = 1 + 4 * (uint32_t)cols.size();
+    const uint32_t width = (key_len + 15) & ~15u;
+    hb::KeyColumns kc{};
+    kc.k = (uint32_t)cols.size();
+    for (size_t q = 0; q < cols.size(); q++) kc.code[q] = t->cols[cols[q]].d_code;
+    uint8_t *cells = nullptr;
+    HB_TRY(t->ctx->pool.alloc(n * width, (void **)&cells));
+    hb::launch_compose_keys(kc, (uint32_t)n, cells, width, stream);
+    const hb_status st = dict_code(t->ctx, cells, width, key_len, n, cp.d_code, stream);
+    t->ctx->pool.free(cells, n * width);  // dict_code waited for the device
+    HB_TRY(st);
+  }
   *out = cp.d_code;
-  *h_out = cp.h_code;
   return record_lazy(cp.code, stream);
 }
 
-// Number of distinct values among codes[0..n) (all below bound).
-uint32_t count_distinct(const uint32_t *codes, size_t n, uint64_t bound) {
-  if (n == 0) return 0;
-  if (bound <= 64 * (uint64_t)n + 4096) {
-    std::vector<uint64_t> bits((size_t)((bound + 63) / 64), 0);
-    for (size_t r = 0; r < n; r++) bits[codes[r] >> 6] |= 1ull << (codes[r] & 63);
-    uint64_t c = 0;
-    for (uint64_t w : bits) c += (uint64_t)__builtin_popcountll(w);
-    return (uint32_t)c;
+// Runs of a layout: read from the device once its build is done.
+hb_status layout_runs(Layout *ly, uint32_t *out) {
+  int64_t v = ly->n_runs.load(std::memory_order_acquire);
+  if (v < 0) {
+    HIP_OK(hipEventSynchronize(ly->lz.ev));
+    v = (int64_t)*ly->h_nruns;
+    ly->n_runs.store(v, std::memory_order_release);
   }
-  std::vector<uint32_t> v(codes, codes + n);
-  std::sort(v.begin(), v.end());
-  return (uint32_t)(std::unique(v.begin(), v.end()) - v.begin());
+  *out = (uint32_t)v;
+  return HB_OK;
 }
 
 // Builds (once per table and rule shape) the join-order layout: keys from the
@@ -1127,13 +1210,14 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   HostTrace trace("ensure_layout");
   const hipStream_t stream = t->ctx->build_stream;
   HB_TRY(wait_ready(t, stream));
-  const uint32_t *key_codes = nullptr, *h_key = nullptr;
-  if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes, &h_key));
+  const uint32_t *key_codes = nullptr;
+  if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes));
   trace.mark("key");
   if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
   trace.mark("sig");
   uint64_t n_codes = 1;
-  if (!eq.empty()) n_codes = t->ctx->dict.code_bound();
+  if (!eq.empty())
+    n_codes = t->ctx->ddict.fallback_used.load() ? (1ull << 32) : t->ctx->ddict.slots;
   std::lock_guard<std::mutex> lk(t->mu);
   Layout &ly = t->layouts[{eq, sim}];
   *out = &ly;
@@ -1149,12 +1233,13 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_perm_fb));
     HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig_fb));
   }
+  if (n == 0 || eq.empty()) ly.n_runs.store(n ? 1 : 0);
   if (n) {
-    // The run count (distinct keys) is counted on the host from the codes it
-    // holds, so the build is asynchronous: batches wait on its event, and its
-    // temporaries return to the pool once the build stream passes them.
-    ly.n_runs = eq.empty() ? 1u : count_distinct(h_key, n, n_codes);
-    trace.mark("distinct");
+    // The run count is counted on the device and copied to pinned memory;
+    // the first batch that needs it waits for the build there.  The
+    // build's temporaries return to the pool once the build stream passes
+    // them.
+    HB_TRY(t->host_alloc(sizeof(uint32_t), (void **)&ly.h_nruns));
     DevicePool &pool = t->ctx->pool;
     hb_ctx::Grave temps;
     auto temp = [&](size_t bytes, void **p) -> hb_status {
@@ -1193,6 +1278,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     HIP_OK(hb::exscan_u32(tmp, ts, flags, run_id, n, stream));
     hb::launch_layout_runs(ly.d_keys, flags, run_id, n, ly.d_run_code, ly.d_run_off, d_nruns,
                            stream);
+    HIP_OK(hipMemcpyAsync(ly.h_nruns, d_nruns, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     if (sim >= 0) {
       hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
       // the first-byte order: keys_in is free again once the first sort has run
@@ -1351,7 +1437,7 @@ hb_status enqueue_batch(Batch *b) {
       sd.jsig_fb = ly->d_jsig_fb ? ly->d_jsig_fb : ly->d_jsig;
       sd.run_code = ly->d_run_code;
       sd.run_off = ly->d_run_off;
-      sd.n_runs = ly->n_runs;
+      HB_TRY(layout_runs(ly, &sd.n_runs));
     }
     if (!rule.sims.empty()) {
       const double thr = (double)rule.sims[0].thr;
@@ -2065,6 +2151,70 @@ Batch::~Batch() {
 
 namespace {
 
+// Returns memory to the pools once `stream` has passed this point.
+hb_status retire_after(hb_ctx *ctx, hb_ctx::Grave &&g, hipStream_t stream) {
+  if (g.dev.empty() && g.host.empty()) return HB_OK;
+  hipEvent_t e = nullptr;
+  HB_TRY(ctx->events.get(&e));
+  g.wait.push_back(e);
+  HIP_OK(hipEventRecord(e, stream));
+  std::lock_guard<std::mutex> lk(ctx->grave_mu);
+  ctx->graves.push_back(std::move(g));
+  return HB_OK;
+}
+
+hb_status check_serialized(const hb_serialized_table *s) {
+  if (!s || (!s->data && s->n_rows) || !s->col_size || !s->col_offset || s->n_cols == 0 ||
+      s->n_cols > 65536)
+    return fail(HB_ERR_INVALID_ARG, "serialized table: null arrays or bad column count");
+  if (s->n_rows > 0xFFFFFFF0ull)
+    return fail(HB_ERR_CAPACITY, "serialized table: %zu rows exceed the 32-bit row index",
+                s->n_rows);
+  for (size_t c = 0; c < s->n_cols; c++)
+    if (s->col_size[c] == 0 || s->col_size[c] > (1u << 30) ||
+        s->col_offset[c] + s->col_size[c] > s->aligned_tuple_size)
+      return fail(HB_ERR_INVALID_ARG, "serialized table: column %zu (offset %zu, size %zu) "
+                  "exceeds aligned_tuple_size %zu", c, s->col_offset[c], s->col_size[c],
+                  s->aligned_tuple_size);
+  return HB_OK;
+}
+
+// A new table whose columns are unpacked on `stream` from the n_rows rows
+// of the reference's row-major layout at d_rows (device memory); the
+// table's ready event is recorded there.  Column widths: col_size + 1 (room
+// for the NUL of a cell that fills its col_size bytes) rounded up to 16.
+hb_status unpack_table(hb_ctx *ctx, const hb_serialized_table *s, const uint8_t *d_rows,
+                       hipStream_t stream, std::unique_ptr<hb_table> &out) {
+  auto t = std::make_unique<hb_table>();
+  t->ctx = ctx;
+  t->n_rows = (int64_t)s->n_rows;
+  t->n_cols = (int32_t)s->n_cols;
+  t->cols.resize(s->n_cols);
+  // column parameters: staged in pinned memory (the table's), copied to the
+  // device on the stream ahead of the kernel
+  hb::UnpackColumn *h_cols = nullptr;
+  void *d_cols = nullptr;
+  HB_TRY(t->host_alloc(s->n_cols * sizeof(hb::UnpackColumn), (void **)&h_cols));
+  HB_TRY(t->dev_alloc(s->n_cols * sizeof(hb::UnpackColumn), &d_cols));
+  for (size_t c = 0; c < s->n_cols; c++) {
+    Column &col = t->cols[c];
+    col.width = (uint32_t)((s->col_size[c] + 1 + 15) & ~size_t(15));
+    const size_t bytes = std::max<size_t>(s->n_rows * col.width, 16);
+    HB_TRY(t->dev_alloc(bytes, (void **)&col.d_bytes));
+    h_cols[c] = hb::UnpackColumn{col.d_bytes, (uint64_t)s->col_offset[c], (uint32_t)s->col_size[c],
+                                 col.width};
+  }
+  HIP_OK(hipMemcpyAsync(d_cols, h_cols, s->n_cols * sizeof(hb::UnpackColumn),
+                        hipMemcpyHostToDevice, stream));
+  hb::launch_unpack_rows(d_rows, s->aligned_tuple_size, (uint32_t)s->n_rows,
+                         (const hb::UnpackColumn *)d_cols, (uint32_t)s->n_cols, stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
+  HIP_OK(hipEventRecord(t->ready, stream));
+  out = std::move(t);
+  return HB_OK;
+}
+
 hb_status check_cell_bytes(const hb_table_view *v) {
   if (!v || v->n_rows < 0 || v->n_cols <= 0 || !v->col_base || !v->col_stride)
     return fail(HB_ERR_INVALID_ARG, "table view: bad shape or null arrays");
@@ -2103,6 +2253,22 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
     int rpl = 0, tb = 0, gpc = 0;
     if (std::sscanf(env, "%d,%d,%d", &rpl, &tb, &gpc) == 3 && valid_filter_shape(rpl, tb, gpc))
       ctx->shape = hb::FilterShape{rpl, (uint32_t)tb, (unsigned)gpc};
+  }
+  if (const char *env = std::getenv("HB_DICT_SLOTS")) {  // log2 of the slot count
+    const int v = std::atoi(env);
+    if (v >= 4 && v <= 30) ctx->ddict.slots = 1ull << v;
+  }
+  if (const char *env = std::getenv("HB_DICT_ARENA_MB")) {
+    const long v = std::atol(env);
+    if (v >= 0 && v <= (1l << 20)) ctx->ddict.arena_bytes = (uint64_t)v << 20;
+  }
+  if (const char *env = std::getenv("HB_DICT_MAX_PROBE")) {
+    const int v = std::atoi(env);
+    if (v >= 1) ctx->ddict.max_probe = (uint32_t)v;
+  }
+  if (const char *env = std::getenv("HB_DICT_HASH_BITS")) {
+    const int v = std::atoi(env);
+    if (v >= 1 && v <= 64) ctx->ddict.hash_bits = (uint32_t)v;
   }
   if (const char *env = std::getenv("HB_INJECT_LAUNCH_FAILURES"))
     ctx->inject_launch_failures.store(std::max(0, std::atoi(env)));
@@ -2179,6 +2345,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   }
   (void)reap_graves(ctx, true);
   for (auto &kv : ctx->allow_tabs) (void)hipFree(kv.second);
+  if (ctx->ddict.mem) (void)hipFree(ctx->ddict.mem);
   ctx->pool.release_all();
   ctx->hpool.release_all();
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
@@ -2236,13 +2403,25 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
     std::lock_guard<std::mutex> lk(mm);
     for (int c = 0; c < nc; c++) maxlen[c] = std::max(maxlen[c], ml[c]);
   });
+  // Pinned staging of the packed cells: returned to the pool once the copy
+  // to the device has run (the device dictionary codes from device memory).
+  hb_ctx::Grave staging;
+  std::vector<char *> h_bytes(nc, nullptr);
+  auto give_back = [&]() {
+    for (auto &m : staging.host) ctx->hpool.free(m.first, m.second);
+  };
   for (int c = 0; c < nc; c++) {
     Column &col = t->cols[c];
     col.width = (uint32_t)((maxlen[c] + 1 + 15) & ~size_t(15));
     const size_t bytes = std::max<size_t>((size_t)n_rows * col.width, 16);
-    HB_TRY(t->host_alloc(bytes, (void **)&col.h_bytes));
-    HB_TRY(t->dev_alloc(bytes, (void **)&col.d_bytes));
-    if (n_rows == 0) std::memset(col.h_bytes, 0, bytes);
+    hb_status st = ctx->hpool.alloc(bytes, (void **)&h_bytes[c], nullptr);
+    if (st == HB_OK) staging.host.push_back({h_bytes[c], bytes});
+    if (st == HB_OK) st = t->dev_alloc(bytes, (void **)&col.d_bytes);
+    if (st != HB_OK) {
+      give_back();
+      return st;
+    }
+    if (n_rows == 0) std::memset(h_bytes[c], 0, bytes);
   }
   trace.mark("widths+alloc");
   // pack (each cell NUL padded to its column's width)
@@ -2255,20 +2434,26 @@ hb_status hb_table_upload_rows(hb_ctx *ctx, const hb_table_view *v, const int64_
         if (i + kAhead < e) __builtin_prefetch(v->col_base[c] + src_row(i + kAhead) * stride);
         const char *s = v->col_base[c] + src_row(i) * stride;
         const size_t len = lc ? lc[i] : strnlen(s, (size_t)stride);
-        char *dst = col.h_bytes + (size_t)i * col.width;
+        char *dst = h_bytes[c] + (size_t)i * col.width;
         std::memcpy(dst, s, len);
         std::memset(dst + len, 0, col.width - len);
       }
     }
   });
-  for (int c = 0; c < nc; c++) {
+  hipError_t e = hipSuccess;
+  for (int c = 0; c < nc && e == hipSuccess; c++) {
     Column &col = t->cols[c];
     const size_t bytes = std::max<size_t>((size_t)n_rows * col.width, 16);
-    HIP_OK(hipMemcpyAsync(col.d_bytes, col.h_bytes, bytes, hipMemcpyHostToDevice,
-                          ctx->copy_stream));
+    e = hipMemcpyAsync(col.d_bytes, h_bytes[c], bytes, hipMemcpyHostToDevice, ctx->copy_stream);
   }
-  HIP_OK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
-  HIP_OK(hipEventRecord(t->ready, ctx->copy_stream));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ready, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(t->ready, ctx->copy_stream);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    give_back();
+    return fail(HB_ERR_HIP, "table upload: %s", hipGetErrorString(e));
+  }
+  HB_TRY(retire_after(ctx, std::move(staging), ctx->copy_stream));
   trace.mark("pack+copy");
   *out = t.release();
   return HB_OK;
@@ -2279,31 +2464,53 @@ hb_status hb_table_upload(hb_ctx *ctx, const hb_table_view *v, hb_table **out) {
 }
 
 hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *s, hb_table **out) {
-  if (!s || (!s->data && s->n_rows) || !s->col_size || !s->col_offset)
-    return fail(HB_ERR_INVALID_ARG, "hb_table_upload_serialized: null argument");
-  std::vector<const char *> base(s->n_cols);
-  std::vector<int64_t> stride(s->n_cols);
-  for (size_t c = 0; c < s->n_cols; c++) {
-    if (s->col_offset[c] + s->col_size[c] > s->aligned_tuple_size)
-      return fail(HB_ERR_INVALID_ARG, "column %zu exceeds aligned_tuple_size", c);
-    base[c] = s->data + s->col_offset[c];
-    stride[c] = (int64_t)s->aligned_tuple_size;
+  if (!ctx || !out) return fail(HB_ERR_INVALID_ARG, "hb_table_upload_serialized: null argument");
+  *out = nullptr;
+  HB_TRY(check_serialized(s));
+  HB_TRY(set_device(ctx));
+  HostTrace trace("table_upload_serialized");
+  // The rows as they are: one parallel copy into pinned staging, one copy
+  // to the device, then the cells are unpacked there.
+  const size_t bytes = std::max<size_t>(s->n_rows * s->aligned_tuple_size, 16);
+  hb_ctx::Grave temps;
+  char *h_rows = nullptr;
+  uint8_t *d_rows = nullptr;
+  HB_TRY(ctx->hpool.alloc(bytes, (void **)&h_rows, nullptr));
+  temps.host.push_back({h_rows, bytes});
+  hb_status st = ctx->pool.alloc(bytes, (void **)&d_rows);
+  if (st != HB_OK) {
+    ctx->hpool.free(h_rows, bytes);
+    return st;
   }
-  // Each cell is capped at its col_size: copy through a view whose stride is
-  // the row size but whose cells stop at col_size bytes.
-  std::vector<std::vector<char>> cols(s->n_cols);
-  std::vector<const char *> cb(s->n_cols);
-  std::vector<int64_t> cs(s->n_cols);
-  for (size_t c = 0; c < s->n_cols; c++) {
-    const size_t w = s->col_size[c] + 1;
-    cols[c].assign(std::max<size_t>(s->n_rows * w, 1), 0);
-    for (size_t r = 0; r < s->n_rows; r++)
-      std::memcpy(cols[c].data() + r * w, base[c] + r * s->aligned_tuple_size, s->col_size[c]);
-    cb[c] = cols[c].data();
-    cs[c] = (int64_t)w;
+  temps.dev.push_back({d_rows, bytes});
+  parallel_for((int64_t)bytes, 1 << 20, [&](int64_t b, int64_t e) {
+    std::memcpy(h_rows + b, s->data + b, (size_t)(e - b));
+  });
+  trace.mark("stage");
+  std::unique_ptr<hb_table> t;
+  hipError_t e = hipMemcpyAsync(d_rows, h_rows, bytes, hipMemcpyHostToDevice, ctx->copy_stream);
+  if (e == hipSuccess) st = unpack_table(ctx, s, d_rows, ctx->copy_stream, t);
+  if (e != hipSuccess || st != HB_OK) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    for (auto &m : temps.host) ctx->hpool.free(m.first, m.second);
+    for (auto &m : temps.dev) ctx->pool.free(m.first, m.second);
+    return e != hipSuccess ? fail(HB_ERR_HIP, "table upload: %s", hipGetErrorString(e)) : st;
   }
-  hb_table_view v{(int64_t)s->n_rows, (int32_t)s->n_cols, cb.data(), cs.data()};
-  return hb_table_upload(ctx, &v, out);
+  HB_TRY(retire_after(ctx, std::move(temps), ctx->copy_stream));
+  trace.mark("copy+unpack");
+  *out = t.release();
+  return HB_OK;
+}
+
+hb_status hb_table_from_device(hb_ctx *ctx, const hb_serialized_table *s, hb_table **out) {
+  if (!ctx || !out) return fail(HB_ERR_INVALID_ARG, "hb_table_from_device: null argument");
+  *out = nullptr;
+  HB_TRY(check_serialized(s));
+  HB_TRY(set_device(ctx));
+  std::unique_ptr<hb_table> t;
+  HB_TRY(unpack_table(ctx, s, (const uint8_t *)s->data, ctx->copy_stream, t));
+  *out = t.release();
+  return HB_OK;
 }
 
 hb_status hb_table_info(const hb_table *t, int64_t *n_rows, int32_t *n_cols) {
@@ -2340,7 +2547,6 @@ hb_status hb_table_slice(hb_ctx *ctx, hb_table *parent, int64_t row_begin, int64
     Column &col = t->cols[c];
     col.width = pc.width;
     col.d_bytes = pc.d_bytes + (size_t)off * pc.width;
-    col.h_bytes = pc.h_bytes + (size_t)off * pc.width;
   }
   root->slices++;
   *out = t.release();

@@ -161,8 +161,23 @@ const char *hb_version(void);
  * copy on the device.  A table may be shared by any number of jobs and
  * stays resident until released (releasing waits for those jobs). */
 hb_status hb_table_upload(hb_ctx *ctx, const hb_table_view *view, hb_table **out);
+/* The partitioner's pinned SerializedTable (core/data_structures/table.h:16-51,
+ * filled by Serialize2PinnedMemory, :75-128), as AsyncSubmit hands it over
+ * (host_producer.h:137-152): the rows are copied as they are (one parallel
+ * copy into staging, one host-to-device copy) and unpacked into columns on
+ * the device.  Cell (r, c) ends at its first NUL or after col_size[c] bytes. */
 hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *t,
                                      hb_table **out);
+/* The reference's device-side table as blocking_kernel receives it
+ * (d_tb_data with its col_size / col_offset, core/gpu/global_func.cuh:104-110,
+ * after AsyncSubmit's host-to-device copy, host_producer.h:137-152): `data`
+ * is a device pointer to n_rows rows of aligned_tuple_size bytes on this
+ * context's device; col_size and col_offset are host arrays.  The cells are
+ * unpacked into a new table on the device (no host copy, no host pass over
+ * the rows).  The call returns before the device has read `data`: keep it
+ * valid and unchanged until a job on the table has completed (or
+ * hb_ctx_synchronize has returned). */
+hb_status hb_table_from_device(hb_ctx *ctx, const hb_serialized_table *t, hb_table **out);
 /* Uploads rows[0..n_rows) of the view, in that order, as a new table: the
  * per-partition upload of the streamed form (the reference re-serialises
  * each partition, core/components/data_mngr.h:209-215, and copies it in

@@ -1,0 +1,163 @@
+"""Tables handed over in the reference's row-major layout, and the device
+dictionary's exact fallback (needs an MI355X).
+
+hb_table_upload_serialized (host rows) and hb_table_from_device (rows
+already in device memory: blocking_kernel's d_tb_data with col_size /
+col_offset, core/gpu/global_func.cuh:104-110) unpack the rows on the
+device.  A cell is its bytes up to the first NUL or col_size bytes (the
+reference's cells are zero padded, core/data_structures/table.h:112-126).
+
+Equality codes come from a dictionary in device memory; rows it cannot
+code exactly (a hash collision, a full table, a full arena) are coded by the
+host.  Contexts created with a hash cut to 3 bits, 16 slots probed twice,
+or no arena force each of those paths: the results stay the oracle's.
+"""
+import ctypes as C
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import hyperblocker_amd as hb
+from hyperblocker_amd import capi, datagen
+from tests import helpers as H
+from tests.test_gpu_parity import PLANS, _adversarial_tables
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hb.Context(0)
+    yield c
+    c.close()
+
+
+class _RawTable:
+    """A row-major table with chosen col_size per column: cells longer than
+    col_size - 1 fill their slot with no NUL, and `junk` bytes are written
+    after the NUL of shorter cells (the reader must stop at the NUL)."""
+
+    def __init__(self, columns, sizes, junk=b""):
+        self.n_rows = len(columns[0])
+        self.n_cols = len(columns)
+        self.col_size = list(sizes)
+        self.col_offset = [int(x) for x in np.concatenate([[0], np.cumsum(sizes)[:-1]])]
+        self.aligned_tuple_size = (((sum(sizes) + 1) >> 6) << 6) + 64
+        ats = self.aligned_tuple_size
+        self.data = np.zeros(max(self.n_rows, 1) * ats, dtype=np.uint8)
+        for c, (col, off, size) in enumerate(zip(columns, self.col_offset, sizes)):
+            for r in range(self.n_rows):
+                v = bytes(col[r])
+                assert len(v) <= size
+                cell = v + (b"\0" + junk if len(v) < size else b"")
+                cell = cell[:size]
+                self.data[r * ats + off: r * ats + off + len(cell)] = np.frombuffer(cell, np.uint8)
+        self._size = (C.c_size_t * self.n_cols)(*self.col_size)
+        self._offset = (C.c_size_t * self.n_cols)(*self.col_offset)
+        self.view = capi.SerializedTable(self.n_rows, self.n_cols, ats, self._size, self._offset,
+                                         self.data.ctypes.data)
+
+
+def test_from_device_reference_layout(ctx):
+    """dblp-acm in the reference layout, rows in device memory: the
+    oracle's committed pairs and audit entries."""
+    g = json.load(open(os.path.join(H.GOLDEN, "dblp_acm.json")))
+    Lh, Rh, plan = H.load_fixture("dblp_acm")
+    sl = hb.DeviceSerializedTable(hb.SerializedTable(H.host_columns(Lh)))
+    sr = hb.DeviceSerializedTable(hb.SerializedTable(H.host_columns(Rh)))
+    dp = ctx.plan(plan)
+    try:
+        tl, tr = ctx.from_device(sl), ctx.from_device(sr)
+        got, audit, stats = ctx.block(dp, tl, tr)
+        tl.release()
+        tr.release()
+        assert hb.sort_pairs(got).tolist() == g["pairs"]
+        H.assert_same_audit(audit, H.audit_from_json(g["audit_entries"]), "from device")
+    finally:
+        dp.release()
+        sl.free()
+        sr.free()
+
+
+@pytest.mark.parametrize("junk", [b"", b"zz\x01garbage"])
+def test_unpack_cells_ending_at_nul_or_col_size(ctx, junk):
+    """Cells filling their col_size with no NUL, junk after a NUL, empty
+    cells and bytes above 127, through both row-major entry points: the
+    pairs and audit entries of the oracle on the cells as defined."""
+    rng = random.Random(7)
+    alpha = b"ab\xe9c"
+    rows = []
+    for i in range(300):
+        rows.append((str(i).encode(), bytes(rng.choice(alpha) for _ in range(rng.randint(0, 12))),
+                     bytes(rng.choice(alpha) for _ in range(rng.randint(0, 7))),
+                     str(rng.randrange(3)).encode()))
+    cols = H.columns_from_rows(rows)
+    sizes = [4, 12, 8, 1]  # ids up to 3 digits + NUL; title cells of 12 bytes fill their slot
+    raw = _RawTable(cols, sizes, junk)
+    plan = hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.6), (2, 0.5)]),
+                                  datagen.rule_yaml([], [(2, 0.9)])])
+    want, want_audit, _ = H.run_oracle(plan, cols, cols)
+    dp = ctx.plan(plan)
+    dev = hb.DeviceSerializedTable(raw)
+    try:
+        for how in ("upload", "device"):
+            t = ctx.upload_serialized(raw) if how == "upload" else ctx.from_device(dev)
+            got, audit, _ = ctx.block(dp, t, t)
+            t.release()
+            H.assert_same_pairs(got, want, how)
+            H.assert_same_audit(H.sort_audit(audit), want_audit, how)
+    finally:
+        dp.release()
+        dev.free()
+
+
+def test_serialized_table_rejects_bad_layout(ctx):
+    raw = _RawTable(H.columns_from_rows([(b"1", b"ab")]), [2, 3])
+    raw._size[1] = 200  # past aligned_tuple_size
+    with pytest.raises(hb.HBError) as e:
+        ctx.upload_serialized(raw)
+    assert e.value.status == capi.HB_ERR_INVALID_ARG
+
+
+def _ctx_with(env):
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return hb.Context(0)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("env", [
+    {"HB_DICT_HASH_BITS": "3"},                          # collisions everywhere
+    {"HB_DICT_SLOTS": "4", "HB_DICT_MAX_PROBE": "2"},    # a full table
+    {"HB_DICT_ARENA_MB": "0"},                           # no room for representatives
+])
+def test_dictionary_fallback_stays_exact(env):
+    """Join keys of 2,000 values and composite keys, over two table pairs
+    in one context (codes must agree across tables, whichever dictionary
+    gave them): pairs and audit entries equal the oracle's."""
+    c = _ctx_with(env)
+    try:
+        for seed in (91, 92):
+            L, R = _adversarial_tables(seed, 1500, 1300, b"abc", 16, n_keys=2000)
+            tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
+            for name in ("eq_only", "eq_sim", "two_eq_two_sim", "multi_rule"):
+                plan = hb.HostPlan.from_yaml(PLANS[name])
+                dp = c.plan(plan)
+                got, audit, _ = c.block(dp, tl, tr)
+                dp.release()
+                want, want_audit, _ = H.run_oracle(plan, L, R)
+                H.assert_same_pairs(got, want, "%s, %s, seed %d" % (env, name, seed))
+                H.assert_same_audit(H.sort_audit(audit), want_audit, "%s, %s" % (env, name))
+            tl.release()
+            tr.release()
+    finally:
+        c.close()
