@@ -84,8 +84,16 @@ struct SideDesc {
   const SigRow *jsig_fb;
   const uint32_t *run_code;
   const uint32_t *run_off;  // n_runs + 1 entries
+  // Runs on the host side: the count, or (a layout still being built) an
+  // upper bound of it that sizes the JR's groups; the count itself is at
+  // *n_runs_dev (the join leaves groups past it empty).
   uint32_t n_runs;
   uint32_t pad;
+  const uint32_t *n_runs_dev;
+  // Set by the layout's build when a row's code was not below the code
+  // count the build assumed (a value coded since, or one the host codes):
+  // the layout is then rebuilt and the batch run again.
+  const uint32_t *bad;
 };
 
 struct JRDesc {
@@ -154,6 +162,7 @@ struct BatchDev {
   JobSets sets;
   unsigned long long *overflow;  // set to 1 when any list or set is full
   unsigned long long *set_overflow;  // set to 1 when a job's dedupe set is full
+  unsigned long long *invalid;   // a JR's layout is flagged bad (SideDesc::bad)
 };
 
 // Join groups of a batch: group gi pairs left positions [l_off, l_off + l_cnt)
@@ -204,8 +213,9 @@ struct AuditList {
 // Dictionary of a context in device memory: equality columns (and the code
 // tuples of composite keys) are coded here, so that equal codes mean equal
 // byte strings across every table of the context.  Open addressing keyed by
-// a 64-bit hash of the cell; the code of a value is its slot index.  The
-// first row to claim a slot copies its bytes to the arena as the slot's
+// a 64-bit hash of the cell; the first row to claim a slot gives the value
+// the next code (codes are dense, so join keys stay narrow) and copies its
+// bytes to the arena as the slot's
 // representative, and every row is then compared byte for byte with the
 // representative of its slot: a row that differs (a hash collision), or that
 // found no slot within max_probe probes, or whose slot has no representative
@@ -214,6 +224,8 @@ struct AuditList {
 struct DictDev {
   unsigned long long *keys;        // slots: hash, or kDictEmpty
   unsigned long long *reps;        // slots: arena offset << 16 | chunks, or kDictNoRep
+  uint32_t *slot_code;             // slots: the value's code (dense: in order of first sight)
+  uint32_t *n_codes;               // codes handed out
   uint8_t *arena;
   unsigned long long *arena_used;  // bytes handed out (may exceed arena_cap)
   unsigned long long arena_cap;
@@ -257,15 +269,28 @@ void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t w
                          hipStream_t stream);
 // Unpacks n_rows rows of row_bytes bytes (the reference's row-major table on
 // the device) into the columns' NUL-padded cells: each cell is copied up to
-// its first NUL or `size` bytes, the rest of its width zeroed.
+// its first NUL or `size` bytes, the rest of its width zeroed.  Work-groups
+// stage rows_per_group rows at a time in LDS (unpack_lds_bytes of their
+// span, at most kUnpackLds).
+constexpr uint32_t kUnpackLds = 64 * 1024;
+inline size_t unpack_lds_bytes(size_t span, size_t cells) {
+  const size_t rows_room = ((span + 31) & ~size_t(31)) + 32;
+  return rows_room + ((span + 31) / 32 + 1) * 4 + cells * 2;
+}
 void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,
-                        const UnpackColumn *cols, uint32_t n_cols, hipStream_t stream);
+                        const UnpackColumn *cols, uint32_t n_cols, uint32_t rows_per_group,
+                        int n_cu, hipStream_t stream);
 void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint16_t *len,
                       SigRow *sig, hipStream_t stream);
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
                          uint32_t *planes, hipStream_t stream);
 void launch_build_hist(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t *hist,
                        hipStream_t stream);
+// The three above in one pass over the cells (what the runtime uses; the
+// separate kernels stay for the tests that compare them).
+void launch_build_col(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
+                      uint16_t *len, SigRow *sig, uint32_t *hist, uint32_t *planes,
+                      hipStream_t stream);
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long long v,
                      hipStream_t stream);
@@ -276,12 +301,34 @@ void launch_allow_table(float c0, float c3, uint32_t *tab, hipStream_t stream);
 // Join order of a table for one rule shape: keys (code << 8) | min(length,
 // 255) (code or length may be absent: null pointer), sorted with the row ids;
 // then the runs of equal code (key >> 8): flags, exclusive scan, run arrays.
+// A code not below code_bound sets *bad (the row then counts as code 0).
 void launch_layout_keys_fb(const uint32_t *code, const SigRow *sig, const uint16_t *len,
-                           uint32_t n, uint64_t *keys, hipStream_t stream);
-void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,
-                        hipStream_t stream);
+                           uint32_t n, uint64_t code_bound, uint32_t *bad, uint64_t *keys,
+                           hipStream_t stream);
+void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n,
+                        uint64_t code_bound, uint32_t *bad, uint64_t *keys, hipStream_t stream);
 void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,
                        hipStream_t stream);
+// The whole layout of a table whose join keys take at most kLayoutMaxCodes
+// codes, by counting sorts: join order (perm, jsig) by (code, length), the
+// first-byte order (perm_fb, jsig_fb; with a similarity column) by (code,
+// first byte), the runs of equal code and their count.  hist (2 * n_codes *
+// 256 counters) must be zero; rank holds 2 * n entries, off 2 * n_codes *
+// 256.  Within a bucket rows come in no particular order.
+constexpr uint32_t kLayoutMaxCodes = 64;
+constexpr uint32_t kLayoutLdsCodes = 32;  // counts taken in LDS (64 KiB) up to this many codes
+struct LayoutSmall {
+  const uint32_t *code;  // null: a rule without equalities (one code, 0)
+  uint32_t *bad;         // set when a code is not below n_codes (the row then counts as code 0)
+  const uint16_t *len;   // null: no similarity column
+  const SigRow *sig;     // null: no similarity column
+  uint32_t n, n_codes;
+  uint32_t *hist, *rank, *off;
+  uint32_t *perm, *perm_fb;
+  SigRow *jsig, *jsig_fb;
+  uint32_t *run_code, *run_off, *n_runs;
+};
+void launch_layout_small(const LayoutSmall &a, hipStream_t stream);
 void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
                       hipStream_t stream);
 // run_code / run_off of every run start (flags, run_id = exclusive scan of

@@ -390,6 +390,114 @@ __global__ void k_build_planes(const uint8_t *__restrict__ bytes, uint32_t width
   }
 }
 
+// 8 x 8 bit transpose (Hacker's Delight, transpose8): bit j of byte i moves
+// to bit i of byte j.  For 8 consecutive cell bytes, byte k of the result
+// holds bit k of each of them: 8 bits of bit-plane k.
+__device__ __forceinline__ uint64_t transpose8(uint64_t x) {
+  uint64_t t;
+  t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+  x = x ^ t ^ (t << 7);
+  t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+  x = x ^ t ^ (t << 14);
+  t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+  x = x ^ t ^ (t << 28);
+  return x;
+}
+
+// Number of non-zero bytes of a dword.
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
+  const uint32_t y = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;  // high bit set where the low 7 bits are not 0
+  return (uint32_t)__popc((y | x) & 0x80808080u);
+}
+
+// Every per-row structure of a similarity column from one read of its
+// cells, four lanes per row: lane q takes the row's 32-byte blocks q, q + 4,
+// ...  Bit-planes of a block: four 8 x 8 bit transposes.  Byte classes: one
+// LDS atomic add per byte into the row's 128 counters (byte & 127, u8 each;
+// zero bytes, which follow the cell's NUL, add nothing), from which the
+// signature's 32 counters (byte & 31) are sums of four.  The length counts
+// the non-zero bytes (every byte after the NUL is zero).  Results equal
+// those of k_build_sig, k_build_hist and k_build_planes.
+__global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ bytes,
+                                                  uint32_t width, int64_t n, uint32_t nwc,
+                                                  uint16_t *__restrict__ len_out,
+                                                  SigRow *__restrict__ sig_out,
+                                                  uint32_t *__restrict__ hist,
+                                                  uint32_t *__restrict__ planes) {
+  __shared__ uint32_t s_h[kHistWords][64];  // [class word][row of the work-group]
+  const uint32_t t = threadIdx.x, q = t & 3u, rl = t >> 2;
+  const int64_t r = (int64_t)blockIdx.x * 64 + rl;
+  const bool valid = r < n;
+  for (uint32_t i = t; i < kHistWords * 64; i += 256) (&s_h[0][0])[i] = 0;
+  __syncthreads();
+  uint32_t L = 0, pre = 0;
+  if (valid) {
+    const uint8_t *cell = bytes + r * (int64_t)width;
+    uint32_t *pl = planes + r * (int64_t)nwc * 8;
+    const uint32_t n32 = (width + 31u) >> 5;
+    for (uint32_t blk = q; blk < n32; blk += 4) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(cell + 32 * blk);
+      const uint4 a = src[0];
+      const uint4 b = 32 * blk + 16 < width ? src[1] : make_uint4(0u, 0u, 0u, 0u);
+      if (blk == 0) pre = a.x;
+      const uint32_t w8[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int d = 0; d < 8; d++) {
+        L += nonzero_bytes(w8[d]);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const uint32_t c = (w8[d] >> (8 * e)) & 0xffu;
+          const uint32_t inc = c ? 1u << ((c & 3u) * 8u) : 0u;
+          atomicAdd(&s_h[(c & 127u) >> 2][rl], inc);
+        }
+      }
+      if (blk < nwc) {
+        uint64_t X[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++) X[g] = transpose8(((uint64_t)w8[2 * g + 1] << 32) | w8[2 * g]);
+        uint32_t P[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const uint32_t sh = 8 * k;
+          P[k] = (uint32_t)((X[0] >> sh) & 0xffu) | ((uint32_t)((X[1] >> sh) & 0xffu) << 8) |
+                 ((uint32_t)((X[2] >> sh) & 0xffu) << 16) | ((uint32_t)((X[3] >> sh) & 0xffu) << 24);
+        }
+        uint4 *d4 = reinterpret_cast<uint4 *>(pl + 8 * blk);
+        d4[0] = make_uint4(P[0], P[1], P[2], P[3]);
+        d4[1] = make_uint4(P[4], P[5], P[6], P[7]);
+      }
+    }
+  }
+  // sums over the row's four lanes (adjacent in the wave)
+#pragma unroll
+  for (int m = 1; m <= 2; m <<= 1) {
+    L += (uint32_t)__shfl_xor((int)L, m);
+    pre |= (uint32_t)__shfl_xor((int)pre, m);  // only lane 0 of the row holds it
+  }
+  __syncthreads();
+  if (!valid) return;
+  {  // lane q writes class words 8q .. 8q + 7 of the row
+    uint32_t o[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = s_h[8 * q + j][rl];
+    uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords + 8 * q);
+    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+  }
+  if (q == 0) {
+    len_out[r] = (uint16_t)min(L, 65535u);
+    SigRow sg;
+#pragma unroll
+    for (int j = 0; j < 8; j++)  // classes 4j .. 4j+3 of 32 = four of the 128 each
+      sg.h[j] = s_h[j][rl] + s_h[j + 8][rl] + s_h[j + 16][rl] + s_h[j + 24][rl];
+    sg.pre = pre;
+    sg.len = L | (L > 255u ? kLongFlag : 0u);
+    sg.rlen = (L >= 1u && L <= 255u) ? 1.0f / (float)L : INFINITY;
+    sg.pad = 0;
+    sig_out[r] = sg;
+  }
+}
+
 __global__ void k_iota(uint32_t *out, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = i;
@@ -459,7 +567,8 @@ __device__ __forceinline__ uint64_t dict_hash_cell(const uint4 *cell, uint32_t w
   return h == kDictEmpty ? kDictEmpty - 1ull : h;
 }
 
-// One thread per row: find or claim the slot of the cell's hash.
+// One thread per row: find or claim the slot of the cell's hash; `code`
+// holds the slot until k_dict_verify replaces it by the value's code.
 __global__ void k_dict_insert(const uint8_t *__restrict__ cells, uint32_t width,
                               uint32_t fixed_len, uint32_t n, DictDev d,
                               uint32_t *__restrict__ code) {
@@ -485,6 +594,7 @@ __global__ void k_dict_insert(const uint8_t *__restrict__ cells, uint32_t width,
           rep = (off << 16) | chunks;
         }
         d.reps[slot] = rep;
+        d.slot_code[slot] = atomicAdd(d.n_codes, 1u);
         c = slot;
         break;
       }
@@ -530,7 +640,9 @@ __global__ void k_dict_verify(const uint8_t *__restrict__ cells, uint32_t width,
       }
     }
   }
-  if (!ok) {
+  if (ok) {
+    code[r] = d.slot_code[c];
+  } else {
     code[r] = kNoCode;
     fb_rows[atomicAdd(fb_count, 1ull)] = r;
   }
@@ -554,42 +666,105 @@ __global__ void k_compose_keys(KeyColumns cols, uint32_t n, uint8_t *__restrict_
   for (; w < (width >> 2); w++) dst[w] = 0u;
 }
 
-// One wave per row (grid-stride): the row's cells are read byte by byte
-// (cells of the reference's layout are not aligned), lane j holding bytes
-// 4j .. 4j+3 of a 256-byte block of the cell, and written as aligned dwords
-// with every byte from the first NUL on set to zero.
+// A work-group unpacks blocks of rows_per_group consecutive rows (grid
+// stride).  A block is one contiguous span of the input: it is loaded into
+// LDS (16-byte loads when the rows are 16-byte aligned), each cell's length
+// (up to its first NUL, at most `size`) is found there, and each column's
+// cells of the block, one contiguous span of that column, are written as
+// dwords with every byte from the cell's end on zero.
+template <bool ALIGNED16>
 __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__ rows,
                                                     uint64_t row_bytes, uint32_t n_rows,
                                                     const UnpackColumn *__restrict__ cols,
-                                                    uint32_t n_cols) {
-  const uint32_t lane = lane_id();
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t r = wave; r < n_rows; r += n_waves) {
-    const uint8_t *row = rows + (size_t)r * row_bytes;
-    for (uint32_t c = 0; c < n_cols; c++) {
-      const UnpackColumn col = cols[c];
-      const uint8_t *src = row + col.offset;
-      uint32_t *dst = reinterpret_cast<uint32_t *>(col.out + (size_t)r * col.width);
-      bool ended = false;  // wave-uniform: a NUL was seen in an earlier block
-      for (uint32_t base = 0; base < col.width; base += 256) {
-        const uint32_t p0 = base + 4 * lane;
-        uint32_t v = 0;
-        if (!ended) {
+                                                    uint32_t n_cols, uint32_t rows_per_group) {
+  extern __shared__ uint4 s_mem4[];
+  uint8_t *s_rows = reinterpret_cast<uint8_t *>(s_mem4);
+  const uint32_t *s_rows32 = reinterpret_cast<const uint32_t *>(s_mem4);
+  const uint32_t span_max = (uint32_t)(rows_per_group * row_bytes);
+  // staged rows (plus 32 bytes read past the end); one bit per staged byte,
+  // set where the byte is zero; the length of every cell of the block
+  const uint32_t rows_room = ((span_max + 31u) & ~31u) + 32u;
+  uint32_t *s_zero = reinterpret_cast<uint32_t *>(s_rows + rows_room);
+  uint8_t *s_zero8 = reinterpret_cast<uint8_t *>(s_zero);
+  uint16_t *s_len = reinterpret_cast<uint16_t *>(s_zero + (span_max + 31u) / 32u + 1u);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n_blocks = (n_rows + rows_per_group - 1) / rows_per_group;
+  for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const uint32_t r0 = blk * rows_per_group;
+    const uint32_t nr = min(rows_per_group, n_rows - r0);
+    const uint32_t span = (uint32_t)(nr * row_bytes);
+    const uint8_t *src = rows + (size_t)r0 * row_bytes;
+    __syncthreads();  // the previous block's reads of LDS are done
+    if (ALIGNED16) {
+      const uint4 *src4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll 8
+      for (uint32_t i = tid; i < span / 16; i += 256) s_mem4[i] = src4[i];
+    } else {
+      for (uint32_t i = tid; i < span; i += 256) s_rows[i] = src[i];
+    }
+    // the bytes from span to the next 32-byte boundary read as zero
+    for (uint32_t i = span + tid; i < ((span + 31u) & ~31u) + 32u; i += 256) s_rows[i] = 0;
+    __syncthreads();
+    // zero-byte bitmap: each thread takes 8 bytes (one 8-bit piece)
+    for (uint32_t i = tid; i < ((span + 31u) / 32u) * 4u; i += 256) {  // to the word's end
+      const uint2 x = reinterpret_cast<const uint2 *>(s_rows32)[i];
+      uint32_t z = 0;
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (p0 + q < col.size) v |= (uint32_t)src[p0 + q] << (8 * q);
+      for (int h = 0; h < 2; h++) {
+        const uint32_t v = h ? x.y : x.x;
+        const uint32_t m = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // zero bytes
+        z |= (((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u)) << (4 * h);
+      }
+      s_zero8[i] = (uint8_t)z;
+    }
+    __syncthreads();
+    // each cell's length: its first zero byte, at most `size`
+    for (uint32_t i = tid; i < nr * n_cols; i += 256) {
+      const uint32_t j = i / n_cols, c = i - j * n_cols;
+      const uint32_t start = j * (uint32_t)row_bytes + (uint32_t)cols[c].offset;
+      const uint32_t size = cols[c].size;
+      uint32_t len = size;
+      for (uint32_t w = start >> 5; (w << 5) < start + size; w++) {
+        uint32_t z = s_zero[w];
+        if ((w << 5) < start) z &= ~0u << (start & 31);
+        if (z) {
+          len = min(size, (w << 5) + (uint32_t)__builtin_ctz(z) - start);
+          break;
         }
-        // bytes past `size` read as 0, so a cell of exactly `size` bytes ends there
-        const uint32_t z = first_zero_byte(v);
-        const uint64_t zmask = __ballot(z < 4 && p0 < col.width);
-        if (zmask) {
-          const uint32_t first = (uint32_t)__builtin_ctzll(zmask);
-          if (lane > first) v = 0;
-          else if (lane == first) v &= z == 0 ? 0u : (0xffffffffu >> (32 - 8 * z));
-          ended = true;
+      }
+      s_len[i] = (uint16_t)len;
+    }
+    __syncthreads();
+    for (uint32_t c = 0; c < n_cols; c++) {
+      const uint32_t wdw = cols[c].width >> 2;  // dwords per cell
+      const uint32_t off = (uint32_t)cols[c].offset;
+      uint32_t *dst = reinterpret_cast<uint32_t *>(cols[c].out + (size_t)r0 * cols[c].width);
+      // thread t takes dword k of cells j, j + per, ... (one division per column)
+      const uint32_t per = wdw <= 256u ? 256u / wdw : 0u;
+      const uint32_t tj = per ? tid / wdw : 0u, tk = per ? tid - tj * wdw : 0u;
+      for (uint32_t it = 0;; it++) {
+        uint32_t j, k;
+        if (per) {
+          if (tj >= per) break;
+          j = tj + it * per;
+          k = tk;
+          if (j >= nr) break;
+        } else {  // cells wider than 1 KiB
+          const uint32_t i = tid + it * 256u;
+          if (i >= nr * wdw) break;
+          j = i / wdw;
+          k = i - j * wdw;
         }
-        if (p0 < col.width) dst[p0 >> 2] = v;
+        const uint32_t len = s_len[j * n_cols + c];
+        uint32_t v = 0;
+        if (4 * k < len) {
+          const uint32_t pos = j * (uint32_t)row_bytes + off + 4 * k;
+          const uint32_t a = s_rows32[pos >> 2], b = s_rows32[(pos >> 2) + 1];
+          v = __builtin_amdgcn_alignbyte(b, a, pos & 3u);
+          const uint32_t keep = len - 4 * k;  // bytes of this dword inside the cell
+          if (keep < 4) v &= (1u << (8 * keep)) - 1u;
+        }
+        dst[(size_t)j * wdw + k] = v;
       }
     }
   }
@@ -599,23 +774,196 @@ __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__
 // first similarity column (0 without one), or the length alone when the rule
 // has no equality.  Sorting them groups rows by code and orders each group by
 // length.
+__device__ __forceinline__ uint64_t checked_code(const uint32_t *code, uint32_t i,
+                                                 uint64_t bound, uint32_t *bad) {
+  const uint64_t c = code ? (uint64_t)code[i] : 0ull;
+  if (c < bound) return c;
+  *bad = 1u;
+  return 0ull;
+}
+
 __global__ void k_layout_keys(const uint32_t *__restrict__ code, const uint16_t *__restrict__ len,
-                              uint32_t n, uint64_t *__restrict__ keys) {
+                              uint32_t n, uint64_t bound, uint32_t *bad,
+                              uint64_t *__restrict__ keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t c = code ? (uint64_t)code[i] : 0ull;
+  const uint64_t c = checked_code(code, i, bound, bad);
   const uint32_t l = len ? min((uint32_t)len[i], 255u) : 0u;
   keys[i] = (c << 8) | l;
+}
+
+// ---- join layouts by counting sort (keys of few codes) ------------------
+//
+// Buckets (code, min(length, 255)) for the join order and (code, first
+// byte) for the first-byte order, n_codes * 256 of each.  k_lsmall_count
+// takes each row's rank in its two buckets with global atomics (the order
+// within a bucket is arbitrary: its rows share the key the layout orders
+// by), k_lsmall_scan turns the counts into bucket offsets and the runs of
+// equal code, and k_lsmall_scatter places every row and its signature.
+
+__device__ __forceinline__ uint32_t lsmall_code(const LayoutSmall &a, uint32_t i) {
+  const uint32_t c = a.code ? a.code[i] : 0u;
+  if (c < a.n_codes) return c;
+  *a.bad = 1u;
+  return 0u;
+}
+
+// Ranks from global atomics (keys of up to kLayoutMaxCodes codes).
+__global__ void k_lsmall_count(LayoutSmall a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const uint32_t base = lsmall_code(a, i) << 8;
+  const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+  a.rank[i] = atomicAdd(a.hist + base + l, 1u);
+  if (a.sig) {
+    const uint32_t fb = a.sig[i].pre & 0xffu;
+    a.rank[a.n + i] = atomicAdd(a.hist + a.n_codes * 256u + base + fb, 1u);
+  }
+}
+
+// The same with the counts of a work-group's 4096 rows taken in LDS first
+// (keys of up to kLayoutLdsCodes codes): one global atomic per bucket the
+// work-group uses.
+constexpr uint32_t kLayoutRowsPerGroup = 4096;
+__global__ __launch_bounds__(1024) void k_lsmall_count_lds(LayoutSmall a) {
+  extern __shared__ uint32_t s_cnt[];  // buckets of both orders
+  const uint32_t t = threadIdx.x;
+  const uint32_t nb = a.n_codes * 256u, nb2 = a.sig ? 2u * nb : nb;
+  for (uint32_t k = t; k < nb2; k += 1024) s_cnt[k] = 0;
+  __syncthreads();
+  const uint32_t i0 = blockIdx.x * kLayoutRowsPerGroup + t;
+  uint32_t b1[4], b2[4], r1[4], r2[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const uint32_t i = i0 + 1024u * u;
+    b1[u] = b2[u] = 0xffffffffu;
+    if (i < a.n) {
+      const uint32_t base = lsmall_code(a, i) << 8;
+      const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+      b1[u] = base + l;
+      if (a.sig) b2[u] = nb + base + (a.sig[i].pre & 0xffu);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    if (b1[u] != 0xffffffffu) r1[u] = atomicAdd(s_cnt + b1[u], 1u);
+    if (b2[u] != 0xffffffffu) r2[u] = atomicAdd(s_cnt + b2[u], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = t; k < nb2; k += 1024) {  // the work-group's base in each bucket
+    const uint32_t c = s_cnt[k];
+    s_cnt[k] = c ? atomicAdd(a.hist + k, c) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const uint32_t i = i0 + 1024u * u;
+    if (b1[u] != 0xffffffffu) a.rank[i] = s_cnt[b1[u]] + r1[u];
+    if (b2[u] != 0xffffffffu) a.rank[a.n + i] = s_cnt[b2[u]] + r2[u];
+  }
+}
+
+// One work-group of 1024 threads: exclusive scans of the two count arrays
+// (16 buckets per thread, so thread 16 c starts code c), then the runs of
+// the join order, one per code with rows (codes < 64: one wave's ballot).
+__global__ __launch_bounds__(1024) void k_lsmall_scan(LayoutSmall a) {
+  __shared__ uint32_t s_part[1024];
+  __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
+  const uint32_t t = threadIdx.x;
+  const uint32_t nb = a.n_codes * 256u;
+  const uint32_t orders = a.sig ? 2u : 1u;
+  const bool mine = 16u * t < nb;
+  for (uint32_t o = 0; o < orders; o++) {
+    uint32_t v[16];
+    if (mine) {
+      const uint4 *h4 = reinterpret_cast<const uint4 *>(a.hist + o * nb + 16u * t);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint4 x = h4[q];
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        v[4 * q + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; q++) v[q] = 0;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) sum += v[q];
+    s_part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {  // inclusive scan of the partial sums
+      const uint32_t x = t >= d ? s_part[t - d] : 0u;
+      __syncthreads();
+      s_part[t] += x;
+      __syncthreads();
+    }
+    uint32_t run = t ? s_part[t - 1] : 0u;
+    if (o == 0 && mine && (t & 15u) == 0) s_code_start[t >> 4] = run;
+    if (mine) {
+      uint4 *o4 = reinterpret_cast<uint4 *>(a.off + o * nb + 16u * t);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        uint4 x;
+        x.x = run;
+        run += v[4 * q];
+        x.y = run;
+        run += v[4 * q + 1];
+        x.z = run;
+        run += v[4 * q + 2];
+        x.w = run;
+        run += v[4 * q + 3];
+        o4[q] = x;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 64) {
+    const uint32_t c = t;
+    const uint32_t lo = c < a.n_codes ? s_code_start[c] : a.n;
+    const uint32_t hi = c + 1 < a.n_codes ? s_code_start[c + 1] : a.n;
+    const bool has = c < a.n_codes && hi > lo;
+    const uint64_t m = __ballot(has);
+    const uint32_t id = rank_in(m);
+    if (has) {
+      a.run_code[id] = c;
+      a.run_off[id] = lo;
+    }
+    if (t == 0) {
+      const uint32_t nr = (uint32_t)__popcll(m);
+      a.run_off[nr] = a.n;
+      *a.n_runs = nr;
+    }
+  }
+}
+
+__global__ void k_lsmall_scatter(LayoutSmall a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const uint32_t base = lsmall_code(a, i) << 8;
+  const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+  const uint32_t pos = a.off[base + l] + a.rank[i];
+  a.perm[pos] = i;
+  if (a.sig) {
+    const SigRow sg = a.sig[i];
+    a.jsig[pos] = sg;
+    const uint32_t fb = sg.pre & 0xffu;
+    const uint32_t pos2 = a.off[a.n_codes * 256u + base + fb] + a.rank[a.n + i];
+    a.perm_fb[pos2] = i;
+    a.jsig_fb[pos2] = sg;
+  }
 }
 
 // Keys of the first-byte order: (code << 16) | (first byte << 8) |
 // min(length, 255).  Sorted, a join group's rows come ordered by first byte.
 __global__ void k_layout_keys_fb(const uint32_t *__restrict__ code, const SigRow *__restrict__ sig,
-                                 const uint16_t *__restrict__ len, uint32_t n,
-                                 uint64_t *__restrict__ keys) {
+                                 const uint16_t *__restrict__ len, uint32_t n, uint64_t bound,
+                                 uint32_t *bad, uint64_t *__restrict__ keys) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t c = code ? (uint64_t)code[i] : 0ull;
+  const uint64_t c = checked_code(code, i, bound, bad);
   const uint32_t f = sig[i].pre & 0xffu;
   const uint32_t l = min((uint32_t)len[i], 255u);
   keys[i] = (c << 16) | (f << 8) | l;
@@ -710,15 +1058,24 @@ __global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
   const JRDesc &d = b.jrs[jr];
   const uint32_t local = gi - b.jr_group_off[jr];
   const SideDesc &L = d.s[0], &R = d.s[1];
+  if (local == 0 && (*L.bad | *R.bad)) atomicAdd(b.invalid, 1ull);
+  if (local >= *L.n_runs_dev) {  // the host sized this JR's groups by a bound
+    g.l_off[gi] = g.l_cnt[gi] = g.r_off[gi] = g.r_cnt[gi] = 0;
+    g.jr[gi] = jr;
+    g.small_pairs[gi] = 0;
+    g.items[gi] = 0;
+    return;
+  }
   const uint32_t code = L.run_code[local];
   const uint32_t l_off = L.run_off[local], l_cnt = L.run_off[local + 1] - l_off;
-  uint32_t lo = 0, hi = R.n_runs;
+  const uint32_t r_runs = *R.n_runs_dev;
+  uint32_t lo = 0, hi = r_runs;
   while (lo < hi) {
     const uint32_t mid = lo + ((hi - lo) >> 1);
     if (R.run_code[mid] < code) lo = mid + 1; else hi = mid;
   }
   uint32_t r_off = 0, r_cnt = 0;
-  if (lo < R.n_runs && R.run_code[lo] == code) {
+  if (lo < r_runs && R.run_code[lo] == code) {
     r_off = R.run_off[lo];
     r_cnt = R.run_off[lo + 1] - r_off;
   }
@@ -1836,6 +2193,14 @@ void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, u
                      bytes, width, n_rows, nwc, planes);
 }
 
+void launch_build_col(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
+                      uint16_t *len, SigRow *sig, uint32_t *hist, uint32_t *planes,
+                      hipStream_t stream) {
+  if (n_rows <= 0) return;
+  k_build_col<<<(uint32_t)((n_rows + 63) / 64), 256, 0, stream>>>(bytes, width, n_rows, nwc, len,
+                                                                  sig, hist, planes);
+}
+
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream) {
   if (!n) return;
   hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256)), dim3(256), 0, stream, out, n);
@@ -1866,17 +2231,37 @@ void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t w
 }
 
 void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,
-                        const UnpackColumn *cols, uint32_t n_cols, hipStream_t stream) {
+                        const UnpackColumn *cols, uint32_t n_cols, uint32_t rows_per_group,
+                        int n_cu, hipStream_t stream) {
   if (!n_rows || !n_cols) return;
-  const uint32_t blocks = std::min<uint32_t>((n_rows + 3) / 4, 8192);
-  k_unpack_rows<<<blocks, 256, 0, stream>>>(rows, row_bytes, n_rows, cols, n_cols);
+  const uint32_t span = (uint32_t)(rows_per_group * row_bytes);
+  const size_t lds = unpack_lds_bytes(span, (size_t)rows_per_group * n_cols);
+  const uint32_t n_blocks = (n_rows + rows_per_group - 1) / rows_per_group;
+  const uint32_t grid = std::min<uint32_t>(n_blocks, (uint32_t)n_cu * 8);
+  if ((((uintptr_t)rows) & 15u) == 0 && (row_bytes & 15u) == 0)
+    k_unpack_rows<true><<<grid, 256, lds, stream>>>(rows, row_bytes, n_rows, cols, n_cols,
+                                                     rows_per_group);
+  else
+    k_unpack_rows<false><<<grid, 256, lds, stream>>>(rows, row_bytes, n_rows, cols, n_cols,
+                                                      rows_per_group);
 }
 
-void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n, uint64_t *keys,
-                        hipStream_t stream) {
+void launch_layout_small(const LayoutSmall &a, hipStream_t stream) {
+  if (!a.n) return;
+  const uint32_t blocks = (a.n + 255) / 256;
+  if (a.n_codes <= kLayoutLdsCodes)
+    k_lsmall_count_lds<<<(a.n + kLayoutRowsPerGroup - 1) / kLayoutRowsPerGroup, 1024,
+                         (a.sig ? 2 : 1) * a.n_codes * 256 * sizeof(uint32_t), stream>>>(a);
+  else
+    k_lsmall_count<<<blocks, 256, 0, stream>>>(a);
+  k_lsmall_scan<<<1, 1024, 0, stream>>>(a);
+  k_lsmall_scatter<<<blocks, 256, 0, stream>>>(a);
+}
+
+void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n,
+                        uint64_t code_bound, uint32_t *bad, uint64_t *keys, hipStream_t stream) {
   if (!n) return;
-  hipLaunchKernelGGL(k_layout_keys, dim3(grid_for(n, 256)), dim3(256), 0, stream, code, len, n,
-                     keys);
+  k_layout_keys<<<(n + 255) / 256, 256, 0, stream>>>(code, len, n, code_bound, bad, keys);
 }
 
 void launch_allow_table(float c0, float c3, uint32_t *tab, hipStream_t stream) {
@@ -1885,10 +2270,10 @@ void launch_allow_table(float c0, float c3, uint32_t *tab, hipStream_t stream) {
 }
 
 void launch_layout_keys_fb(const uint32_t *code, const SigRow *sig, const uint16_t *len,
-                           uint32_t n, uint64_t *keys, hipStream_t stream) {
+                           uint32_t n, uint64_t code_bound, uint32_t *bad, uint64_t *keys,
+                           hipStream_t stream) {
   if (!n) return;
-  hipLaunchKernelGGL(k_layout_keys_fb, dim3(grid_for(n, 256)), dim3(256), 0, stream, code, sig,
-                     len, n, keys);
+  k_layout_keys_fb<<<(n + 255) / 256, 256, 0, stream>>>(code, sig, len, n, code_bound, bad, keys);
 }
 
 void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigRow *jsig,

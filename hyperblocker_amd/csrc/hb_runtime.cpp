@@ -26,6 +26,8 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <set>
+#include <tuple>
 #include <thread>
 #include <memory>
 #include <mutex>
@@ -576,6 +578,7 @@ struct hb_ctx {
   hb::FilterShape shape{2, 192, 16};  // filter launch shape; HB_FILTER_SHAPE overrides
   hipStream_t copy_stream = nullptr;
   hipStream_t build_stream = nullptr;  // one-off per-table builds that the host waits for
+  hipStream_t code_stream = nullptr;   // dictionary coding (the host waits for each column)
   std::vector<hipStream_t> streams;
   std::atomic<uint64_t> next_stream{0};
   DevicePool pool;
@@ -601,6 +604,9 @@ struct hb_ctx {
   // Test hook: this many batch launches fail before any device work
   // (HB_INJECT_LAUNCH_FAILURES at creation), to exercise the error path.
   std::atomic<int> inject_launch_failures{0};
+  // Test hook: build a similarity column's structures with the three
+  // separate kernels instead of the fused one (HB_SEPARATE_COLUMN_BUILDS).
+  bool separate_col_builds = false;
   // Device dictionary of equality codes (hb::DictDev), created on first use;
   // its kernels run on the build stream, one column at a time.  `dict`
   // above codes the rows it leaves to the host.
@@ -615,6 +621,12 @@ struct hb_ctx {
     uint32_t hash_bits = 64;
     std::atomic<bool> fallback_used{false};  // some code came from the host dictionary
     std::atomic<uint64_t> fallback_rows{0};
+    std::atomic<uint64_t> code_bound{0};     // device codes handed out, as last read
+    // Per column index: codings in a row that added no code.  A column whose
+    // last coding added none is coded without waiting (its layouts check
+    // that every code is below the count already known).
+    std::mutex stable_mu;
+    std::unordered_map<int, int> stable;
   } ddict;
 };
 
@@ -649,6 +661,14 @@ struct Column {
   // equality predicates
   Lazy code;
   uint32_t *d_code = nullptr;
+  // Coding left to finish on the host (a column coded without waiting):
+  // the device's list of rows for the host dictionary, and a pinned copy
+  // of (their count, codes handed out) written when the coding is done.
+  bool validated = false;
+  uint32_t *d_fb_rows = nullptr;
+  unsigned long long *d_fb_count = nullptr;
+  unsigned long long *h_status = nullptr;
+  uint64_t bound_at_launch = 0;  // codes handed out when the coding was launched
 };
 
 // Join key of a table for a list of two or more equality columns: one
@@ -672,10 +692,14 @@ struct Layout {
   hb::SigRow *d_jsig_fb = nullptr;
   uint32_t *d_run_code = nullptr;
   uint32_t *d_run_off = nullptr;  // n_runs + 1
-  // Number of runs: counted on the device by the build and copied to
-  // h_nruns (pinned); -1 until the host has read it there.
+  // Number of runs: counted on the device by the build (d_nruns) and copied
+  // to h_nruns (pinned); -1 until the host has read it there.  runs_bound
+  // bounds it before then (rows, or the codes handed out).
   std::atomic<int64_t> n_runs{-1};
   uint32_t *h_nruns = nullptr;
+  uint32_t *d_nruns = nullptr;
+  uint32_t runs_bound = 0;
+  uint32_t *d_bad = nullptr;  // set by the build when a code was not below its bound
 };
 
 }  // namespace
@@ -685,7 +709,11 @@ struct hb_table {
   int64_t n_rows = 0;
   int32_t n_cols = 0;
   std::vector<Column> cols;
-  std::map<std::pair<std::vector<int>, int>, Layout> layouts;  // (equality columns, similarity column)
+  // (equality columns, similarity column) -> layout; a layout rebuilt after
+  // its codes changed keeps its old object (and memory) in `retired` until
+  // the table is released, for batches that still point to it.
+  std::map<std::pair<std::vector<int>, int>, std::unique_ptr<Layout>> layouts;
+  std::vector<std::unique_ptr<Layout>> retired;
   std::map<std::vector<int>, Composite> composites;             // (equality columns)
   hipEvent_t ready = nullptr;                // upload finished (a slice: its root's)
   mutable std::atomic<bool> ready_done{false};  // `ready` seen complete
@@ -821,7 +849,8 @@ enum : int {
   kCntFinal = 1,
   kCntAudit = 2,
   kCntSetOverflow = 3,
-  kCntFixed = 4,
+  kCntInvalid = 4,  // a layout the batch used is flagged bad: rebuild it and run again
+  kCntFixed = 5,
 };
 
 }  // namespace
@@ -967,6 +996,8 @@ hb_status record_lazy(Lazy &lz, hipStream_t stream) {
   return HB_OK;
 }
 
+hb_status retire_after(hb_ctx *ctx, hb_ctx::Grave &&g, hipStream_t stream);
+
 hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   Column &col = t->cols[c];
   if (t->parent) {  // borrow the parent's signatures at the slice's offset
@@ -992,9 +1023,14 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
   col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
   HB_TRY(t->dev_alloc(n * col.nwc * 32, (void **)&col.d_planes));
   HB_TRY(t->dev_alloc(n * hb::kHistWords * 4, (void **)&col.d_hist));
-  hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
-  hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
-  hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
+  if (t->ctx->separate_col_builds) {
+    hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
+    hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
+    hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
+  } else {
+    hb::launch_build_col(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_len, col.d_sig,
+                         col.d_hist, col.d_planes, stream);
+  }
   HIP_OK(hipGetLastError());
   return record_lazy(col.sig, stream);
 }
@@ -1029,92 +1065,149 @@ hb_status dict_create(hb_ctx *ctx, hipStream_t stream) {
   std::lock_guard<std::mutex> lk(dd.mu);
   if (dd.created) return HB_OK;
   const size_t slot_bytes = dd.slots * 8;
-  const size_t bytes = 2 * slot_bytes + dd.arena_bytes + 256;
+  // keys, reps, codes per slot; counters; arena
+  const size_t bytes = 2 * slot_bytes + dd.slots * 4 + 256 + dd.arena_bytes;
   HIP_OK(hipMalloc(&dd.mem, bytes));
   char *p = (char *)dd.mem;
   dd.dev.keys = (unsigned long long *)p;
   dd.dev.reps = (unsigned long long *)(p + slot_bytes);
-  dd.dev.arena_used = (unsigned long long *)(p + 2 * slot_bytes);
-  dd.dev.arena = (uint8_t *)(p + 2 * slot_bytes + 256);
+  dd.dev.slot_code = (uint32_t *)(p + 2 * slot_bytes);
+  char *counters = p + 2 * slot_bytes + dd.slots * 4;
+  dd.dev.arena_used = (unsigned long long *)counters;
+  dd.dev.n_codes = (uint32_t *)(counters + 64);
+  dd.dev.arena = (uint8_t *)(counters + 256);
   dd.dev.arena_cap = dd.arena_bytes;
   dd.dev.mask = (uint32_t)(dd.slots - 1);
   dd.dev.max_probe = dd.max_probe;
   dd.dev.hash_bits = dd.hash_bits;
   hb::launch_fill_u64(dd.dev.keys, dd.slots, hb::kDictEmpty, stream);
-  hb::launch_fill_u64(dd.dev.arena_used, 1, 0ull, stream);
+  hb::launch_fill_u64((unsigned long long *)counters, 256 / 8, 0ull, stream);
   HIP_OK(hipGetLastError());
   dd.created = true;
   return HB_OK;
 }
 
-// Dictionary codes of n cells (device memory, `width` bytes each; a cell
-// ends at its first NUL, or has fixed_len bytes) into d_code, on `stream`.
-// Waits for the device: rows the device dictionary could not code (hash
-// collisions, a full table or arena) are coded by the host dictionary, from
-// kFallbackCodeBase up, and written back before this returns.
-hb_status dict_code(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t fixed_len,
-                    size_t n, uint32_t *d_code, hipStream_t stream) {
-  if (n == 0) return HB_OK;
+// Launches the dictionary coding of n cells (device memory, `width` bytes
+// each; a cell ends at its first NUL, or has fixed_len bytes) into d_code on
+// `stream`, with the rows left to the host listed in fb_rows (n entries) and
+// counted in *fb_count, and (that count, the codes handed out) copied to the
+// pinned h_status[0..1] when done.  Nothing waits here.
+hb_status dict_launch(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t fixed_len,
+                      size_t n, uint32_t *d_code, uint32_t *fb_rows,
+                      unsigned long long *fb_count, unsigned long long *h_status,
+                      hipStream_t stream) {
   HB_TRY(dict_create(ctx, stream));
-  DevicePool &pool = ctx->pool;
-  uint32_t *fb_rows = nullptr;
-  unsigned long long *fb_count = nullptr, *h_count = nullptr;
-  HB_TRY(pool.alloc(n * sizeof(uint32_t), (void **)&fb_rows));
-  hb_status st = pool.alloc(sizeof(unsigned long long), (void **)&fb_count);
-  if (st == HB_OK) st = ctx->hpool.alloc(sizeof(unsigned long long), (void **)&h_count, nullptr);
-  hipEvent_t ev = nullptr;
-  if (st == HB_OK) st = ctx->events.get(&ev);
-  auto done = [&](hb_status r) {
-    if (ev) ctx->events.put(ev);
-    if (h_count) ctx->hpool.free(h_count, sizeof(unsigned long long));
-    if (fb_count) pool.free(fb_count, sizeof(unsigned long long));
-    pool.free(fb_rows, n * sizeof(uint32_t));
-    return r;
-  };
-  if (st != HB_OK) return done(st);
+  h_status[0] = h_status[1] = 0;
+  if (n == 0) return HB_OK;
   hb::launch_fill_u64(fb_count, 1, 0ull, stream);
   hb::launch_dict_code(cells, width, fixed_len, (uint32_t)n, ctx->ddict.dev, d_code, fb_rows,
                        fb_count, stream);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h_count, fb_count, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                       stream);
-  if (e == hipSuccess) e = hipEventRecord(ev, stream);
-  if (e == hipSuccess) e = hipEventSynchronize(ev);
-  if (e != hipSuccess) return done(fail(HB_ERR_HIP, "dictionary coding: %s", hipGetErrorString(e)));
-  const unsigned long long n_fb = *h_count;
-  if (n_fb) {  // rare: code those rows on the host (exact byte keys)
-    std::vector<uint32_t> rows(n_fb), codes(n);
-    std::vector<char> host_cells(n * (size_t)width);
-    e = hipMemcpy(rows.data(), fb_rows, n_fb * sizeof(uint32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
-      e = hipMemcpy(host_cells.data(), cells, host_cells.size(), hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
-      e = hipMemcpy(codes.data(), d_code, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
-    if (e != hipSuccess)
-      return done(fail(HB_ERR_HIP, "dictionary fallback: %s", hipGetErrorString(e)));
-    for (uint32_t r : rows) {
-      const char *cell = host_cells.data() + (size_t)r * width;
-      const std::string_view v(cell, fixed_len ? fixed_len : strnlen(cell, width));
-      const uint32_t c = ctx->dict.code_one(v, Dictionary::hash(v));
-      if (c >= hb::kFallbackCodeBase)
-        return done(fail(HB_ERR_CAPACITY, "dictionary: more than 2^31 distinct values"));
-      codes[r] = hb::kFallbackCodeBase | c;
-    }
-    e = hipMemcpy(d_code, codes.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e != hipSuccess)
-      return done(fail(HB_ERR_HIP, "dictionary fallback: %s", hipGetErrorString(e)));
-    ctx->ddict.fallback_used.store(true);
-    ctx->ddict.fallback_rows.fetch_add(n_fb);
-  }
-  return done(HB_OK);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(h_status, fb_count, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        stream));
+  HIP_OK(hipMemcpyAsync(h_status + 1, ctx->ddict.dev.n_codes, sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, stream));
+  return HB_OK;
 }
 
-// Dictionary codes of column c (on first use; a slice borrows its root's).
-hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
+// Completes a coding whose h_status has arrived: rows the device dictionary
+// could not code (hash collisions, a full table or arena) are coded by the
+// host dictionary, from kFallbackCodeBase up, and written back (a copy the
+// host waits for); the context's code count is raised to what was read.
+hb_status dict_finish(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t fixed_len,
+                      size_t n, uint32_t *d_code, const uint32_t *fb_rows,
+                      const unsigned long long *h_status) {
+  {  // codes of these cells are below the count read after their kernels
+    const uint64_t nc = h_status[1] & 0xffffffffull;
+    uint64_t cur = ctx->ddict.code_bound.load();
+    while (cur < nc && !ctx->ddict.code_bound.compare_exchange_weak(cur, nc)) {
+    }
+  }
+  const unsigned long long n_fb = h_status[0];
+  if (n_fb == 0) return HB_OK;
+  std::vector<uint32_t> rows(n_fb), codes(n);
+  std::vector<char> host_cells(n * (size_t)width);
+  hipError_t e = hipMemcpy(rows.data(), fb_rows, n_fb * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(host_cells.data(), cells, host_cells.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(codes.data(), d_code, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(HB_ERR_HIP, "dictionary fallback: %s", hipGetErrorString(e));
+  for (uint32_t r : rows) {
+    const char *cell = host_cells.data() + (size_t)r * width;
+    const std::string_view v(cell, fixed_len ? fixed_len : strnlen(cell, width));
+    const uint32_t c = ctx->dict.code_one(v, Dictionary::hash(v));
+    if (c >= hb::kFallbackCodeBase)
+      return fail(HB_ERR_CAPACITY, "dictionary: more than 2^31 distinct values");
+    codes[r] = hb::kFallbackCodeBase | c;
+  }
+  e = hipMemcpy(d_code, codes.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(HB_ERR_HIP, "dictionary fallback: %s", hipGetErrorString(e));
+  ctx->ddict.fallback_used.store(true);
+  ctx->ddict.fallback_rows.fetch_add(n_fb);
+  return HB_OK;
+}
+
+// Codes n cells into d_code and waits: dict_launch + dict_finish with
+// temporaries of its own.
+hb_status dict_code(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t fixed_len,
+                    size_t n, uint32_t *d_code, hipStream_t stream) {
+  if (n == 0) return HB_OK;
+  DevicePool &pool = ctx->pool;
+  uint32_t *fb_rows = nullptr;
+  unsigned long long *fb_count = nullptr, *h_status = nullptr;
+  HB_TRY(pool.alloc(n * sizeof(uint32_t), (void **)&fb_rows));
+  hb_status st = pool.alloc(sizeof(unsigned long long), (void **)&fb_count);
+  if (st == HB_OK) st = ctx->hpool.alloc(2 * sizeof(unsigned long long), (void **)&h_status, nullptr);
+  hipEvent_t ev = nullptr;
+  if (st == HB_OK) st = ctx->events.get(&ev);
+  if (st == HB_OK) st = dict_launch(ctx, cells, width, fixed_len, n, d_code, fb_rows, fb_count,
+                                    h_status, stream);
+  if (st == HB_OK) {
+    hipError_t e = hipEventRecord(ev, stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev);
+    st = e == hipSuccess ? HB_OK : fail(HB_ERR_HIP, "dictionary coding: %s", hipGetErrorString(e));
+  }
+  if (st == HB_OK) st = dict_finish(ctx, cells, width, fixed_len, n, d_code, fb_rows, h_status);
+  if (ev) ctx->events.put(ev);
+  if (h_status) ctx->hpool.free(h_status, 2 * sizeof(unsigned long long));
+  if (fb_count) pool.free(fb_count, sizeof(unsigned long long));
+  pool.free(fb_rows, n * sizeof(uint32_t));
+  return st;
+}
+
+// Finishes the coding of column c of a root table (waiting for it if it
+// still runs) and updates the column's prediction.  Called with t->mu held.
+hb_status validate_code_locked(hb_table *t, int c) {
+  Column &col = t->cols[c];
+  if (col.validated || !col.code.built) return HB_OK;
+  HIP_OK(hipEventSynchronize(col.code.ev));
+  HB_TRY(dict_finish(t->ctx, col.d_bytes, col.width, 0, (size_t)t->n_rows, col.d_code,
+                     col.d_fb_rows, col.h_status));
+  col.validated = true;
+  const bool added = (col.h_status[1] & 0xffffffffull) > col.bound_at_launch ||
+                     col.h_status[0] != 0;
+  std::lock_guard<std::mutex> lk(t->ctx->ddict.stable_mu);
+  int &st = t->ctx->ddict.stable[c];
+  st = added ? 0 : st + 1;
+  return HB_OK;
+}
+
+hb_status validate_code(hb_table *t, int c) {
+  while (t->parent) t = t->parent;
+  std::lock_guard<std::mutex> lk(t->mu);
+  return validate_code_locked(t, c);
+}
+
+// Dictionary codes of column c (on first use; a slice borrows its root's),
+// coded on the code stream.  When the column index's last coding added no
+// code and `may_defer` is set, the host does not wait: `stream` waits for
+// the codes on the device, and the column is validated later (layouts built
+// from it check every code against the count then known).  Otherwise the
+// host waits and the codes are final when this returns.
+hb_status ensure_code(hb_table *t, int c, hipStream_t stream, bool may_defer) {
   Column &col = t->cols[c];
   if (t->parent) {  // borrow the parent's codes at the slice's offset
-    HB_TRY(ensure_code(t->parent, c, stream));
+    HB_TRY(ensure_code(t->parent, c, stream, may_defer));
     const Column &pc = t->parent->cols[c];
     std::lock_guard<std::mutex> lk(t->mu);
     if (!col.code.built) {
@@ -1124,11 +1217,31 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
     return HB_OK;
   }
   std::lock_guard<std::mutex> lk(t->mu);
-  if (col.code.built) return wait_lazy(col.code, stream);
+  if (col.code.built) {
+    if (!may_defer) HB_TRY(validate_code_locked(t, c));
+    return wait_lazy(col.code, stream);
+  }
+  hb_ctx *ctx = t->ctx;
   const size_t n = (size_t)t->n_rows;
-  HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&col.d_code));
-  HB_TRY(dict_code(t->ctx, col.d_bytes, col.width, 0, n, col.d_code, stream));
-  return record_lazy(col.code, stream);
+  const size_t n1 = std::max<size_t>(n, 1);
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&col.d_code));
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&col.d_fb_rows));
+  HB_TRY(t->dev_alloc(sizeof(unsigned long long), (void **)&col.d_fb_count));
+  HB_TRY(t->host_alloc(2 * sizeof(unsigned long long), (void **)&col.h_status));
+  bool defer = false;
+  if (may_defer) {
+    std::lock_guard<std::mutex> sl(ctx->ddict.stable_mu);
+    auto it = ctx->ddict.stable.find(c);
+    defer = it != ctx->ddict.stable.end() && it->second > 0;
+  }
+  const hipStream_t cs = ctx->code_stream;
+  HB_TRY(wait_ready(t, cs));
+  col.bound_at_launch = ctx->ddict.code_bound.load();
+  HB_TRY(dict_launch(ctx, col.d_bytes, col.width, 0, n, col.d_code, col.d_fb_rows,
+                     col.d_fb_count, col.h_status, cs));
+  HB_TRY(record_lazy(col.code, cs));
+  if (!defer) HB_TRY(validate_code_locked(t, c));
+  return wait_lazy(col.code, stream);
 }
 
 // Join key codes of the equality columns `cols`: one column's codes, or for
@@ -1137,7 +1250,7 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream) {
 hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stream,
                      const uint32_t **out) {
   if (cols.size() == 1) {
-    HB_TRY(ensure_code(t, cols[0], stream));
+    HB_TRY(ensure_code(t, cols[0], stream, true));
     *out = t->cols[cols[0]].d_code;
     return HB_OK;
   }
@@ -1150,7 +1263,8 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
     *out = pc + t->row_off;
     return HB_OK;
   }
-  for (int c : cols) HB_TRY(ensure_code(t, c, stream));
+  // the tuple of column codes is coded: they must be final first
+  for (int c : cols) HB_TRY(ensure_code(t, c, t->ctx->code_stream, false));
   std::lock_guard<std::mutex> lk(t->mu);
   Composite &cp = t->composites[cols];
   if (cp.code.built) {
@@ -1166,20 +1280,32 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
     kc.k = (uint32_t)cols.size();
     for (size_t q = 0; q < cols.size(); q++) kc.code[q] = t->cols[cols[q]].d_code;
     uint8_t *cells = nullptr;
+    const hipStream_t cs = t->ctx->code_stream;
     HB_TRY(t->ctx->pool.alloc(n * width, (void **)&cells));
-    hb::launch_compose_keys(kc, (uint32_t)n, cells, width, stream);
-    const hb_status st = dict_code(t->ctx, cells, width, key_len, n, cp.d_code, stream);
+    hb::launch_compose_keys(kc, (uint32_t)n, cells, width, cs);
+    const hb_status st = dict_code(t->ctx, cells, width, key_len, n, cp.d_code, cs);
     t->ctx->pool.free(cells, n * width);  // dict_code waited for the device
     HB_TRY(st);
   }
   *out = cp.d_code;
-  return record_lazy(cp.code, stream);
+  return record_lazy(cp.code, t->ctx->code_stream);
 }
 
-// Runs of a layout: read from the device once its build is done.
-hb_status layout_runs(Layout *ly, uint32_t *out) {
+// Runs of a layout for sizing a batch's groups: the count once its build is
+// seen done; while it runs, its bound (rows, or codes handed out) if
+// *budget still holds it (the batch then reserves groups that stay empty),
+// else the count after waiting for the build.
+hb_status layout_runs(Layout *ly, uint32_t *out, uint64_t *budget) {
   int64_t v = ly->n_runs.load(std::memory_order_acquire);
   if (v < 0) {
+    const hipError_t q = hipEventQuery(ly->lz.ev);
+    if (q == hipErrorNotReady && *budget >= ly->runs_bound) {
+      *budget -= ly->runs_bound;
+      *out = ly->runs_bound;
+      return HB_OK;
+    }
+    if (q != hipSuccess && q != hipErrorNotReady)
+      return fail(HB_ERR_HIP, "layout build: %s", hipGetErrorString(q));
     HIP_OK(hipEventSynchronize(ly->lz.ev));
     v = (int64_t)*ly->h_nruns;
     ly->n_runs.store(v, std::memory_order_release);
@@ -1202,9 +1328,9 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   {
     std::lock_guard<std::mutex> lk(t->mu);
     auto it = t->layouts.find({eq, sim});
-    if (it != t->layouts.end() && it->second.lz.built) {
-      *out = &it->second;
-      return wait_lazy(it->second.lz, caller);
+    if (it != t->layouts.end() && it->second && it->second->lz.built) {
+      *out = it->second.get();
+      return wait_lazy(it->second->lz, caller);
     }
   }
   HostTrace trace("ensure_layout");
@@ -1217,9 +1343,12 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   trace.mark("sig");
   uint64_t n_codes = 1;
   if (!eq.empty())
-    n_codes = t->ctx->ddict.fallback_used.load() ? (1ull << 32) : t->ctx->ddict.slots;
+    n_codes = t->ctx->ddict.fallback_used.load() ? (1ull << 32)
+                                                 : std::max<uint64_t>(1, t->ctx->ddict.code_bound.load());
   std::lock_guard<std::mutex> lk(t->mu);
-  Layout &ly = t->layouts[{eq, sim}];
+  std::unique_ptr<Layout> &slot = t->layouts[{eq, sim}];
+  if (!slot) slot = std::make_unique<Layout>();
+  Layout &ly = *slot;
   *out = &ly;
   if (ly.lz.built) return wait_lazy(ly.lz, caller);
   const uint32_t n = (uint32_t)t->n_rows;
@@ -1233,7 +1362,12 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&ly.d_perm_fb));
     HB_TRY(t->dev_alloc(n1 * sizeof(hb::SigRow), (void **)&ly.d_jsig_fb));
   }
+  HB_TRY(t->dev_alloc(sizeof(uint64_t), (void **)&ly.d_nruns));
+  HB_TRY(t->dev_alloc(sizeof(uint64_t), (void **)&ly.d_bad));
+  hb::launch_fill_u64((unsigned long long *)ly.d_bad, 1, 0ull, stream);
+  ly.runs_bound = (uint32_t)std::min<uint64_t>(n, n_codes);
   if (n == 0 || eq.empty()) ly.n_runs.store(n ? 1 : 0);
+  if (n == 0) hb::launch_fill_u64((unsigned long long *)ly.d_nruns, 1, 0ull, stream);
   if (n) {
     // The run count is counted on the device and copied to pinned memory;
     // the first batch that needs it waits for the build there.  The
@@ -1247,15 +1381,43 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       temps.dev.push_back({*p, bytes});
       return HB_OK;
     };
+    if (n_codes <= hb::kLayoutMaxCodes) {  // counting sorts: 4 launches
+      hb::LayoutSmall a{};
+      a.code = key_codes;
+      a.bad = ly.d_bad;
+      a.len = sim >= 0 ? t->cols[sim].d_len : nullptr;
+      a.sig = sim >= 0 ? t->cols[sim].d_sig : nullptr;
+      a.n = n;
+      a.n_codes = (uint32_t)n_codes;
+      const size_t nb = 2 * (size_t)n_codes * 256;
+      HB_TRY(temp(nb * sizeof(uint32_t), (void **)&a.hist));
+      HB_TRY(temp(nb * sizeof(uint32_t), (void **)&a.off));
+      HB_TRY(temp(2 * n1 * sizeof(uint32_t), (void **)&a.rank));
+      a.perm = ly.d_perm;
+      a.perm_fb = ly.d_perm_fb;
+      a.jsig = ly.d_jsig;
+      a.jsig_fb = ly.d_jsig_fb;
+      a.run_code = ly.d_run_code;
+      a.run_off = ly.d_run_off;
+      a.n_runs = ly.d_nruns;
+      hb::launch_fill_u64((unsigned long long *)a.hist, nb / 2, 0ull, stream);
+      hb::launch_layout_small(a, stream);
+      HIP_OK(hipMemcpyAsync(ly.h_nruns, ly.d_nruns, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            stream));
+      HIP_OK(hipGetLastError());
+      HB_TRY(retire_after(t->ctx, std::move(temps), stream));
+      HB_TRY(record_lazy(ly.lz, stream));
+      trace.mark("launch");
+      return wait_lazy(ly.lz, caller);
+    }
     uint64_t *keys_in = nullptr;
-    uint32_t *iota = nullptr, *flags = nullptr, *run_id = nullptr, *d_nruns = nullptr;
+    uint32_t *iota = nullptr, *flags = nullptr, *run_id = nullptr, *d_nruns = ly.d_nruns;
     HB_TRY(temp(n1 * sizeof(uint64_t), (void **)&keys_in));
     HB_TRY(temp(n1 * sizeof(uint32_t), (void **)&iota));
     HB_TRY(temp(n1 * sizeof(uint32_t), (void **)&flags));
     HB_TRY(temp((n1 + 1) * sizeof(uint32_t), (void **)&run_id));
-    HB_TRY(temp(sizeof(uint32_t), (void **)&d_nruns));
-    hb::launch_layout_keys(key_codes,
-                           sim >= 0 ? t->cols[sim].d_len : nullptr, n, keys_in, stream);
+    hb::launch_layout_keys(key_codes, sim >= 0 ? t->cols[sim].d_len : nullptr, n, n_codes,
+                           ly.d_bad, keys_in, stream);
     hb::launch_iota(iota, n, stream);
     int end_bit = 8;
     while (end_bit < 64 && (1ull << (end_bit - 8)) < n_codes) end_bit++;
@@ -1282,8 +1444,8 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     if (sim >= 0) {
       hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
       // the first-byte order: keys_in is free again once the first sort has run
-      hb::launch_layout_keys_fb(key_codes, t->cols[sim].d_sig, t->cols[sim].d_len, n, keys_in,
-                                stream);
+      hb::launch_layout_keys_fb(key_codes, t->cols[sim].d_sig, t->cols[sim].d_len, n, n_codes,
+                                ly.d_bad, keys_in, stream);
       HIP_OK(hb::sort_pairs_u64(tmp, tf, keys_in, keys_fb, iota, ly.d_perm_fb, n, end_bit_fb,
                                 stream));
       hb::launch_gather_sig(ly.d_perm_fb, n, t->cols[sim].d_sig, ly.d_jsig_fb, stream);
@@ -1299,6 +1461,32 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   HB_TRY(record_lazy(ly.lz, stream));
   trace.mark("launch");
   return wait_lazy(ly.lz, caller);
+}
+
+// The slow path of a layout flagged bad: finishes the coding of its key
+// column, and if its build saw a code beyond the bound it assumed, retires
+// it and builds it again with the codes now known (the batches that used it
+// run again).  Waits for the device.
+hb_status revalidate_layout(hb_table *t, const std::vector<int> &eq, int sim) {
+  if (eq.size() == 1) HB_TRY(validate_code(t, eq[0]));
+  Layout *ly = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(t->mu);
+    auto it = t->layouts.find({eq, sim});
+    if (it == t->layouts.end() || !it->second || !it->second->lz.built) return HB_OK;
+    ly = it->second.get();
+  }
+  HIP_OK(hipEventSynchronize(ly->lz.ev));
+  uint32_t bad = 0;
+  HIP_OK(hipMemcpy(&bad, ly->d_bad, sizeof(bad), hipMemcpyDeviceToHost));
+  if (!bad) return HB_OK;
+  {
+    std::lock_guard<std::mutex> lk(t->mu);
+    auto it = t->layouts.find({eq, sim});
+    if (it != t->layouts.end() && it->second.get() == ly) t->retired.push_back(std::move(it->second));
+  }
+  Layout *fresh = nullptr;
+  return ensure_layout(t, eq, sim, t->ctx->build_stream, &fresh);
 }
 
 float filter_coef(double thr, int p) {
@@ -1374,6 +1562,8 @@ hb_status enqueue_batch(Batch *b) {
     int variant;
     Layout *ly[2];
     const uint32_t *tab;
+    std::vector<int> eq[2];
+    int sim[2];
   };
   std::vector<JR> jrs;
   float memo_thr = -1.0f;  // threshold of memo_tab
@@ -1389,7 +1579,7 @@ hb_status enqueue_batch(Batch *b) {
         HB_TRY(ensure_sig(it.R, s.cr, st));
       }
       trace.mark("sig");
-      JR jr{j, k, hb::kFilterEq, {nullptr, nullptr}, nullptr};
+      JR jr{j, k, hb::kFilterEq, {nullptr, nullptr}, nullptr, {}, {-1, -1}};
       if (!rule.sims.empty()) {
         const float thr = rule.sims[0].thr;
         if (thr != memo_thr) {
@@ -1409,8 +1599,20 @@ hb_status enqueue_batch(Batch *b) {
       HB_TRY(ensure_layout(it.L, el, sl, st, &jr.ly[0]));
       HB_TRY(ensure_layout(it.R, er, sr, st, &jr.ly[1]));
       trace.mark("layout");
-      jrs.push_back(jr);
+      jr.eq[0] = std::move(el);
+      jr.eq[1] = std::move(er);
+      jr.sim[0] = sl;
+      jr.sim[1] = sr;
+      jrs.push_back(std::move(jr));
     }
+  }
+  if (b->capture) {  // a captured batch is replayed as is: its layouts must be final
+    for (JR &jr : jrs)
+      for (int side = 0; side < 2; side++) {
+        hb_table *tb = side ? b->items[jr.job].R : b->items[jr.job].L;
+        HB_TRY(revalidate_layout(tb, jr.eq[side], jr.sim[side]));
+        HB_TRY(ensure_layout(tb, jr.eq[side], jr.sim[side], st, &jr.ly[side]));
+      }
   }
   // JRs of one filter variant are contiguous (one filter launch each).
   std::stable_sort(jrs.begin(), jrs.end(),
@@ -1421,6 +1623,9 @@ hb_status enqueue_batch(Batch *b) {
   std::vector<hb::SimDesc> sims;
   bool any_refine = false;
   bool may_defer = false;  // some column pair can hold a pair with both cells over 256 bytes
+  // groups the batch may reserve for layouts still being built (past it, the
+  // host waits for their run counts)
+  uint64_t bound_budget = 1ull << 22;
   b->max_stage = 1;
   for (uint32_t q = 0; q < n_jr; q++) {
     const JR &jr = jrs[q];
@@ -1437,7 +1642,9 @@ hb_status enqueue_batch(Batch *b) {
       sd.jsig_fb = ly->d_jsig_fb ? ly->d_jsig_fb : ly->d_jsig;
       sd.run_code = ly->d_run_code;
       sd.run_off = ly->d_run_off;
-      HB_TRY(layout_runs(ly, &sd.n_runs));
+      sd.n_runs_dev = ly->d_nruns;
+      sd.bad = ly->d_bad;
+      HB_TRY(layout_runs(ly, &sd.n_runs, &bound_budget));
     }
     if (!rule.sims.empty()) {
       const double thr = (double)rule.sims[0].thr;
@@ -1519,6 +1726,7 @@ hb_status enqueue_batch(Batch *b) {
   bd.job_cnt = d_cnt + b->cnt_job_off;
   bd.overflow = d_cnt + kCntOverflow;
   bd.set_overflow = d_cnt + kCntSetOverflow;
+  bd.invalid = d_cnt + kCntInvalid;
   // per-job output sets (jobs with several rules)
   std::vector<unsigned long long> set_off(n_jobs, 0);
   unsigned long long set_total = 0;
@@ -1687,6 +1895,33 @@ hb_status enqueue_batch(Batch *b) {
 // Called with b->mu held once ev_end has completed.
 hb_status batch_complete(Batch *b) {
   const unsigned long long *c = b->h_cnt;
+  if (c[kCntInvalid]) {  // a layout was built on codes not yet final
+    std::set<std::tuple<hb_table *, std::vector<int>, int>> seen;
+    for (const auto &it : b->items)
+      for (const PlanRule &rule : it.plan->rules) {
+        std::vector<int> el, er;
+        for (const auto &e : rule.eq) {
+          el.push_back(e.first);
+          er.push_back(e.second);
+        }
+        const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
+        const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
+        if (seen.insert({it.L, el, sl}).second) HB_TRY(revalidate_layout(it.L, el, sl));
+        if (seen.insert({it.R, er, sr}).second) HB_TRY(revalidate_layout(it.R, er, sr));
+      }
+    b->retries++;
+    if (b->retries > 12) return fail(HB_ERR_CAPACITY, "batch: layouts keep changing");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    batch_free_memory(b);
+    if (b->borrowed) {
+      b->borrowed = false;
+      if (b->graph_stale) b->graph_stale->store(true);
+      for (auto &m : b->marks)
+        if (!m) HB_TRY(b->ctx->events.get(&m));
+    }
+    HB_TRY(enqueue_batch(b));
+    return HB_NOT_READY;
+  }
   const int n_stage_slots = b->max_stage + 2;
   unsigned long long need_list = 0, need_defer = 0;
   for (int s = 0; s < n_stage_slots; s++) {
@@ -2206,8 +2441,14 @@ hb_status unpack_table(hb_ctx *ctx, const hb_serialized_table *s, const uint8_t 
   }
   HIP_OK(hipMemcpyAsync(d_cols, h_cols, s->n_cols * sizeof(hb::UnpackColumn),
                         hipMemcpyHostToDevice, stream));
+  // rows staged per work-group: about 16 KiB of them, within the LDS limit
+  const size_t rpg = std::min<size_t>(256, std::max<size_t>(1, (16u << 10) / s->aligned_tuple_size));
+  if (hb::unpack_lds_bytes(s->aligned_tuple_size * rpg, rpg * s->n_cols) > hb::kUnpackLds)
+    return fail(HB_ERR_CAPACITY, "serialized table: rows of %zu bytes exceed the %u-byte "
+                "staging limit", s->aligned_tuple_size, hb::kUnpackLds);
   hb::launch_unpack_rows(d_rows, s->aligned_tuple_size, (uint32_t)s->n_rows,
-                         (const hb::UnpackColumn *)d_cols, (uint32_t)s->n_cols, stream);
+                         (const hb::UnpackColumn *)d_cols, (uint32_t)s->n_cols, (uint32_t)rpg,
+                         ctx->n_cu, stream);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
   HIP_OK(hipEventRecord(t->ready, stream));
@@ -2270,6 +2511,7 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
     const int v = std::atoi(env);
     if (v >= 1 && v <= 64) ctx->ddict.hash_bits = (uint32_t)v;
   }
+  if (std::getenv("HB_SEPARATE_COLUMN_BUILDS")) ctx->separate_col_builds = true;
   if (const char *env = std::getenv("HB_INJECT_LAUNCH_FAILURES"))
     ctx->inject_launch_failures.store(std::max(0, std::atoi(env)));
   if (const char *env = std::getenv("HB_SMALL_PAIRS")) {
@@ -2283,6 +2525,7 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
   }
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&ctx->build_stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&ctx->code_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; i++) {
     hipStream_t s;
     HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -2351,6 +2594,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->build_stream) (void)hipStreamDestroy(ctx->build_stream);
+  if (ctx->code_stream) (void)hipStreamDestroy(ctx->code_stream);
   delete ctx;
   return HB_OK;
 }
@@ -2565,7 +2809,9 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
   g.dev = std::move(t->dev_mem);
   g.host = std::move(t->host_mem);
   for (auto &kv : t->layouts)
-    if (kv.second.lz.ev) g.owned.push_back(kv.second.lz.ev);
+    if (kv.second && kv.second->lz.ev) g.owned.push_back(kv.second->lz.ev);
+  for (auto &ly : t->retired)
+    if (ly->lz.ev) g.owned.push_back(ly->lz.ev);
   if (!t->parent) {
     for (Column &c : t->cols)
       for (Lazy *lz : {&c.sig, &c.code})
@@ -2577,6 +2823,7 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
   std::vector<hipStream_t> all(ctx->streams.begin(), ctx->streams.end());
   all.push_back(ctx->copy_stream);
   all.push_back(ctx->build_stream);
+  all.push_back(ctx->code_stream);
   for (hipStream_t s : all) {
     hipEvent_t e = nullptr;
     HB_TRY(ctx->events.get(&e));

@@ -161,3 +161,70 @@ def test_dictionary_fallback_stays_exact(env):
             tr.release()
     finally:
         c.close()
+
+
+def test_fused_column_build_equals_separate_kernels():
+    """A similarity column's signatures, 128-class counters and bit-planes
+    come from one fused kernel; a context created with
+    HB_SEPARATE_COLUMN_BUILDS builds them with the three separate kernels.
+    Jobs whose first thresholds go through the bound filter (0.89), the
+    refine pass (0.6) and cells over 256 bytes give the same pairs, audit
+    entries and filter / exact-stage counts in both."""
+    L, R = _adversarial_tables(77, 900, 800, b"abcdefgh", 60, long_frac=0.05)
+    plans = [hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.89), (2, 0.5)])]),
+             hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.6)])]),
+             hb.HostPlan.from_yaml(PLANS["multi_rule"])]
+    got = {}
+    for name, env in (("fused", {}), ("separate", {"HB_SEPARATE_COLUMN_BUILDS": "1"})):
+        c = _ctx_with(env)
+        try:
+            tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
+            got[name] = []
+            for plan in plans:
+                dp = c.plan(plan)
+                pairs, audit, st = c.block(dp, tl, tr)
+                dp.release()
+                got[name].append((hb.sort_pairs(pairs), H.sort_audit(audit), st))
+            tl.release()
+            tr.release()
+        finally:
+            c.close()
+    for k, (a, b) in enumerate(zip(got["fused"], got["separate"])):
+        H.assert_same_pairs(a[0], b[0], "plan %d" % k)
+        H.assert_same_audit(a[1], b[1], "plan %d" % k)
+        for key in ("candidates", "filtered", "jw_exact"):
+            assert a[2][key] == b[2][key], (k, key, a[2][key], b[2][key])
+    want, want_audit, _ = H.run_oracle(plans[1], L, R)
+    H.assert_same_pairs(got["fused"][1][0], want, "refine plan vs oracle")
+
+
+@pytest.mark.parametrize("env", [{}, {"HB_DICT_HASH_BITS": "3"}])
+def test_deferred_coding_reruns_when_codes_change(env):
+    """A key column whose last codings added no value is coded without the
+    host waiting; layouts built from it check every code against the count
+    then known.  Fresh tables with the same key values run without a rerun;
+    a table bringing new key values (and, with a 3-bit hash, rows the host
+    must code) flags its layout, which is rebuilt, and the batch runs again:
+    every result equals the oracle's."""
+    c = _ctx_with(env)
+    try:
+        plan = hb.HostPlan.from_yaml(PLANS["eq_sim"])
+        dp = c.plan(plan)
+        same = [_adversarial_tables(300 + k, 700, 600, b"abcd", 20, n_keys=5) for k in range(4)]
+        new = _adversarial_tables(399, 700, 600, b"abcd", 20, n_keys=60)
+        retries = []
+        for k, (L, R) in enumerate(same + [new, same[0]]):
+            tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
+            got, audit, st = c.block(dp, tl, tr)
+            tl.release()
+            tr.release()
+            want, want_audit, _ = H.run_oracle(plan, L, R)
+            H.assert_same_pairs(got, want, "%s table pair %d" % (env, k))
+            H.assert_same_audit(H.sort_audit(audit), want_audit, "%s table pair %d" % (env, k))
+            retries.append(st["retries"])
+        if not env:
+            assert retries[4] >= 1, retries  # the new values were caught and the job rerun
+            assert retries[2] == 0 and retries[3] == 0, retries
+        dp.release()
+    finally:
+        c.close()
