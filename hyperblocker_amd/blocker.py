@@ -191,6 +191,15 @@ class DeviceBytes:
                                         C.c_size_t(buf.nbytes), 1) != 0:  # host to device
             raise RuntimeError("hipMemcpy to the device failed")
 
+    def to_host(self, n_bytes=None):
+        """The buffer's first n_bytes (all by default) copied to host memory."""
+        n = self.n_bytes if n_bytes is None else int(n_bytes)
+        out = np.zeros(max(n, 1), np.uint8)
+        if n and DeviceBytes._hip.hipMemcpy(C.c_void_p(out.ctypes.data), self.ptr,
+                                            C.c_size_t(n), 2) != 0:  # device to host
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out[:n]
+
     def free(self):
         if self.ptr and self.ptr.value:
             DeviceBytes._hip.hipDeviceSynchronize()
@@ -395,6 +404,22 @@ class Context:
         check(lib().hb_autotune_filter_shape(self._h, n, plans, ls, rs, cand, nc, best,
                                              C.byref(ms)))
         return (best[0], best[1], best[2]), ms.value
+
+    def block_noalloc(self, plan, l_table, r_table, pairs_buf, pair_cap, audit_buf, audit_cap):
+        """hb_block_noalloc: the job of block() on prepared tables, writing
+        into device buffers (DeviceBytes); returns (n_pairs, n_audit).
+        Raises HBError (status HB_ERR_CAPACITY) when the outputs do not fit,
+        with the counts in .counts."""
+        n, na = C.c_uint64(), C.c_uint64()
+        rc = lib().hb_block_noalloc(self._h, plan._h, l_table._h, r_table._h,
+                                    pairs_buf.ptr if pairs_buf else None, pair_cap,
+                                    audit_buf.ptr if audit_buf else None, audit_cap,
+                                    C.byref(n), C.byref(na))
+        if rc != capi.HB_OK:
+            e = capi.HBError(rc, lib().hb_last_error().decode(errors="replace"))
+            e.counts = (n.value, na.value)
+            raise e
+        return n.value, na.value
 
     def block(self, plan, l_table, r_table):
         """Synchronous blocking of l_table x r_table: (pairs uint32[n,2] in no

@@ -34,7 +34,8 @@ EXPORTS = [
     "hb_plan_create", "hb_plan_release",
     "hb_job_submit", "hb_batch_submit", "hb_batch_graph_create", "hb_batch_graph_run",
     "hb_batch_graph_destroy", "hb_job_query", "hb_job_wait", "hb_job_results",
-    "hb_job_stats_get", "hb_job_release", "hb_batch_results", "hb_jobs_release", "hb_block", "hb_free", "hb_autotune_filter_shape",
+    "hb_job_stats_get", "hb_job_release", "hb_batch_results", "hb_jobs_release", "hb_block", "hb_free",
+    "hb_block_noalloc", "hb_autotune_filter_shape",
     "hb_server_create", "hb_server_submit", "hb_server_info", "hb_server_destroy",
     "hb_rules_load", "hb_rules_parse", "hb_host_plan_view", "hb_host_plan_free",
     "hb_csv_load", "hb_host_table_view", "hb_host_table_cell", "hb_host_table_free",
@@ -130,6 +131,8 @@ def lib():
         "hb_table_release": ([VP, VP], st),
         "hb_plan_create": ([VP, C.POINTER(SerializedPlan), PVP], st),
         "hb_plan_release": ([VP, VP], st),
+        "hb_block_noalloc": ([VP, VP, VP, VP, VP, C.c_uint64, VP, C.c_uint64,
+                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], st),
         "hb_job_submit": ([VP, VP, VP, VP, C.c_uint64], st),
         "hb_batch_submit": ([VP, C.c_uint32, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
                              C.POINTER(C.c_uint64)], st),

@@ -254,6 +254,12 @@ struct UnpackColumn {
   uint32_t width;    // multiple of 16, > size
 };
 
+// Working memory of hb_block_noalloc: a file-scope device array of this
+// many bytes per GPU (hb_kernels.hip); noalloc_scratch gives its device
+// address on the current device (hipGetSymbolAddress).
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+hipError_t noalloc_scratch(void **ptr);
+
 // Kernel launch wrappers (hb_kernels.hip).  All are asynchronous on stream.
 
 // Dictionary codes of n cells (each `width` bytes, 16-byte aligned, NUL

@@ -2214,6 +2214,22 @@ void launch_fill_u64(unsigned long long *p, unsigned long long n, unsigned long 
   hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(256), 0, stream, p, n, v);
 }
 
+// ---- scratch of the no-allocation entry point --------------------------
+//
+// hb_block_noalloc carves all of a job's working memory (counters,
+// descriptors, join groups, pair lists, dedupe sets, scan temporaries) from
+// this array instead of allocating: each GPU the library runs on holds its
+// own copy for the life of the process.  Host code reaches it only through
+// the device address noalloc_scratch returns (the array's name is a
+// host-side stand-in); calls that overlap on one GPU share it and race.
+}  // namespace hb
+static __device__ __align__(256) unsigned char scratch[hb::kNoAllocScratchBytes];
+namespace hb {
+
+hipError_t noalloc_scratch(void **ptr) {
+  return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
 void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, uint32_t n,
                       DictDev d, uint32_t *code, uint32_t *fb_rows,
                       unsigned long long *fb_count, hipStream_t stream) {

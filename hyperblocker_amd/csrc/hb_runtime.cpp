@@ -607,6 +607,11 @@ struct hb_ctx {
   // Test hook: build a similarity column's structures with the three
   // separate kernels instead of the fused one (HB_SEPARATE_COLUMN_BUILDS).
   bool separate_col_builds = false;
+  // hb_block_noalloc: the device address of this GPU's copy of the
+  // library's static scratch array (looked up at creation) and the stream
+  // its work runs on.
+  void *scratch = nullptr;
+  hipStream_t noalloc_stream = nullptr;
   // Device dictionary of equality codes (hb::DictDev), created on first use;
   // its kernels run on the build stream, one column at a time.  `dict`
   // above codes the rows it leaves to the host.
@@ -2246,6 +2251,301 @@ std::shared_ptr<Batch> job_batch(Job *j, bool block) {
 
 }  // namespace
 
+// ---- one job without allocation (hb_block_noalloc) ----------------------
+//
+// The job's tables must be prepared: every structure the plan needs on them
+// (signatures, codes, join layouts) built by an earlier job.  Descriptors
+// and counters pass through fixed host arrays; all device working memory is
+// carved from the context's scratch (the static array of hb_kernels.hip);
+// the pairs and audit entries are written straight into the caller's
+// device buffers.  Synchronous.
+namespace {
+
+constexpr int kNoAllocMaxRules = 16;
+constexpr int kNoAllocMaxSims = 64;
+
+// The table's built layout for (eq, sim), found without allocating.
+Layout *find_layout(hb_table *t, const PlanRule &rule, bool left, int sim) {
+  std::lock_guard<std::mutex> lk(t->mu);
+  for (auto &kv : t->layouts) {
+    if (kv.first.second != sim || kv.first.first.size() != rule.eq.size() || !kv.second ||
+        !kv.second->lz.built)
+      continue;
+    bool same = true;
+    for (size_t k = 0; k < rule.eq.size() && same; k++)
+      same = kv.first.first[k] == (left ? rule.eq[k].first : rule.eq[k].second);
+    if (same) return kv.second.get();
+  }
+  return nullptr;
+}
+
+struct Bump {  // carves 256-byte aligned pieces of the scratch
+  char *base;
+  size_t cap, used = 0;
+  void *take(size_t bytes) {
+    const size_t off = (used + 255) & ~size_t(255);
+    if (off + bytes > cap) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
+
+hb_status block_noalloc(hb_ctx *ctx, const hb_plan *plan, const hb_table *lc, const hb_table *rc,
+                        uint32_t *d_pairs, uint64_t pair_cap, hb_audit_entry *d_audit,
+                        uint64_t audit_cap, uint64_t *n_pairs, uint64_t *n_audit) {
+  HB_TRY(check_job_args(ctx, plan, lc, rc));
+  if ((pair_cap && !d_pairs) || (audit_cap && !d_audit))
+    return fail(HB_ERR_INVALID_ARG, "hb_block_noalloc: null output buffer");
+  const int n_rules = (int)plan->rules.size();
+  if (n_rules > kNoAllocMaxRules)
+    return fail(HB_ERR_CAPACITY, "hb_block_noalloc: %d rules (at most %d)", n_rules,
+                kNoAllocMaxRules);
+  hb_table *L = const_cast<hb_table *>(lc), *R = const_cast<hb_table *>(rc);
+  HB_TRY(set_device(ctx));
+  const hipStream_t st = ctx->noalloc_stream;
+  HB_TRY(wait_ready(L, st));
+  HB_TRY(wait_ready(R, st));
+  hb::FilterShape shape;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    shape = ctx->shape;
+  }
+  // job-rules, in filter-variant order
+  hb::JRDesc hd[kNoAllocMaxRules];
+  hb::SimDesc sims[kNoAllocMaxSims];
+  int variant[kNoAllocMaxRules], order[kNoAllocMaxRules];
+  uint32_t jr_group_off[kNoAllocMaxRules + 1];
+  int n_sims = 0, max_stage = 1;
+  bool any_refine = false, may_defer = false;
+  for (int k = 0; k < n_rules; k++) {
+    variant[k] = plan->rules[k].sims.empty() ? hb::kFilterEq : hb::kFilterSim;
+    if (!plan->rules[k].sims.empty()) {
+      const float c0 = filter_coef(plan->rules[k].sims[0].thr, 0);
+      const float c3 = filter_coef(plan->rules[k].sims[0].thr, 3);
+      uint32_t b0, b3;
+      std::memcpy(&b0, &c0, 4);
+      std::memcpy(&b3, &c3, 4);
+      std::lock_guard<std::mutex> lk(ctx->allow_mu);
+      if (ctx->allow_tabs.count(((uint64_t)b0 << 32) | b3)) variant[k] = hb::kFilterSimTable;
+    }
+    order[k] = k;
+  }
+  std::stable_sort(order, order + n_rules, [&](int a, int b) { return variant[a] < variant[b]; });
+  jr_group_off[0] = 0;
+  for (int q = 0; q < n_rules; q++) {
+    const PlanRule &rule = plan->rules[order[q]];
+    hb::JRDesc &d = hd[q];
+    std::memset(&d, 0, sizeof(d));
+    const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
+    const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
+    for (const auto &s : rule.sims) {
+      if (!L->cols[s.cl].sig.built || !R->cols[s.cr].sig.built)
+        return fail(HB_ERR_INVALID_ARG, "hb_block_noalloc: the tables are not prepared for "
+                    "this plan (run a job of the plan on them first)");
+      HB_TRY(wait_lazy(L->cols[s.cl].sig, st));
+      HB_TRY(wait_lazy(R->cols[s.cr].sig, st));
+    }
+    for (int side = 0; side < 2; side++) {
+      hb_table *t = side ? R : L;
+      Layout *ly = find_layout(t, rule, side == 0, side ? sr : sl);
+      if (!ly)
+        return fail(HB_ERR_INVALID_ARG, "hb_block_noalloc: the tables are not prepared for "
+                    "this plan (run a job of the plan on them first)");
+      HB_TRY(wait_lazy(ly->lz, st));
+      uint64_t no_bound = 0;
+      hb::SideDesc &sd = d.s[side];
+      HB_TRY(layout_runs(ly, &sd.n_runs, &no_bound));
+      uint32_t bad = 0;
+      HIP_OK(hipMemcpy(&bad, ly->d_bad, sizeof(bad), hipMemcpyDeviceToHost));
+      if (bad)
+        return fail(HB_ERR_INVALID_ARG, "hb_block_noalloc: a layout of these tables is being "
+                    "rebuilt (run a regular job on them first)");
+      sd.perm = ly->d_perm;
+      sd.jsig = ly->d_jsig;
+      sd.perm_fb = ly->d_perm_fb ? ly->d_perm_fb : ly->d_perm;
+      sd.jsig_fb = ly->d_jsig_fb ? ly->d_jsig_fb : ly->d_jsig;
+      sd.run_code = ly->d_run_code;
+      sd.run_off = ly->d_run_off;
+      sd.n_runs_dev = ly->d_nruns;
+      sd.bad = ly->d_bad;
+    }
+    if (!rule.sims.empty()) {
+      const double thr = (double)rule.sims[0].thr;
+      d.c0 = filter_coef(thr, 0);
+      d.c1 = filter_coef(thr, 1);
+      d.c2 = filter_coef(thr, 2);
+      d.c3 = filter_coef(thr, 3);
+      if (variant[order[q]] == hb::kFilterSimTable) {
+        uint32_t b0, b3;
+        std::memcpy(&b0, &d.c0, 4);
+        std::memcpy(&b3, &d.c3, 4);
+        std::lock_guard<std::mutex> lk(ctx->allow_mu);
+        d.allow_tab = ctx->allow_tabs.at(((uint64_t)b0 << 32) | b3);
+      }
+    }
+    d.job = 0;
+    d.n_sims = (int32_t)rule.sims.size();
+    d.sim_off = (uint32_t)n_sims;
+    d.multi = n_rules > 1 ? 1u : 0u;
+    d.refine = (!rule.sims.empty() && rule.sims[0].thr < kRefineBelow) ? 1u : 0u;
+    any_refine |= d.refine != 0;
+    if (n_sims + (int)rule.sims.size() > kNoAllocMaxSims)
+      return fail(HB_ERR_CAPACITY, "hb_block_noalloc: more than %d similarity predicates",
+                  kNoAllocMaxSims);
+    for (const auto &s : rule.sims) {
+      const Column &cl = L->cols[s.cl];
+      const Column &cr = R->cols[s.cr];
+      hb::SimDesc x{};
+      x.col_l = hb::DevColumn{cl.d_bytes, cl.d_len, cl.width, cl.d_planes, cl.nwc, cl.d_sig,
+                              cl.d_hist};
+      x.col_r = hb::DevColumn{cr.d_bytes, cr.d_len, cr.width, cr.d_planes, cr.nwc, cr.d_sig,
+                              cr.d_hist};
+      x.thr = s.thr;
+      x.slot = s.slot;
+      may_defer |= std::min(cl.width, cr.width) > 257;
+      x.c0 = filter_coef((double)s.thr, 0);
+      x.c1 = filter_coef((double)s.thr, 1);
+      x.c2 = filter_coef((double)s.thr, 2);
+      x.c3 = filter_coef((double)s.thr, 3);
+      sims[n_sims++] = x;
+    }
+    max_stage = std::max(max_stage, (int)rule.sims.size());
+    jr_group_off[q + 1] = jr_group_off[q] + d.s[0].n_runs;
+  }
+  const uint32_t n_groups = jr_group_off[n_rules];
+  // counter block (as a batch of one job lays it out)
+  const int n_stage_slots = max_stage + 2;
+  const uint32_t stride = 2;
+  const int cnt_stage_off = kCntFixed;
+  const int cnt_job_off = cnt_stage_off + 2 * n_stage_slots;
+  const int cnt_words = cnt_job_off + (int)(hb::kJobRows * stride) + 4 * (int)stride;
+  constexpr int kMaxCntWords = 512;
+  if (cnt_words > kMaxCntWords) return fail(HB_ERR_CAPACITY, "hb_block_noalloc: plan too long");
+  // scratch
+  Bump sc{(char *)ctx->scratch, hb::kNoAllocScratchBytes};
+  unsigned long long *d_cnt = (unsigned long long *)sc.take(cnt_words * 8);
+  alignas(16) char hdesc[sizeof(hd) + sizeof(jr_group_off) + sizeof(sims) + 256];
+  size_t o = 0;
+  auto place = [&](const void *src, size_t bytes) {
+    o = (o + 15) & ~size_t(15);
+    std::memcpy(hdesc + o, src, bytes);
+    const size_t at = o;
+    o += bytes;
+    return at;
+  };
+  const size_t at_jrs = place(hd, n_rules * sizeof(hb::JRDesc));
+  const size_t at_off = place(jr_group_off, (n_rules + 1) * sizeof(uint32_t));
+  const size_t at_sims = place(sims, std::max(n_sims, 1) * sizeof(hb::SimDesc));
+  // a dedupe set (several rules) holds every distinct output pair at most
+  // half full: sized from the output capacity, since it cannot grow here
+  unsigned long long set_cap = 0;
+  if (n_rules > 1)
+    set_cap = pow2_at_least(2 * std::min<unsigned long long>(
+                                    pair_cap, (unsigned long long)L->n_rows * R->n_rows) + 1024);
+  const unsigned long long set_off0 = 0;
+  const size_t at_setoff = place(&set_off0, 8);
+  const size_t at_setcap = place(&set_cap, 8);
+  char *ddesc = (char *)sc.take(o);
+  unsigned long long *d_sets = set_cap ? (unsigned long long *)sc.take(set_cap * 8) : nullptr;
+  uint32_t *u32arr[5];
+  uint64_t *u64arr[3];
+  for (auto &a : u32arr) a = (uint32_t *)sc.take(((size_t)n_groups + 1) * 4);
+  for (auto &a : u64arr) a = (uint64_t *)sc.take(((size_t)n_groups + 1) * 8);
+  size_t tmp_bytes = 0;
+  HIP_OK(hb::exscan_u64(nullptr, tmp_bytes, u64arr[1], u64arr[2], n_groups + 1, st));
+  tmp_bytes = std::max<size_t>(tmp_bytes, 16);
+  void *tmp = sc.take(tmp_bytes);
+  bool fits = d_cnt && ddesc && (!set_cap || d_sets) && tmp;
+  for (auto *a : u32arr) fits = fits && a;
+  for (auto *a : u64arr) fits = fits && a;
+  if (!fits)
+    return fail(HB_ERR_CAPACITY, "hb_block_noalloc: %u join groups do not fit the %zu-byte "
+                "scratch", n_groups, (size_t)hb::kNoAllocScratchBytes);
+  // the rest: two pair lists and the deferred list (1/64 of an entry per list entry)
+  const size_t used = ((sc.used + 255) & ~size_t(255)) + 3 * 256;
+  const size_t left = sc.cap > used ? sc.cap - used : 0;
+  const unsigned long long list_cap = (unsigned long long)(left / (sizeof(hb::PairEntry) * (2 + 1.0 / 64)));
+  const unsigned long long defer_cap = std::max<unsigned long long>(list_cap / 64, 1);
+  hb::PairEntry *lists[2] = {(hb::PairEntry *)sc.take(list_cap * sizeof(hb::PairEntry)),
+                             (hb::PairEntry *)sc.take(list_cap * sizeof(hb::PairEntry))};
+  hb::PairEntry *defer_buf = (hb::PairEntry *)sc.take(defer_cap * sizeof(hb::PairEntry));
+  if (!lists[0] || !lists[1] || !defer_buf || list_cap < 1024)
+    return fail(HB_ERR_CAPACITY, "hb_block_noalloc: no room for pair lists in the scratch");
+  HIP_OK(hipMemcpy(ddesc, hdesc, o, hipMemcpyHostToDevice));
+  hb::BatchDev bd{};
+  bd.jrs = (const hb::JRDesc *)(ddesc + at_jrs);
+  bd.jr_group_off = (const uint32_t *)(ddesc + at_off);
+  bd.n_jr = (uint32_t)n_rules;
+  bd.cnt_stride = stride;
+  bd.sims = (const hb::SimDesc *)(ddesc + at_sims);
+  bd.job_cnt = d_cnt + cnt_job_off;
+  bd.overflow = d_cnt + kCntOverflow;
+  bd.set_overflow = d_cnt + kCntSetOverflow;
+  bd.invalid = d_cnt + kCntInvalid;
+  bd.sets.slots = d_sets;
+  bd.sets.off = set_cap ? (const unsigned long long *)(ddesc + at_setoff) : nullptr;
+  bd.sets.cap = set_cap ? (const unsigned long long *)(ddesc + at_setcap) : nullptr;
+  hb::GroupArrays g{};
+  g.n = n_groups;
+  g.tile_a = 256u * (uint32_t)shape.rows_per_lane;
+  g.tile_b = shape.tile_b;
+  g.small_pairs_max = shape.small_pairs_max;
+  g.l_off = u32arr[0];
+  g.l_cnt = u32arr[1];
+  g.r_off = u32arr[2];
+  g.r_cnt = u32arr[3];
+  g.jr = u32arr[4];
+  g.small_pairs = u64arr[0];
+  g.items = u64arr[1];
+  g.item_off = u64arr[2];
+  // the launch sequence of a batch of one job (enqueue_batch), outputs direct
+  hb::launch_fill_u64(d_cnt, cnt_words, 0ull, st);
+  if (set_cap) hb::launch_fill_u64(d_sets, set_cap, ~0ull, st);
+  hb::launch_batch_join(bd, g, st);
+  HIP_OK(hb::exscan_u64(tmp, tmp_bytes, g.items, g.item_off, n_groups + 1, st));
+  unsigned long long *list_cnt = d_cnt + cnt_stage_off;
+  unsigned long long *defer_cnt = list_cnt + n_stage_slots;
+  const hb::PairList list0{lists[0], list_cnt + 0, list_cap};
+  for (int q = 0; q < n_rules;) {
+    int e = q;
+    while (e < n_rules && variant[order[e]] == variant[order[q]]) e++;
+    hb::launch_filter(bd, g, jr_group_off[q], jr_group_off[e], variant[order[q]], list0, shape,
+                      ctx->n_cu, st);
+    q = e;
+  }
+  const hb::PairList fin{nullptr, d_cnt + kCntFinal, pair_cap};
+  const hb::AuditList aud{nullptr, d_cnt + kCntAudit, audit_cap};
+  const int off = any_refine ? 1 : 0;
+  if (any_refine)
+    hb::launch_refine(bd, list0, hb::PairList{lists[1], list_cnt + 1, list_cap}, ctx->n_cu, st);
+  for (int s = 1; s <= max_stage; s++) {
+    const int q = s - 1 + off;
+    const hb::PairList in{lists[q & 1], list_cnt + q, list_cap};
+    const hb::PairList next{lists[(q + 1) & 1], list_cnt + q + 1, list_cap};
+    const hb::PairList defer{defer_buf, defer_cnt + s, defer_cap};
+    hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, may_defer,
+                     (uint2 *)d_pairs, d_audit, ctx->n_cu, st);
+  }
+  HIP_OK(hipGetLastError());
+  unsigned long long c[kMaxCntWords];
+  HIP_OK(hipMemcpyAsync(c, d_cnt, cnt_words * 8, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  if (n_pairs) *n_pairs = c[kCntFinal];
+  if (n_audit) *n_audit = c[kCntAudit];
+  if (c[kCntInvalid])
+    return fail(HB_ERR_INVALID_ARG, "hb_block_noalloc: a layout of these tables is being rebuilt");
+  if (c[kCntFinal] > pair_cap || c[kCntAudit] > audit_cap)
+    return fail(HB_ERR_CAPACITY, "hb_block_noalloc: %llu pairs and %llu audit entries exceed the "
+                "output capacity (%llu, %llu)", c[kCntFinal], c[kCntAudit],
+                (unsigned long long)pair_cap, (unsigned long long)audit_cap);
+  if (c[kCntOverflow] || c[kCntSetOverflow])
+    return fail(HB_ERR_CAPACITY, "hb_block_noalloc: the job's pair lists exceed the %zu-byte "
+                "scratch", (size_t)hb::kNoAllocScratchBytes);
+  return HB_OK;
+}
+
+}  // namespace
+
 // A job set run repeatedly as one captured batch (hb_batch_graph_*).
 struct hb_batch_graph {
   hb_ctx *ctx = nullptr;
@@ -2526,6 +2826,9 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&ctx->build_stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&ctx->code_stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&ctx->noalloc_stream, hipStreamNonBlocking));
+  // the scratch array's address on this device (each GPU has its own copy)
+  HIP_OK(hb::noalloc_scratch(&ctx->scratch));
   for (int i = 0; i < 4; i++) {
     hipStream_t s;
     HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -2595,6 +2898,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->build_stream) (void)hipStreamDestroy(ctx->build_stream);
   if (ctx->code_stream) (void)hipStreamDestroy(ctx->code_stream);
+  if (ctx->noalloc_stream) (void)hipStreamDestroy(ctx->noalloc_stream);
   delete ctx;
   return HB_OK;
 }
@@ -3061,6 +3365,12 @@ hb_status hb_block(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, const hb
 }
 
 void hb_free(void *p) { std::free(p); }
+
+hb_status hb_block_noalloc(hb_ctx *ctx, const hb_plan *plan, const hb_table *l, const hb_table *r,
+                           uint32_t *d_pairs, uint64_t pair_cap, hb_audit_entry *d_audit,
+                           uint64_t audit_cap, uint64_t *n_pairs, uint64_t *n_audit) {
+  return block_noalloc(ctx, plan, l, r, d_pairs, pair_cap, d_audit, audit_cap, n_pairs, n_audit);
+}
 
 }  // extern "C"
 

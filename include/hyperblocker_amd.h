@@ -269,6 +269,24 @@ hb_status hb_block(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
                    hb_audit_entry **audit, uint64_t *n_audit, hb_job_stats *stats);
 void hb_free(void *p);
 
+/* One blocking job that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  The job of hb_block on tables already prepared for the
+ * plan (every signature, code and join layout it needs built by an earlier
+ * regular job on them; else HB_ERR_INVALID_ARG), run synchronously with all
+ * working memory taken from one static device array of the library (64 MiB
+ * per GPU, held until the process ends) and no hipMalloc / hipFree.  The
+ * pairs (2 * n_pairs uint32, (l, r)) and audit entries are written to the
+ * caller's device buffers d_pairs / d_audit (pair_cap pairs, audit_cap
+ * entries); *n_pairs / *n_audit receive the counts even when they exceed
+ * the capacities, which returns HB_ERR_CAPACITY (so does a job whose pair
+ * lists do not fit the static array).  Calls that overlap on one GPU share
+ * the array and race: serialise them. */
+hb_status hb_block_noalloc(hb_ctx *ctx, const hb_plan *plan, const hb_table *l,
+                           const hb_table *r, uint32_t *d_pairs, uint64_t pair_cap,
+                           hb_audit_entry *d_audit, uint64_t audit_cap, uint64_t *n_pairs,
+                           uint64_t *n_audit);
+
 /* ---- serving loop ----------------------------------------------------------
  * No counterpart in the reference.  A server owns a worker thread that
  * admits queued jobs into batches (at most max_jobs_per_batch jobs, at most
