@@ -361,15 +361,17 @@ void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, 
 // to `out`.
 void launch_refine(BatchDev b, PairList in, PairList out, int n_cu, hipStream_t stream);
 
-// Exact stage `stage` (1-based similarity index) on list `in`: JRs without
-// similarity predicates pass their pairs to the output directly; pairs
-// passing their last predicate go to `final_` (deduplicated per job where
-// needed), the others to `next`.  Pairs with a shorter string above 256
-// bytes run one wave per pair (second kernel, via `defer`; not launched
-// when `may_defer` is false, i.e. no column pair can hold such a pair).
-// With `direct_pairs` set (a batch of one job) the final pairs go there as
-// (left, right) at their final-list positions instead, and audit entries go
-// to `direct_audit`: the job's output needs no scatter.
+// Exact pass on list `in`: similarity predicates `stage` and `stage` + 1
+// (1-based; the second where the pair's rule has it, evaluated only for
+// pairs passing the first).  JRs without similarity predicates pass their
+// pairs to the output directly; pairs passing their last predicate go to
+// `final_` (deduplicated per job where needed), the others to `next`.
+// Pairs with a shorter string above 256 bytes in either predicate run one
+// wave per pair (second kernel, via `defer`; not launched when `may_defer`
+// is false, i.e. no column pair can hold such a pair).  With
+// `direct_pairs` set (a batch of one job) the final pairs go there as
+// (left, right) at their final-list positions instead, and audit entries
+// go to `direct_audit`: the job's output needs no scatter.
 void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList final_,
                   PairList defer, AuditList audit, bool count_filtered, bool may_defer,
                   uint2 *direct_pairs, hb_audit_entry *direct_audit, int n_cu,

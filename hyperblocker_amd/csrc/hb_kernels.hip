@@ -16,13 +16,14 @@
 //      tagged with their JR.  Large groups run as tiles (512 rows in lanes x
 //      256 rows staged in LDS and read as broadcasts), small groups as
 //      explicit pair lists.
-//   3. k_exact_lanes: each surviving pair (one per lane) gets the exact greedy
+//   3. k_exact_two: each surviving pair (one per lane) gets the exact greedy
 //      Jaro match count from bit-planes of its shorter string, then the
-//      reference's fp32/fp64 tail.  Shorter strings above 256 bytes go to
-//      k_exact (one wave per pair).  Later similarity predicates run as
-//      further stages on the shrinking list; pairs passing a JR's last
-//      predicate enter the final list once per job (hash set for jobs with
-//      several rules).
+//      reference's fp32/fp64 tail, for two consecutive similarity
+//      predicates per pass (in separate waves, decided in order).  Shorter
+//      strings above 256 bytes go to k_exact (one wave per pair).  Later
+//      predicates run as further passes on the shrinking list; pairs
+//      passing a JR's last predicate enter the final list once per job
+//      (hash set for jobs with several rules).
 //   4. k_scatter: the final list is copied into per-job ranges.
 //
 // Floating point: contraction is off for this file, fp32 '/' is correctly
@@ -1704,17 +1705,7 @@ __device__ __forceinline__ bool decide(const ExactArgs &a, const PairEntry &e, u
   return pass;
 }
 
-// Decision, output and audit of one evaluated pair (lane-local values,
-// called by every lane of the wave).
-__device__ __forceinline__ void emit_result(const ExactArgs &a, const PairEntry &e, uint32_t job,
-                                            bool last, float thr_f, int32_t slot, int m, int S,
-                                            int T, int p, bool ok) {
-  const bool pass = decide(a, e, job, thr_f, slot, m, S, T, p, ok);
-  finalize(a, pass && last, e.l, e.r, job);
-  list_append(pass && !last, e, a.next, a.b.overflow);
-}
-
-// Output of one work-group pass of k_exact_lanes (256 pairs): entries for
+// Output of one work-group pass of k_exact_two (up to 256 pairs): entries for
 // the next stage and for the final list are gathered in LDS and leave with
 // one global atomic per list; the final entries' match count is added to
 // their job with one atomic when they all belong to one job.
@@ -1737,7 +1728,7 @@ __device__ __forceinline__ void stage_put(bool ok, const PairEntry &e, PairEntry
   if (ok) buf[base + rank_in(mask)] = e;
 }
 
-// finalize() for k_exact_lanes: into the group's final buffer.
+// finalize() for k_exact_two: into the group's final buffer.
 __device__ __forceinline__ void stage_final(const ExactArgs &a, StageOut &so, bool ok, uint32_t l,
                                             uint32_t r, uint32_t job) {
   const bool emit = ok && set_insert(a.b, job, l, r);
@@ -1966,39 +1957,71 @@ __global__ __launch_bounds__(256) void k_refine(const BatchDev b, const PairList
   }
 }
 
-// One pair per lane.  Pairs whose shorter string is longer than 256 bytes
-// are deferred to k_exact (one wave per pair, 257-slot semantics).
-// A pair of k_exact_lanes between its load and its evaluation.
+// A pass of the exact stage evaluates two consecutive similarity
+// predicates of each pair's rule, s = a.stage and s + 1 (when the rule has
+// it), 256 pairs per work-group of 8 waves: waves 0-3 run the greedy chains
+// of predicate s, waves 4-7 those of predicate s + 1 for the same pairs
+// (each SIMD gets one wave of each), so a pass keeps twice as many chains in
+// flight as one predicate would.  The decisions then follow the reference's
+// order (global_func.cuh:120-167): predicate s + 1 counts (its JW
+// evaluation, its audit entry, its decision) only for pairs that passed
+// predicate s.  Pairs whose shorter string of either predicate is longer
+// than 128 bytes are deferred to k_exact (one wave per pair, with the
+// 257-slot semantics beyond 256 bytes) for both predicates; lanes hold at
+// most 4 plane words.
 struct ExactSlot {
   PairEntry e;
-  uint32_t job, sim_off, la, lb;
+  uint32_t job, sim_off;
   int32_t n_sims;
-  uint32_t act;
+  uint32_t act;         // predicate s is evaluated (bit 0), and s + 1 (bit 1)
+  uint32_t la[2], lb[2];
 };
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_exact_lanes(ExactArgs a) {
+// The greedy match count, common prefix and lengths of one chain.
+__device__ __forceinline__ void lane_chain(const SimDesc &sd, const PairEntry &e, int la, int lb,
+                                           bool act, int &m, int &p) {
+  const int S = min(la, lb), T = max(la, lb);
+  const bool swap = la > lb;  // device_func.cuh:71-79: left stays inner on ties
+  const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
+  const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
+  const uint8_t *outer = swap ? ca : cb;
+  const uint32_t *planes = swap ? sd.col_r.planes + (uint64_t)e.r * sd.col_r.nwc * 8
+                                : sd.col_l.planes + (uint64_t)e.l * sd.col_l.nwc * 8;
+  const int nwc = (int)(swap ? sd.col_r.nwc : sd.col_l.nwc);
+  p = act ? lane_prefix(ca, cb, S) : 0;
+  const bool live = act && S > 0;
+  const bool gt32 = __ballot(live && S > 32) != 0, gt64 = __ballot(live && S > 64) != 0;
+  if (!gt32) m = lane_matches<1>(planes, nwc, S, T, outer, live);
+  else if (!gt64) m = lane_matches<2>(planes, nwc, S, T, outer, live);
+  else m = lane_matches<4>(planes, nwc, S, T, outer, live);  // S <= 128 here
+  if (!live) m = 0;
+}
+
+constexpr uint32_t kExactThreads = 512;
+__global__ __launch_bounds__(kExactThreads) void k_exact_two(ExactArgs a) {
+  constexpr uint32_t kPass = kExactThreads / 2;  // pairs per work-group pass
+  constexpr int kClasses = 16;     // n_out classes of 16 steps
   __shared__ StageOut so;
-  // The pass's pairs are regrouped by n_out (the greedy loop's trip count)
-  // in 16 classes of 16 steps, so the waves of the group hold pairs of
-  // similar length: a wave runs to its longest pair.
-  constexpr int kClasses = 16;
-  __shared__ uint32_t s_ccnt[kClasses];
-  __shared__ ExactSlot s_slot[256];
+  __shared__ ExactSlot s_slot[kPass];
+  __shared__ uint32_t s_ccnt[2][kClasses];
+  __shared__ uint16_t s_order[2][kPass];  // pairs of each chain half, by n_out class
+  __shared__ int32_t s_m[2][kPass], s_p[2][kPass];
   const unsigned long long n_in = *a.in.count;
   const unsigned long long n_use = n_in < a.in.cap ? n_in : a.in.cap;
   const uint32_t tid = threadIdx.x;
-  // Passes of 256 pairs, the same number for every thread of the group (the
-  // output flush synchronises the group).
-  for (uint64_t blk = (uint64_t)blockIdx.x * 256; blk < n_use; blk += (uint64_t)gridDim.x * 256) {
+  const int st = a.stage;
+  for (uint64_t blk = (uint64_t)blockIdx.x * kPass; blk < n_use;
+       blk += (uint64_t)gridDim.x * kPass) {
     if (tid == 0) {
       so.n_next = 0;
       so.n_fin = 0;
       so.fin_job = kNoJob;
     }
-    if (tid < kClasses) s_ccnt[tid] = 0;
+    if (tid < 2 * kClasses) (&s_ccnt[0][0])[tid] = 0;
     __syncthreads();
-    uint32_t cls = 0, rank = 0;
-    {
+    // waves 0-1: load the pass's pairs
+    uint32_t cls[2] = {0, 0}, rank[2] = {0, 0};
+    if (tid < kPass) {
       const uint64_t idx = blk + tid;
       bool act = idx < n_use;
       ExactSlot sl{};
@@ -2012,62 +2035,81 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (a.count_filtered) job_count(a.b, kJobFiltered, act, sl.job);
       // a rule without similarity predicates is complete after the filter
       stage_final(a, so, act && sl.n_sims == 0, sl.e.l, sl.e.r, sl.job);
-      act = act && sl.n_sims >= a.stage;
-      if (act) {
-        const SimDesc &sd = a.b.sims[sl.sim_off + a.stage - 1];
-        sl.la = sd.col_l.len[sl.e.l];
-        sl.lb = sd.col_r.len[sl.e.r];
+      act = act && sl.n_sims >= st;
+      const bool two = act && sl.n_sims >= st + 1;
+      int Smax = 0;
+      for (int h = 0; h < 2; h++) {
+        if (h == 0 ? act : two) {
+          const SimDesc &sd = a.b.sims[sl.sim_off + st - 1 + h];
+          sl.la[h] = sd.col_l.len[sl.e.l];
+          sl.lb[h] = sd.col_r.len[sl.e.r];
+          Smax = max(Smax, (int)min(sl.la[h], sl.lb[h]));
+        }
       }
-      const int S = (int)min(sl.la, sl.lb), T = (int)max(sl.la, sl.lb);
-      const bool deferred = act && S > 256;
+      const bool deferred = act && Smax > 128;
       list_append(deferred, sl.e, a.defer, a.b.overflow);
       act = act && !deferred;
-      sl.act = act ? 1u : 0u;
-      if (act) cls = (uint32_t)min(min(T, S + ((S + 1) >> 1)) >> 4, kClasses - 1);
-      rank = atomicAdd(&s_ccnt[cls], 1u);
-      __syncthreads();
-      uint32_t off = 0;  // exclusive prefix over the classes below cls
-      for (uint32_t c = 0; c < cls; c++) off += s_ccnt[c];
-      s_slot[off + rank] = sl;
-      __syncthreads();
+      sl.act = act ? (two ? 3u : 1u) : 0u;
+      s_slot[tid] = sl;
+      for (int h = 0; h < 2; h++) {
+        if (sl.act & (1u << h)) {
+          const int S = (int)min(sl.la[h], sl.lb[h]), T = (int)max(sl.la[h], sl.lb[h]);
+          cls[h] = (uint32_t)min(min(T, S + ((S + 1) >> 1)) >> 4, kClasses - 1);
+        }
+        rank[h] = atomicAdd(&s_ccnt[h][cls[h]], 1u);
+      }
     }
-    const ExactSlot sl = s_slot[tid];
-    const PairEntry e = sl.e;
-    const uint32_t job = sl.job;
-    const int32_t n_sims = sl.n_sims;
-    const bool act = sl.act != 0;
-    SimDesc sd{};
-    if (act) sd = a.b.sims[sl.sim_off + a.stage - 1];
-    const int la = (int)sl.la, lb = (int)sl.lb;
-    const int S = min(la, lb), T = max(la, lb);
-    const bool swap = la > lb;  // device_func.cuh:71-79: left stays inner on ties
-    const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
-    const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
-    const uint8_t *outer = swap ? ca : cb;
-    const uint32_t *planes = swap ? sd.col_r.planes + (uint64_t)e.r * sd.col_r.nwc * 8
-                                  : sd.col_l.planes + (uint64_t)e.l * sd.col_l.nwc * 8;
-    const int nwc = (int)(swap ? sd.col_r.nwc : sd.col_l.nwc);
-    const int p = act ? lane_prefix(ca, cb, S) : 0;
-    const bool live = act && S > 0;
-    int m = 0;
-    const bool gt32 = __ballot(live && S > 32) != 0, gt64 = __ballot(live && S > 64) != 0;
-    const bool gt128 = __ballot(live && S > 128) != 0;
-    if (!gt32) m = lane_matches<1>(planes, nwc, S, T, outer, live);
-    else if (!gt64) m = lane_matches<2>(planes, nwc, S, T, outer, live);
-    else if (!gt128) m = lane_matches<4>(planes, nwc, S, T, outer, live);
-    else m = lane_matches<8>(planes, nwc, S, T, outer, live);
-    if (!live) m = 0;
-    const bool last = n_sims == a.stage;
-    const bool pass = decide(a, e, job, sd.thr, sd.slot, m, S, T, p, act);
-    stage_final(a, so, pass && last, e.l, e.r, job);
-    stage_put(pass && !last, e, so.next, so.n_next);
-    job_count(a.b, kJobJw, act, job);
+    __syncthreads();
+    if (tid < kPass) {
+      for (int h = 0; h < 2; h++) {
+        uint32_t off = 0;  // exclusive prefix over the classes below
+        for (uint32_t c = 0; c < cls[h]; c++) off += s_ccnt[h][c];
+        s_order[h][off + rank[h]] = (uint16_t)tid;
+      }
+    }
+    __syncthreads();
+    {  // the chains: half h = tid / 128 evaluates predicate st + h
+      const int h = (int)(tid / kPass);
+      const uint32_t i = s_order[h][tid % kPass];
+      const ExactSlot &sl = s_slot[i];
+      const bool act = (sl.act >> h) & 1u;
+      SimDesc sd{};
+      if (act) sd = a.b.sims[sl.sim_off + st - 1 + h];
+      int m = 0, p = 0;
+      lane_chain(sd, sl.e, (int)sl.la[h], (int)sl.lb[h], act, m, p);
+      s_m[h][i] = m;
+      s_p[h][i] = p;
+    }
+    __syncthreads();
+    if (tid < kPass) {  // decisions in the reference's order
+      const ExactSlot sl = s_slot[tid];
+      const bool act = sl.act & 1u;
+      SimDesc sd{};
+      if (act) sd = a.b.sims[sl.sim_off + st - 1];
+      const int S0 = (int)min(sl.la[0], sl.lb[0]), T0 = (int)max(sl.la[0], sl.lb[0]);
+      const bool pass0 = decide(a, sl.e, sl.job, sd.thr, sd.slot, s_m[0][tid], S0, T0,
+                                s_p[0][tid], act);
+      job_count(a.b, kJobJw, act, sl.job);
+      const bool eval1 = pass0 && (sl.act & 2u);  // predicate st + 1, reached
+      SimDesc sd1{};
+      if (eval1) sd1 = a.b.sims[sl.sim_off + st];
+      const int S1 = (int)min(sl.la[1], sl.lb[1]), T1 = (int)max(sl.la[1], sl.lb[1]);
+      const bool pass1 = decide(a, sl.e, sl.job, sd1.thr, sd1.slot, s_m[1][tid], S1, T1,
+                                s_p[1][tid], eval1);
+      job_count(a.b, kJobJw, eval1, sl.job);
+      // the rule's last predicate decides; earlier ones send the pair on
+      const int last_done = (sl.act & 2u) ? st + 1 : st;
+      const bool passed = (sl.act & 2u) ? pass1 : pass0;
+      const bool done = act && sl.n_sims == last_done;
+      stage_final(a, so, passed && done, sl.e.l, sl.e.r, sl.job);
+      stage_put(passed && !done && act, sl.e, so.next, so.n_next);
+    }
     stage_flush(a, so);
   }
 }
 
 // One wave per pair (scalar greedy); used for pairs deferred by
-// k_exact_lanes, i.e. shorter strings of more than 256 bytes.
+// k_exact_two, i.e. shorter strings of more than 256 bytes.
 __global__ __launch_bounds__(256) void k_exact(ExactArgs a) {
   const uint32_t lane = lane_id();
   const unsigned long long n_in = *a.in.count;
@@ -2077,30 +2119,40 @@ __global__ __launch_bounds__(256) void k_exact(ExactArgs a) {
   for (uint64_t idx = wave; idx < n_use; idx += n_waves) {
     const PairEntry e = a.in.e[idx];
     const JRDesc &d = a.b.jrs[e.jr];
-    const SimDesc sd = a.b.sims[d.sim_off + a.stage - 1];
-    const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
-    const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
-    const int la = sd.col_l.len[e.l], lb = sd.col_r.len[e.r];
-    const int S = min(la, lb), T = max(la, lb);
-    // device_func.cuh:71-79: the left cell stays inner unless strictly longer.
-    const bool swap = la > lb;
-    const uint8_t *inner = swap ? cb : ca;
-    const uint8_t *outer = swap ? ca : cb;
-    const uint32_t outer_width = swap ? sd.col_l.width : sd.col_r.width;
-    const int p = common_prefix(ca, cb, S);
-    int m = 0;
-    if (S > 0) {
-      const uint32_t *odw = reinterpret_cast<const uint32_t *>(outer);
-      const int ndw = (int)(outer_width >> 2);
-      if (S <= 64)
-        m = greedy_matches<1>(inner, S, odw, ndw, T);
-      else if (S <= 128)
-        m = greedy_matches<2>(inner, S, odw, ndw, T);
-      else
-        m = greedy_matches<5>(inner, S, odw, ndw, T);
+    // predicates stage and stage + 1 (as in k_exact_two), in order
+    bool pass = true;
+    int stage = a.stage;
+    const int last = min(d.n_sims, a.stage + 1);
+    for (; stage <= last && pass; stage++) {
+      const SimDesc sd = a.b.sims[d.sim_off + stage - 1];
+      const uint8_t *ca = sd.col_l.bytes + (uint64_t)e.l * sd.col_l.width;
+      const uint8_t *cb = sd.col_r.bytes + (uint64_t)e.r * sd.col_r.width;
+      const int la = sd.col_l.len[e.l], lb = sd.col_r.len[e.r];
+      const int S = min(la, lb), T = max(la, lb);
+      // device_func.cuh:71-79: the left cell stays inner unless strictly longer.
+      const bool swap = la > lb;
+      const uint8_t *inner = swap ? cb : ca;
+      const uint8_t *outer = swap ? ca : cb;
+      const uint32_t outer_width = swap ? sd.col_l.width : sd.col_r.width;
+      const int p = common_prefix(ca, cb, S);
+      int m = 0;
+      if (S > 0) {
+        const uint32_t *odw = reinterpret_cast<const uint32_t *>(outer);
+        const int ndw = (int)(outer_width >> 2);
+        if (S <= 64)
+          m = greedy_matches<1>(inner, S, odw, ndw, T);
+        else if (S <= 128)
+          m = greedy_matches<2>(inner, S, odw, ndw, T);
+        else
+          m = greedy_matches<5>(inner, S, odw, ndw, T);
+      }
+      pass = decide(a, e, d.job, sd.thr, sd.slot, m, S, T, p, lane == 0);
+      pass = __shfl(pass ? 1 : 0, 0) != 0;  // lane 0's decision, for the wave
+      job_count(a.b, kJobJw, lane == 0, d.job);
     }
-    emit_result(a, e, d.job, d.n_sims == a.stage, sd.thr, sd.slot, m, S, T, p, lane == 0);
-    job_count(a.b, kJobJw, lane == 0, d.job);
+    const bool done = last == d.n_sims;
+    finalize(a, pass && done && lane == 0, e.l, e.r, d.job);
+    list_append(pass && !done && lane == 0, e, a.next, a.b.overflow);
   }
 }
 
@@ -2357,7 +2409,7 @@ void launch_exact(BatchDev b, PairList in, int stage, PairList next, PairList fi
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
   ExactArgs a{b,      in,    stage,        count_filtered ? 1 : 0, next, final_, defer, audit,
               direct_pairs, direct_audit};
-  hipLaunchKernelGGL(k_exact_lanes, dim3(cus * 8), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(k_exact_two, dim3(cus * 4), dim3(kExactThreads), 0, stream, a);
   if (!may_defer) return;
   ExactArgs d{b, defer, stage, 0, next, final_, defer, audit, direct_pairs, direct_audit};
   d.defer.cap = 0;  // deferred pairs are not deferred again

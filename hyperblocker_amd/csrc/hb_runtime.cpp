@@ -1854,11 +1854,12 @@ hb_status enqueue_batch(Batch *b) {
   if (any_refine)
     hb::launch_refine(bd, list0, hb::PairList{lists[1], list_cnt + 1, b->list_cap}, ctx->n_cu,
                       st);
-  for (int s = 1; s <= b->max_stage; s++) {
-    const int q = s - 1 + off;  // slot of this stage's input
+  // one pass per two similarity predicates (predicates s and s + 1)
+  for (int s = 1, pass = 0; s <= b->max_stage; s += 2, pass++) {
+    const int q = pass + off;  // slot of this pass's input
     const hb::PairList in{lists[q & 1], list_cnt + q, b->list_cap};
     const hb::PairList next{lists[(q + 1) & 1], list_cnt + q + 1, b->list_cap};
-    const hb::PairList defer{defer_buf, defer_cnt + s, b->defer_cap};
+    const hb::PairList defer{defer_buf, defer_cnt + pass, b->defer_cap};
     hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, may_defer,
                      direct ? (uint2 *)dout : nullptr,
                      direct ? (hb_audit_entry *)daud : nullptr, ctx->n_cu, st);
@@ -2518,11 +2519,11 @@ hb_status block_noalloc(hb_ctx *ctx, const hb_plan *plan, const hb_table *lc, co
   const int off = any_refine ? 1 : 0;
   if (any_refine)
     hb::launch_refine(bd, list0, hb::PairList{lists[1], list_cnt + 1, list_cap}, ctx->n_cu, st);
-  for (int s = 1; s <= max_stage; s++) {
-    const int q = s - 1 + off;
+  for (int s = 1, pass = 0; s <= max_stage; s += 2, pass++) {
+    const int q = pass + off;
     const hb::PairList in{lists[q & 1], list_cnt + q, list_cap};
     const hb::PairList next{lists[(q + 1) & 1], list_cnt + q + 1, list_cap};
-    const hb::PairList defer{defer_buf, defer_cnt + s, defer_cap};
+    const hb::PairList defer{defer_buf, defer_cnt + pass, defer_cap};
     hb::launch_exact(bd, in, s, next, fin, defer, aud, s == 1 && !any_refine, may_defer,
                      (uint2 *)d_pairs, d_audit, ctx->n_cu, st);
   }
