@@ -319,12 +319,21 @@ class Context:
         check(lib().hb_table_from_device(self._h, C.byref(table.view), C.byref(h)))
         return DeviceTable(self, table, _handle=h, _n_rows=table.n_rows)
 
-    def upload_serialized(self, table):
+    def upload_serialized(self, table, rows=None):
         """hb_table_upload_serialized of a SerializedTable (the reference's
-        row-major handoff)."""
+        row-major handoff), or of its rows `rows` in that order
+        (hb_table_upload_serialized_rows: a partition)."""
         h = C.c_void_p()
-        check(lib().hb_table_upload_serialized(self._h, C.byref(table.view), C.byref(h)))
-        return DeviceTable(self, table, _handle=h, _n_rows=table.n_rows)
+        if rows is None:
+            check(lib().hb_table_upload_serialized(self._h, C.byref(table.view), C.byref(h)))
+            n = table.n_rows
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.int64)
+            check(lib().hb_table_upload_serialized_rows(
+                self._h, C.byref(table.view), rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                len(rows), C.byref(h)))
+            n = len(rows)
+        return DeviceTable(self, table, _handle=h, _n_rows=n)
 
     def plan(self, host_plan):
         return DevicePlan(self, host_plan)

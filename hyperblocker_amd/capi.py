@@ -29,7 +29,7 @@ EXPORTS = [
     "hb_ctx_create", "hb_ctx_destroy", "hb_ctx_synchronize", "hb_ctx_memory_info",
     "hb_ctx_set_filter_shape",
     "hb_ctx_get_filter_shape", "hb_last_error", "hb_version",
-    "hb_table_upload", "hb_table_upload_serialized", "hb_table_from_device", "hb_table_upload_rows",
+    "hb_table_upload", "hb_table_upload_serialized", "hb_table_upload_serialized_rows", "hb_table_from_device", "hb_table_upload_rows",
     "hb_table_slice", "hb_table_info", "hb_table_release",
     "hb_plan_create", "hb_plan_release",
     "hb_job_submit", "hb_batch_submit", "hb_batch_graph_create", "hb_batch_graph_run",
@@ -124,6 +124,8 @@ def lib():
         "hb_table_upload": ([VP, C.POINTER(TableView), PVP], st),
         "hb_table_upload_serialized": ([VP, C.POINTER(SerializedTable), PVP], st),
         "hb_table_from_device": ([VP, C.POINTER(SerializedTable), PVP], st),
+        "hb_table_upload_serialized_rows": ([VP, C.POINTER(SerializedTable), C.POINTER(C.c_int64),
+                                             C.c_int64, PVP], st),
         "hb_table_upload_rows": ([VP, C.POINTER(TableView), C.POINTER(C.c_int64), C.c_int64,
                                   PVP], st),
         "hb_table_slice": ([VP, VP, C.c_int64, C.c_int64, PVP], st),

@@ -273,6 +273,8 @@ void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, 
 // (little endian), zero padded to width (>= 1 + 4k, multiple of 16).
 void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t width,
                          hipStream_t stream);
+// code[r] = kNoCode for the rows with some column code kNoCode.
+void launch_mask_keys(KeyColumns cols, uint32_t n, uint32_t *code, hipStream_t stream);
 // Unpacks n_rows rows of row_bytes bytes (the reference's row-major table on
 // the device) into the columns' NUL-padded cells: each cell is copied up to
 // its first NUL or `size` bytes, the rest of its width zeroed.  Work-groups

@@ -667,6 +667,16 @@ __global__ void k_compose_keys(KeyColumns cols, uint32_t n, uint8_t *__restrict_
   for (; w < (width >> 2); w++) dst[w] = 0u;
 }
 
+// Composite codes of rows with a column code still left to the host
+// (kNoCode) become kNoCode: the layouts built from them are flagged bad.
+__global__ void k_mask_keys(KeyColumns cols, uint32_t n, uint32_t *__restrict__ code) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  bool pending = false;
+  for (uint32_t q = 0; q < cols.k; q++) pending |= cols.code[q][r] == kNoCode;
+  if (pending) code[r] = kNoCode;
+}
+
 // A work-group unpacks blocks of rows_per_group consecutive rows (grid
 // stride).  A block is one contiguous span of the input: it is loaded into
 // LDS (16-byte loads when the rows are 16-byte aligned), each cell's length
@@ -2296,6 +2306,11 @@ void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t w
                          hipStream_t stream) {
   if (!n) return;
   k_compose_keys<<<(n + 255) / 256, 256, 0, stream>>>(cols, n, cells, width);
+}
+
+void launch_mask_keys(KeyColumns cols, uint32_t n, uint32_t *code, hipStream_t stream) {
+  if (!n) return;
+  k_mask_keys<<<(n + 255) / 256, 256, 0, stream>>>(cols, n, code);
 }
 
 void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,

@@ -632,6 +632,7 @@ struct hb_ctx {
     // that every code is below the count already known).
     std::mutex stable_mu;
     std::unordered_map<int, int> stable;
+    std::map<std::vector<int>, int> stable_key;  // the same per list of key columns
   } ddict;
 };
 
@@ -681,6 +682,19 @@ struct Column {
 struct Composite {
   Lazy code;
   uint32_t *d_code = nullptr;
+  // A composite coded without waiting: its key cells, the device's list of
+  // rows for the host dictionary, the pinned (their count, codes handed
+  // out), and whether every column's codes were final when it was composed
+  // (else rows whose column code was still left to the host come out as
+  // kNoCode, and the composite is coded again once the columns are final).
+  bool validated = false;
+  bool cols_final = false;
+  uint8_t *d_cells = nullptr;
+  uint32_t width = 0, key_len = 0;
+  uint32_t *d_fb_rows = nullptr;
+  unsigned long long *d_fb_count = nullptr;
+  unsigned long long *h_status = nullptr;
+  uint64_t bound_at_launch = 0;
 };
 
 // The rows of a table in join order for one rule shape: sorted by the join
@@ -1141,8 +1155,8 @@ hb_status dict_finish(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_
     const char *cell = host_cells.data() + (size_t)r * width;
     const std::string_view v(cell, fixed_len ? fixed_len : strnlen(cell, width));
     const uint32_t c = ctx->dict.code_one(v, Dictionary::hash(v));
-    if (c >= hb::kFallbackCodeBase)
-      return fail(HB_ERR_CAPACITY, "dictionary: more than 2^31 distinct values");
+    if (c >= hb::kFallbackCodeBase - 1)  // kFallbackCodeBase | c must not be kNoCode
+      return fail(HB_ERR_CAPACITY, "dictionary: more than 2^31 - 1 distinct values");
     codes[r] = hb::kFallbackCodeBase | c;
   }
   e = hipMemcpy(d_code, codes.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
@@ -1249,9 +1263,57 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream, bool may_defer) {
   return wait_lazy(col.code, stream);
 }
 
+// Finishes the coding of a root table's composite key (waiting for it if it
+// still runs): its columns' codings, its own rows left to the host, and, if
+// a column's rows were left to the host after the composite was composed,
+// the composite again, from the final codes, into a fresh buffer (layouts
+// built from the old one saw those rows as kNoCode and are flagged bad).
+// Updates the prediction for this list of columns.  Called with t->mu held.
+hb_status validate_key_locked(hb_table *t, const std::vector<int> &cols, Composite &cp) {
+  if (cp.validated || !cp.code.built) return HB_OK;
+  hb_ctx *ctx = t->ctx;
+  bool redo = false;
+  for (int c : cols) {
+    HB_TRY(validate_code_locked(t, c));
+    redo |= !cp.cols_final && t->cols[c].h_status[0] != 0;
+  }
+  const size_t n = (size_t)t->n_rows;
+  HIP_OK(hipEventSynchronize(cp.code.ev));
+  HB_TRY(dict_finish(ctx, cp.d_cells, cp.width, cp.key_len, n, cp.d_code, cp.d_fb_rows,
+                     cp.h_status));
+  cp.validated = true;
+  const bool added = (cp.h_status[1] & 0xffffffffull) > cp.bound_at_launch ||
+                     cp.h_status[0] != 0 || redo;
+  {
+    std::lock_guard<std::mutex> lk(ctx->ddict.stable_mu);
+    int &st = ctx->ddict.stable_key[cols];
+    st = added ? 0 : st + 1;
+  }
+  if (!redo) return HB_OK;
+  hb::KeyColumns kc{};
+  kc.k = (uint32_t)cols.size();
+  for (size_t q = 0; q < cols.size(); q++) kc.code[q] = t->cols[cols[q]].d_code;
+  uint32_t *fresh = nullptr;
+  HB_TRY(t->dev_alloc(n * sizeof(uint32_t), (void **)&fresh));
+  const hipStream_t cs = ctx->code_stream;
+  hb::launch_compose_keys(kc, (uint32_t)n, cp.d_cells, cp.width, cs);
+  HB_TRY(dict_code(ctx, cp.d_cells, cp.width, cp.key_len, n, fresh, cs));
+  cp.d_code = fresh;
+  return record_lazy(cp.code, cs);
+}
+
+hb_status validate_key(hb_table *t, const std::vector<int> &cols) {
+  while (t->parent) t = t->parent;
+  std::lock_guard<std::mutex> lk(t->mu);
+  auto it = t->composites.find(cols);
+  return it == t->composites.end() ? HB_OK : validate_key_locked(t, cols, it->second);
+}
+
 // Join key codes of the equality columns `cols`: one column's codes, or for
 // several the composite codes (the dictionary codes of each row's tuple of
-// column codes), so that equal keys mean every listed cell equal.
+// column codes), so that equal keys mean every listed cell equal.  Like a
+// column, a composite whose last coding for these columns added no code is
+// coded without waiting; its layouts check its codes.
 hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stream,
                      const uint32_t **out) {
   if (cols.size() == 1) {
@@ -1268,32 +1330,49 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
     *out = pc + t->row_off;
     return HB_OK;
   }
-  // the tuple of column codes is coded: they must be final first
-  for (int c : cols) HB_TRY(ensure_code(t, c, t->ctx->code_stream, false));
+  hb_ctx *ctx = t->ctx;
+  bool defer = false;
+  {
+    std::lock_guard<std::mutex> sl(ctx->ddict.stable_mu);
+    auto it = ctx->ddict.stable_key.find(cols);
+    defer = it != ctx->ddict.stable_key.end() && it->second > 0;
+  }
+  // the tuple of column codes is coded on the code stream, after them
+  const hipStream_t cs = ctx->code_stream;
+  for (int c : cols) HB_TRY(ensure_code(t, c, cs, defer));
   std::lock_guard<std::mutex> lk(t->mu);
   Composite &cp = t->composites[cols];
   if (cp.code.built) {
+    if (!defer) HB_TRY(validate_key_locked(t, cols, cp));
     *out = cp.d_code;
     return wait_lazy(cp.code, stream);
   }
   const size_t n = (size_t)t->n_rows;
-  HB_TRY(t->dev_alloc(std::max<size_t>(n, 1) * sizeof(uint32_t), (void **)&cp.d_code));
-  if (n) {
-    const uint32_t key_len = 1 + 4 * (uint32_t)cols.size();
-    const uint32_t width = (key_len + 15) & ~15u;
-    hb::KeyColumns kc{};
-    kc.k = (uint32_t)cols.size();
-    for (size_t q = 0; q < cols.size(); q++) kc.code[q] = t->cols[cols[q]].d_code;
-    uint8_t *cells = nullptr;
-    const hipStream_t cs = t->ctx->code_stream;
-    HB_TRY(t->ctx->pool.alloc(n * width, (void **)&cells));
-    hb::launch_compose_keys(kc, (uint32_t)n, cells, width, cs);
-    const hb_status st = dict_code(t->ctx, cells, width, key_len, n, cp.d_code, cs);
-    t->ctx->pool.free(cells, n * width);  // dict_code waited for the device
-    HB_TRY(st);
+  const size_t n1 = std::max<size_t>(n, 1);
+  cp.key_len = 1 + 4 * (uint32_t)cols.size();
+  cp.width = (cp.key_len + 15) & ~15u;
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&cp.d_code));
+  HB_TRY(t->dev_alloc(n1 * cp.width, (void **)&cp.d_cells));
+  HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&cp.d_fb_rows));
+  HB_TRY(t->dev_alloc(sizeof(unsigned long long), (void **)&cp.d_fb_count));
+  HB_TRY(t->host_alloc(2 * sizeof(unsigned long long), (void **)&cp.h_status));
+  hb::KeyColumns kc{};
+  kc.k = (uint32_t)cols.size();
+  cp.cols_final = true;
+  for (size_t q = 0; q < cols.size(); q++) {
+    kc.code[q] = t->cols[cols[q]].d_code;
+    cp.cols_final &= t->cols[cols[q]].validated;
   }
+  cp.bound_at_launch = ctx->ddict.code_bound.load();
+  if (n) hb::launch_compose_keys(kc, (uint32_t)n, cp.d_cells, cp.width, cs);
+  HB_TRY(dict_launch(ctx, cp.d_cells, cp.width, cp.key_len, n, cp.d_code, cp.d_fb_rows,
+                     cp.d_fb_count, cp.h_status, cs));
+  if (n && !cp.cols_final) hb::launch_mask_keys(kc, (uint32_t)n, cp.d_code, cs);
+  HIP_OK(hipGetLastError());
+  HB_TRY(record_lazy(cp.code, cs));
+  if (!defer) HB_TRY(validate_key_locked(t, cols, cp));
   *out = cp.d_code;
-  return record_lazy(cp.code, t->ctx->code_stream);
+  return wait_lazy(cp.code, stream);
 }
 
 // Runs of a layout for sizing a batch's groups: the count once its build is
@@ -1348,7 +1427,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   trace.mark("sig");
   uint64_t n_codes = 1;
   if (!eq.empty())
-    n_codes = t->ctx->ddict.fallback_used.load() ? (1ull << 32)
+    n_codes = t->ctx->ddict.fallback_used.load() ? (uint64_t)hb::kNoCode  // kNoCode stays bad
                                                  : std::max<uint64_t>(1, t->ctx->ddict.code_bound.load());
   std::lock_guard<std::mutex> lk(t->mu);
   std::unique_ptr<Layout> &slot = t->layouts[{eq, sim}];
@@ -1474,6 +1553,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
 // run again).  Waits for the device.
 hb_status revalidate_layout(hb_table *t, const std::vector<int> &eq, int sim) {
   if (eq.size() == 1) HB_TRY(validate_code(t, eq[0]));
+  if (eq.size() > 1) HB_TRY(validate_key(t, eq));
   Layout *ly = nullptr;
   {
     std::lock_guard<std::mutex> lk(t->mu);
@@ -3013,14 +3093,32 @@ hb_status hb_table_upload(hb_ctx *ctx, const hb_table_view *v, hb_table **out) {
 }
 
 hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *s, hb_table **out) {
+  return hb_table_upload_serialized_rows(ctx, s, nullptr, 0, out);
+}
+
+hb_status hb_table_upload_serialized_rows(hb_ctx *ctx, const hb_serialized_table *s,
+                                          const int64_t *rows, int64_t n_rows, hb_table **out) {
   if (!ctx || !out) return fail(HB_ERR_INVALID_ARG, "hb_table_upload_serialized: null argument");
   *out = nullptr;
   HB_TRY(check_serialized(s));
+  if (rows) {
+    if (n_rows < 0 || n_rows > 0xFFFFFFF0ll)
+      return fail(HB_ERR_INVALID_ARG, "hb_table_upload_serialized_rows: bad row count %lld",
+                  (long long)n_rows);
+    for (int64_t i = 0; i < n_rows; i++)
+      if (rows[i] < 0 || (size_t)rows[i] >= s->n_rows)
+        return fail(HB_ERR_INVALID_ARG, "hb_table_upload_serialized_rows: row %lld out of range",
+                    (long long)rows[i]);
+  } else {
+    n_rows = (int64_t)s->n_rows;
+  }
   HB_TRY(set_device(ctx));
   HostTrace trace("table_upload_serialized");
-  // The rows as they are: one parallel copy into pinned staging, one copy
-  // to the device, then the cells are unpacked there.
-  const size_t bytes = std::max<size_t>(s->n_rows * s->aligned_tuple_size, 16);
+  // The rows as they are (or the listed rows, gathered in that order): one
+  // parallel copy into pinned staging, one copy to the device, then the
+  // cells are unpacked there.
+  const size_t ats = s->aligned_tuple_size;
+  const size_t bytes = std::max<size_t>((size_t)n_rows * ats, 16);
   hb_ctx::Grave temps;
   char *h_rows = nullptr;
   uint8_t *d_rows = nullptr;
@@ -3032,13 +3130,25 @@ hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *s, 
     return st;
   }
   temps.dev.push_back({d_rows, bytes});
-  parallel_for((int64_t)bytes, 1 << 20, [&](int64_t b, int64_t e) {
-    std::memcpy(h_rows + b, s->data + b, (size_t)(e - b));
-  });
+  if (!rows) {
+    parallel_for((int64_t)bytes, 1 << 20, [&](int64_t b, int64_t e) {
+      std::memcpy(h_rows + b, s->data + b, (size_t)(e - b));
+    });
+  } else {  // a partition's rows: one row copy each, source rows fetched ahead
+    constexpr int64_t kAhead = 8;
+    parallel_for(n_rows, 1024, [&](int64_t b, int64_t e) {
+      for (int64_t i = b; i < e; i++) {
+        if (i + kAhead < e) __builtin_prefetch(s->data + (size_t)rows[i + kAhead] * ats);
+        std::memcpy(h_rows + (size_t)i * ats, s->data + (size_t)rows[i] * ats, ats);
+      }
+    });
+  }
   trace.mark("stage");
+  hb_serialized_table part = *s;
+  part.n_rows = (size_t)n_rows;
   std::unique_ptr<hb_table> t;
   hipError_t e = hipMemcpyAsync(d_rows, h_rows, bytes, hipMemcpyHostToDevice, ctx->copy_stream);
-  if (e == hipSuccess) st = unpack_table(ctx, s, d_rows, ctx->copy_stream, t);
+  if (e == hipSuccess) st = unpack_table(ctx, &part, d_rows, ctx->copy_stream, t);
   if (e != hipSuccess || st != HB_OK) {
     (void)hipStreamSynchronize(ctx->copy_stream);
     for (auto &m : temps.host) ctx->hpool.free(m.first, m.second);

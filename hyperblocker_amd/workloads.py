@@ -24,6 +24,7 @@ partitioning: host packing, host-to-device copies, kernels and result copies.
 Nothing here computes a predicate: every job runs through the C ABI.
 """
 import collections
+import concurrent.futures
 import math
 import random
 import threading
@@ -310,6 +311,11 @@ def run_c5(ctx, n_jobs=1500, load=0.7, log=print):
 
 
 def run_c3(ctx, inflight=3, reps=2, log=print):
+    """c3 streamed: each hash partition's rows are uploaded while earlier
+    partitions run.  Two handoffs are timed: column arrays gathered and
+    packed by the host (hb_table_upload_rows), and the reference's
+    row-major SerializedTable, one row copy per partition row with the cells
+    unpacked on the device (hb_table_upload_serialized_rows)."""
     t = time.perf_counter()
     L, R, _, cfg = datagen.make("c3")
     lt, rt = hbk.ColumnTable(L), hbk.ColumnTable(R)
@@ -320,13 +326,27 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
     part_s = time.perf_counter() - t
     pairs_total = sum(len(a) * len(b) for a, b in parts)
     log("c3: %d partitions (%.3g pairs) in %.2f s" % (len(parts), pairs_total, part_s))
+    sl, sr = hbk.SerializedTable(L), hbk.SerializedTable(R)
     dp = ctx.plan(plan)
+    uploads = {
+        "columns": lambda lr, rr: (ctx.upload(lt, rows=lr), ctx.upload(rt, rows=rr)),
+        "serialized": lambda lr, rr: (ctx.upload_serialized(sl, rows=lr),
+                                      ctx.upload_serialized(sr, rows=rr)),
+    }
 
-    def stream():
+    def stream(upload):
+        """Partitions in order: a producer thread uploads up to two
+        partitions ahead (host_producer.h's role) while this thread submits
+        and drains; upload_s is the producer's time, overlapped."""
         t0 = time.perf_counter()
         pending = collections.deque()
         out = {"matches": 0, "device_ms": 0.0, "upload_s": 0.0, "submit_s": 0.0,
                "drain_s": 0.0, "candidates": 0, "jw_exact": 0, "audit": 0}
+
+        def produce(lr, rr):
+            tu = time.perf_counter()
+            tables = upload(lr, rr)
+            return tables, time.perf_counter() - tu
 
         def drain():
             td = time.perf_counter()
@@ -342,38 +362,54 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
             out["jw_exact"] += st["jw_exact"]
             out["drain_s"] += time.perf_counter() - td
 
-        for lr, rr in parts:
-            tu = time.perf_counter()
-            tl, tr = ctx.upload(lt, rows=lr), ctx.upload(rt, rows=rr)
-            ts = time.perf_counter()
-            out["upload_s"] += ts - tu
-            pending.append((ctx.submit(dp, tl, tr), tl, tr))
-            out["submit_s"] += time.perf_counter() - ts
-            if len(pending) >= inflight:
-                drain()
+        with concurrent.futures.ThreadPoolExecutor(1) as producer:
+            ahead = collections.deque(producer.submit(produce, lr, rr) for lr, rr in parts[:2])
+            nxt = 2
+            while ahead:
+                (tl, tr), du = ahead.popleft().result()
+                out["upload_s"] += du
+                if nxt < len(parts):
+                    ahead.append(producer.submit(produce, *parts[nxt]))
+                    nxt += 1
+                ts = time.perf_counter()
+                pending.append((ctx.submit(dp, tl, tr), tl, tr))
+                out["submit_s"] += time.perf_counter() - ts
+                if len(pending) >= inflight:
+                    drain()
         while pending:
             drain()
         out["wall_s"] = time.perf_counter() - t0
         return out
 
     try:
-        runs = []
-        for k in range(reps + 1):
-            r = stream()
-            log("c3: stream %d: %.3f s (host: uploads %.3f s, submits %.3f s, drains %.3f s;"
-                " device %.1f ms)" % (k, r["wall_s"], r["upload_s"], r["submit_s"], r["drain_s"],
-                                      r["device_ms"]))
-            runs.append(r)
-        best = min(runs[1:], key=lambda r: r["wall_s"])
-        return {
-            "partitions": len(parts), "pairs": pairs_total, "partition_s": part_s,
-            "streamed_s": best["wall_s"], "streamed_pairs_per_s": pairs_total / best["wall_s"],
-            "first_stream_s": runs[0]["wall_s"], "device_ms": best["device_ms"],
-            "device_pairs_per_s": pairs_total / (best["device_ms"] * 1e-3),
-            "host_upload_s": best["upload_s"], "host_submit_s": best["submit_s"],
-            "host_drain_s": best["drain_s"], "matches": best["matches"],
-            "candidates": best["candidates"], "jw_exact": best["jw_exact"],
-            "audit": best["audit"], "inflight": inflight,
-        }
+        out = {"partitions": len(parts), "pairs": pairs_total, "partition_s": part_s,
+               "inflight": inflight}
+        for handoff, upload in uploads.items():
+            runs = []
+            for k in range(reps + 1):
+                r = stream(upload)
+                log("c3 (%s): stream %d: %.3f s (host: uploads %.3f s, submits %.3f s, "
+                    "drains %.3f s; device %.1f ms)" % (handoff, k, r["wall_s"], r["upload_s"],
+                                                       r["submit_s"], r["drain_s"],
+                                                       r["device_ms"]))
+                runs.append(r)
+            best = min(runs[1:], key=lambda r: r["wall_s"])
+            res = {
+                "streamed_s": best["wall_s"], "streamed_pairs_per_s": pairs_total / best["wall_s"],
+                "first_stream_s": runs[0]["wall_s"], "device_ms": best["device_ms"],
+                "device_pairs_per_s": pairs_total / (best["device_ms"] * 1e-3),
+                "host_upload_s": best["upload_s"], "host_submit_s": best["submit_s"],
+                "host_drain_s": best["drain_s"], "matches": best["matches"],
+                "candidates": best["candidates"], "jw_exact": best["jw_exact"],
+                "audit": best["audit"],
+            }
+            if handoff == "columns":
+                out.update(res)
+            else:
+                out.update({"serialized_" + k: v for k, v in res.items()})
+        if out["serialized_matches"] != out["matches"] or \
+                out["serialized_audit"] != out["audit"]:
+            raise RuntimeError("c3: the two handoffs gave different results")
+        return out
     finally:
         dp.release()

@@ -168,6 +168,12 @@ hb_status hb_table_upload(hb_ctx *ctx, const hb_table_view *view, hb_table **out
  * the device.  Cell (r, c) ends at its first NUL or after col_size[c] bytes. */
 hb_status hb_table_upload_serialized(hb_ctx *ctx, const hb_serialized_table *t,
                                      hb_table **out);
+/* The same for rows[0..n_rows) of the SerializedTable, in that order: a
+ * partition's re-serialisation (DataPartitioning, data_mngr.h:209-215) and
+ * its host-to-device copy (AsyncSubmit, host_producer.h:137-152) in one
+ * call, one row copy per row. */
+hb_status hb_table_upload_serialized_rows(hb_ctx *ctx, const hb_serialized_table *t,
+                                          const int64_t *rows, int64_t n_rows, hb_table **out);
 /* The reference's device-side table as blocking_kernel receives it
  * (d_tb_data with its col_size / col_offset, core/gpu/global_func.cuh:104-110,
  * after AsyncSubmit's host-to-device copy, host_producer.h:137-152): `data`

@@ -114,6 +114,41 @@ def test_unpack_cells_ending_at_nul_or_col_size(ctx, junk):
         dev.free()
 
 
+@pytest.mark.parametrize("n_pick", [0, 1, 217])
+def test_serialized_partition_rows(ctx, n_pick):
+    """hb_table_upload_serialized_rows: rows listed out of order (a
+    partition of the reference's serialized table) give the oracle's pairs
+    and audit entries on those rows, in the listed order; a row index past
+    the table is an error."""
+    rng = np.random.default_rng(n_pick)
+    alpha = b"abcde\xe9"
+    rows = [(str(i).encode(), bytes(alpha[j] for j in rng.integers(0, 6, rng.integers(0, 15))),
+             bytes(alpha[j] for j in rng.integers(0, 6, rng.integers(0, 9))),
+             str(int(rng.integers(0, 4))).encode()) for i in range(400)]
+    cols = H.columns_from_rows(rows)
+    raw = _RawTable(cols, [4, 14, 9, 2], b"q\x02")
+    pick_l = rng.permutation(400)[:n_pick]
+    pick_r = rng.permutation(400)[:max(n_pick - 17, 0)]
+    plan = hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.6), (2, 0.5)]),
+                                  datagen.rule_yaml([], [(1, 0.85)])])
+    want, want_audit, _ = H.run_oracle(plan, [c[pick_l] for c in cols],
+                                       [c[pick_r] for c in cols])
+    dp = ctx.plan(plan)
+    try:
+        tl = ctx.upload_serialized(raw, rows=pick_l)
+        tr = ctx.upload_serialized(raw, rows=pick_r)
+        got, audit, _ = ctx.block(dp, tl, tr)
+        tl.release()
+        tr.release()
+        H.assert_same_pairs(got, want, "partition rows")
+        H.assert_same_audit(H.sort_audit(audit), want_audit, "partition rows")
+        with pytest.raises(hb.HBError) as e:
+            ctx.upload_serialized(raw, rows=[0, 400])
+        assert e.value.status == capi.HB_ERR_INVALID_ARG
+    finally:
+        dp.release()
+
+
 def test_serialized_table_rejects_bad_layout(ctx):
     raw = _RawTable(H.columns_from_rows([(b"1", b"ab")]), [2, 3])
     raw._size[1] = 200  # past aligned_tuple_size
@@ -198,17 +233,19 @@ def test_fused_column_build_equals_separate_kernels():
     H.assert_same_pairs(got["fused"][1][0], want, "refine plan vs oracle")
 
 
+@pytest.mark.parametrize("plan_name", ["eq_sim", "two_eq_two_sim"])
 @pytest.mark.parametrize("env", [{}, {"HB_DICT_HASH_BITS": "3"}])
-def test_deferred_coding_reruns_when_codes_change(env):
-    """A key column whose last codings added no value is coded without the
-    host waiting; layouts built from it check every code against the count
-    then known.  Fresh tables with the same key values run without a rerun;
-    a table bringing new key values (and, with a 3-bit hash, rows the host
-    must code) flags its layout, which is rebuilt, and the batch runs again:
-    every result equals the oracle's."""
+def test_deferred_coding_reruns_when_codes_change(env, plan_name):
+    """A key column (or list of key columns, coded as a composite) whose
+    last codings added no value is coded without the host waiting; layouts
+    built from it check every code against the count then known.  Fresh
+    tables with the same key values run without a rerun; a table bringing
+    new key values (and, with a 3-bit hash, rows the host must code) flags
+    its layout, which is rebuilt, and the batch runs again: every result
+    equals the oracle's."""
     c = _ctx_with(env)
     try:
-        plan = hb.HostPlan.from_yaml(PLANS["eq_sim"])
+        plan = hb.HostPlan.from_yaml(PLANS[plan_name])
         dp = c.plan(plan)
         same = [_adversarial_tables(300 + k, 700, 600, b"abcd", 20, n_keys=5) for k in range(4)]
         new = _adversarial_tables(399, 700, 600, b"abcd", 20, n_keys=60)
@@ -226,5 +263,38 @@ def test_deferred_coding_reruns_when_codes_change(env):
             assert retries[4] >= 1, retries  # the new values were caught and the job rerun
             assert retries[2] == 0 and retries[3] == 0, retries
         dp.release()
+    finally:
+        c.close()
+
+
+def test_deferred_coding_after_host_fallback():
+    """A context whose device dictionary is full (64 slots): once some
+    column's rows were coded by the host, a key column predicted stable
+    (coded without waiting) that meets new values has rows still left to the
+    host when its layout is built.  Those rows are flagged, the layout is
+    rebuilt once their codes are final, and the results equal the oracle's,
+    for a single key column and for a composite."""
+    c = _ctx_with({"HB_DICT_SLOTS": "64", "HB_DICT_MAX_PROBE": "64"})
+    try:
+        few = [_adversarial_tables(500 + k, 500, 400, b"abcd", 20, n_keys=4) for k in range(2)]
+        strings = _adversarial_tables(520, 500, 400, b"abcdefgh", 30)
+        many = _adversarial_tables(530, 500, 400, b"abcd", 20, n_keys=150)
+        steps = [(PLANS[p], t) for p in ("eq_sim", "two_eq_two_sim") for t in few]
+        steps += [([datagen.rule_yaml([1], [])], strings)]            # host codes most of column 1
+        steps += [(PLANS["eq_sim"], many), (PLANS["two_eq_two_sim"], many)]
+        retries = []
+        for k, (rules, (L, R)) in enumerate(steps):
+            plan = hb.HostPlan.from_yaml(rules)
+            dp = c.plan(plan)
+            tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
+            got, audit, st = c.block(dp, tl, tr)
+            tl.release()
+            tr.release()
+            dp.release()
+            want, want_audit, _ = H.run_oracle(plan, L, R)
+            H.assert_same_pairs(got, want, "step %d" % k)
+            H.assert_same_audit(H.sort_audit(audit), want_audit, "step %d" % k)
+            retries.append(st["retries"])
+        assert retries[-2] >= 1 and retries[-1] >= 1, retries
     finally:
         c.close()
