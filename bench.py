@@ -160,7 +160,7 @@ def job_set_bench(args):
 
     with hb.Context(0) as ctx:
         if args.workload == "c3":
-            r = workloads.run_c3(ctx, log=log)
+            r = workloads.run_c3(ctx, reps=args.reps, log=log)
             value, desc = r["serialized_streamed_pairs_per_s"], (
                 "c3: 1M x 1M rows, 8 columns, 3 rules, %d hash partitions streamed from "
                 "row-major serialized tables (uploads overlapped with compute)" % r["partitions"])
@@ -202,6 +202,8 @@ def main():
                     help="skip the served-vs-plain trace (c5) in the default run")
     ap.add_argument("--serve-jobs", type=int, default=1500,
                     help="jobs of the served trace in the default run")
+    ap.add_argument("--reps", type=int, default=2,
+                    help="timed passes over c3's partitions per handoff (after one untimed)")
     ap.add_argument("--jobs", type=int, default=0,
                     help="jobs of the c4/c5 job-set workloads (0 = their default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

@@ -1909,18 +1909,29 @@ __device__ __forceinline__ int lane_matches(const uint32_t *row_planes, int nwc,
   return lane_greedy<NW, false>(P, S, h, n_out, outer, n_steps);
 }
 
-// JW bound of a pair for one similarity predicate with 128-class counters:
-// false only when JW is certainly below thr - 2e-6 (the test of may_pass on
-// a tighter m_ub: a matched pair of bytes is a pair of equal bytes, so
-// m <= sum over classes of min(count_a, count_b) for any class partition).
-__device__ __forceinline__ bool hist_pass(const SimDesc &sd, uint32_t l, uint32_t r) {
-  const SigRow sa = sd.col_l.sig[l], sb = sd.col_r.sig[r];
-  const uint32_t la = sa.len & 0xffffu, lb = sb.len & 0xffffu;
-  if (((sa.len | sb.len) & kLongFlag) || la == 0 || lb == 0) return true;
-  const uint32_t x = sa.pre ^ sb.pre;
-  if (x == 0) return true;
-  const uint4 *ha = reinterpret_cast<const uint4 *>(sd.col_l.hist + (uint64_t)l * kHistWords);
-  const uint4 *hb = reinterpret_cast<const uint4 *>(sd.col_r.hist + (uint64_t)r * kHistWords);
+// The refine test of one pair (l, r) against similarity predicate sd: the
+// bound of may_pass with 128-class counters, which is false only when JW is
+// certainly below thr - 2e-6 (a matched pair of bytes is a pair of equal
+// bytes, so m <= sum over classes of min(count_a, count_b) for any class
+// partition).  Both rows' counters are loaded before the early outs (long
+// or empty cells, equal first 4 bytes), so the loads go out together.
+struct RefineDesc {
+  const SigRow *sig_l, *sig_r;
+  const uint32_t *hist_l, *hist_r;
+  float c0, c1, c2, c3;
+};
+
+template <class SD>
+__device__ __forceinline__ RefineDesc refine_desc(SD &sd) {
+  return RefineDesc{sd.col_l.sig, sd.col_r.sig, sd.col_l.hist, sd.col_r.hist,
+                    sd.c0,        sd.c1,        sd.c2,         sd.c3};
+}
+
+__device__ __forceinline__ bool hist_pass(const RefineDesc &sd, uint32_t l, uint32_t r) {
+  const uint2 sa = *reinterpret_cast<const uint2 *>(&sd.sig_l[l].pre);  // (pre, len)
+  const uint2 sb = *reinterpret_cast<const uint2 *>(&sd.sig_r[r].pre);
+  const uint4 *ha = reinterpret_cast<const uint4 *>(sd.hist_l + (uint64_t)l * kHistWords);
+  const uint4 *hb = reinterpret_cast<const uint4 *>(sd.hist_r + (uint64_t)r * kHistWords);
   uint32_t sad = 0;
 #pragma unroll
   for (int k = 0; k < kHistWords / 4; k++) {
@@ -1930,6 +1941,9 @@ __device__ __forceinline__ bool hist_pass(const SimDesc &sd, uint32_t l, uint32_
     sad = __builtin_amdgcn_sad_u8(u.z, v.z, sad);
     sad = __builtin_amdgcn_sad_u8(u.w, v.w, sad);
   }
+  const uint32_t la = sa.y & 0xffffu, lb = sb.y & 0xffffu;
+  const uint32_t x = sa.x ^ sb.x;
+  if (((sa.y | sb.y) & kLongFlag) || la == 0 || lb == 0 || x == 0) return true;
   const uint32_t mub = (la + lb - sad) >> 1;
   const uint32_t S = min(la, lb), T = max(la, lb);
   const uint32_t p = ((uint32_t)(__ffs(x) - 1)) >> 3;
@@ -1938,33 +1952,83 @@ __device__ __forceinline__ bool hist_pass(const SimDesc &sd, uint32_t l, uint32_
 }
 
 // One pair per lane: the first similarity predicate of the pair's rule must
-// pass hist_pass (JRs with `refine` set).  Only the first: the predicates are evaluated in order, and
-// a later one failing does not spare an earlier one's evaluation (or its
-// audit entry).  Rules without similarity predicates pass through.
+// pass hist_pass (JRs with `refine` set).  Only the first: the predicates
+// are evaluated in order, and a later one failing does not spare an earlier
+// one's evaluation (or its audit entry).  Rules without similarity
+// predicates pass through.
+//
+// A work-group takes kRefineSpan consecutive entries at a time: entries of
+// one filter flush are consecutive and share the rows of one tile, so the
+// rows' counters are read again from the group's L2 rather than from HBM.
+// Kept entries collect in a per-wave LDS buffer (one list atomic per up to
+// kWaveOut entries), and filter counts in a per-wave running sum for the
+// current job (one atomic when the job changes and at the end): single
+// global counters taking an atomic from every wave are what limited this
+// kernel otherwise.  Where a wave's 64 entries share one JR, its
+// descriptors are read with scalar loads.
+constexpr uint64_t kRefineSpan = 1024;
+
 __global__ __launch_bounds__(256) void k_refine(const BatchDev b, const PairList in,
                                                 const PairList out) {
+  __shared__ PairEntry s_out[4][kWaveOut];
+  PairEntry *wbuf = s_out[threadIdx.x >> 6];
+  uint32_t wn = 0;  // entries in wbuf (wave-uniform)
+  uint32_t cur_job = 0;
+  unsigned long long cur_cnt = 0;  // filter survivors of cur_job not yet counted
+  auto count_flush = [&]() {
+    if (cur_cnt && lane_id() == 0)
+      atomicAdd(b.job_cnt + (size_t)job_row(kJobFiltered, blockIdx.x & (kStatStripes - 1)) *
+                                b.cnt_stride + cur_job,
+                cur_cnt);
+    cur_cnt = 0;
+  };
   const unsigned long long n_in = *in.count;
   const unsigned long long n_use = n_in < in.cap ? n_in : in.cap;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint64_t q0 = base0; q0 < n_use; q0 += stride) {
-    const uint64_t idx = q0 + lane_id();
-    const bool act = idx < n_use;
-    PairEntry e{0u, 0u, 0u, 0u};
-    uint32_t job = 0, sim_off = 0, refine = 0;
-    int32_t n_sims = 0;
-    if (act) {
-      e = in.e[idx];
-      const JRDesc &d = b.jrs[e.jr];
-      job = d.job;
-      n_sims = d.n_sims;
-      sim_off = d.sim_off;
-      refine = d.refine;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint64_t g0 = (uint64_t)blockIdx.x * kRefineSpan; g0 < n_use;
+       g0 += (uint64_t)gridDim.x * kRefineSpan) {
+    const uint64_t g_end = min(g0 + kRefineSpan, (uint64_t)n_use);
+    for (uint64_t q0 = g0 + wave * 64; q0 < g_end; q0 += blockDim.x) {  // wave-uniform
+      const uint64_t idx = q0 + lane;
+      const bool act = idx < g_end;
+      PairEntry e{0u, 0u, 0u, 0u};
+      if (act) e = in.e[idx];
+      const uint64_t am = __ballot(act);
+      const uint32_t jr0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)e.jr);
+      bool keep;
+      if (__ballot(act && e.jr == jr0) == am) {  // one JR: scalar descriptor loads
+        typedef const __attribute__((address_space(4))) JRDesc ConstJRDesc;
+        typedef const __attribute__((address_space(4))) SimDesc ConstSimDesc;
+        ConstJRDesc &d = *(ConstJRDesc *)(b.jrs + jr0);
+        const uint32_t job = d.job;
+        if (job != cur_job) {
+          count_flush();
+          cur_job = job;
+        }
+        cur_cnt += (unsigned long long)__popcll(am);
+        const bool test = d.n_sims != 0 && d.refine;
+        ConstSimDesc &sd = *(ConstSimDesc *)(b.sims + d.sim_off);
+        keep = act && (!test || hist_pass(refine_desc(sd), e.l, e.r));
+      } else {
+        uint32_t job = 0, sim_off = 0, refine = 0;
+        int32_t n_sims = 0;
+        if (act) {
+          const JRDesc &d = b.jrs[e.jr];
+          job = d.job;
+          n_sims = d.n_sims;
+          sim_off = d.sim_off;
+          refine = d.refine;
+        }
+        job_count(b, kJobFiltered, act, job);
+        keep = act &&
+               (n_sims == 0 || !refine || hist_pass(refine_desc(b.sims[sim_off]), e.l, e.r));
+      }
+      wave_append(keep, e, wbuf, wn, out, b.overflow);
     }
-    job_count(b, kJobFiltered, act, job);
-    const bool keep = act && (n_sims == 0 || !refine || hist_pass(b.sims[sim_off], e.l, e.r));
-    list_append(keep, e, out, b.overflow);
   }
+  wave_flush(wbuf, wn, out, b.overflow);
+  count_flush();
 }
 
 // A pass of the exact stage evaluates two consecutive similarity

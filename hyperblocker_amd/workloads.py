@@ -341,7 +341,7 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
         t0 = time.perf_counter()
         pending = collections.deque()
         out = {"matches": 0, "device_ms": 0.0, "upload_s": 0.0, "submit_s": 0.0,
-               "drain_s": 0.0, "candidates": 0, "jw_exact": 0, "audit": 0}
+               "drain_s": 0.0, "candidates": 0, "filtered": 0, "jw_exact": 0, "audit": 0}
 
         def produce(lr, rr):
             tu = time.perf_counter()
@@ -359,6 +359,7 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
             out["audit"] += len(audit)
             out["device_ms"] += st["device_ms"]
             out["candidates"] += st["candidates"]
+            out["filtered"] += st["filtered"]
             out["jw_exact"] += st["jw_exact"]
             out["drain_s"] += time.perf_counter() - td
 
@@ -400,7 +401,8 @@ def run_c3(ctx, inflight=3, reps=2, log=print):
                 "device_pairs_per_s": pairs_total / (best["device_ms"] * 1e-3),
                 "host_upload_s": best["upload_s"], "host_submit_s": best["submit_s"],
                 "host_drain_s": best["drain_s"], "matches": best["matches"],
-                "candidates": best["candidates"], "jw_exact": best["jw_exact"],
+                "candidates": best["candidates"], "filtered": best["filtered"],
+                "jw_exact": best["jw_exact"],
                 "audit": best["audit"],
             }
             if handoff == "columns":
