@@ -49,11 +49,18 @@ static_assert(sizeof(SigRow) == 48, "SigRow layout");
 
 constexpr uint32_t kLongFlag = 0x80000000u;
 
-// 128-class byte counters of a similarity column: class = byte & 127, one u8
-// counter per class, counter k in byte (k & 3) of word k >> 2.  Counters of
-// cells longer than 255 bytes may wrap; such cells carry kLongFlag in their
+// 128-class byte counters of a similarity column (class = byte & 127),
+// saturated at 15 and kept as nibbles: word k holds classes 8k .. 8k + 7,
+// class 8k + e (e < 4) in the low nibble of byte e and class 8k + 4 + e in
+// its high nibble, so that (w & 0x0f0f0f0f) and ((w >> 4) & 0x0f0f0f0f) are
+// u8 counters for v_sad_u8.  Saturation only lowers |a - b| per class, so
+// the bound built on these counters stays an upper bound.  Counters of cells
+// longer than 255 bytes may be wrong; such cells carry kLongFlag in their
 // signature and no bound rejects them.
-constexpr int kHistWords = 32;
+constexpr int kHistWords = 16;
+// The u8 counters the builders accumulate per row (class k in byte k & 3 of
+// word k >> 2).
+constexpr int kClassWords = 32;
 
 // Bit-planes of a similarity column (positions 0..32*nwc-1, nwc <= 8):
 // uint32 planes[row][w][k], bit b of word (w, k) is bit k of byte 32w+b.

@@ -114,6 +114,25 @@ __device__ __forceinline__ void job_count(const BatchDev &b, int k, bool ok, uin
   }
 }
 
+// Adds v of every lane with ok set to counter k of the lane's job: one
+// atomic per distinct job in the wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void job_add(const BatchDev &b, int k, bool ok, uint32_t job,
+                                        unsigned long long v) {
+  uint64_t mask = __ballot(ok);
+  while (mask) {
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t lj = (uint32_t)__builtin_amdgcn_readlane((int)job, leader);
+    const bool same = ok && job == lj;
+    unsigned long long sum = same ? v : 0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d);
+    if ((int)lane_id() == leader)
+      atomicAdd(b.job_cnt + (size_t)job_row(k, blockIdx.x & (kStatStripes - 1)) * b.cnt_stride + lj,
+                sum);
+    mask &= ~__ballot(same);
+  }
+}
+
 // A slot of the lane's job among fill[job]: consecutive per job, one atomic
 // per distinct job in the wave.
 __device__ __forceinline__ unsigned long long job_slot(bool ok, uint32_t job,
@@ -337,17 +356,27 @@ __global__ void k_build_sig(const uint8_t *__restrict__ bytes, uint32_t width, i
   sig_out[r] = s;
 }
 
-// 128-class counters of each cell (layout: kHistWords above).  Each thread
+// Two words of u8 class counters (classes 8k .. 8k + 3 and 8k + 4 .. 8k + 7)
+// as one word of nibble counters saturated at 15 (the kHistWords layout).
+__device__ __forceinline__ uint32_t sat_nibbles(uint32_t lo4, uint32_t hi4) {
+  auto sat = [](uint32_t x) {
+    const uint32_t big = (((x >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu) & 0x10101010u;  // byte >= 16
+    return (x | ((big >> 4) * 0x0fu)) & 0x0f0f0f0fu;
+  };
+  return sat(lo4) | (sat(hi4) << 4);
+}
+
+// 128-class counters of each cell (layout: kHistWords).  Each thread
 // counts one row in its own column of an LDS array, so no synchronisation
 // is needed.
 __global__ __launch_bounds__(256) void k_build_hist(const uint8_t *__restrict__ bytes,
                                                    uint32_t width, int64_t n,
                                                    uint32_t *__restrict__ hist) {
-  __shared__ uint32_t s_h[kHistWords][256];
+  __shared__ uint32_t s_h[kClassWords][256];
   const uint32_t t = threadIdx.x;
   const int64_t r = (int64_t)blockIdx.x * 256 + t;
 #pragma unroll
-  for (int w = 0; w < kHistWords; w++) s_h[w][t] = 0;
+  for (int w = 0; w < kClassWords; w++) s_h[w][t] = 0;
   if (r >= n) return;
   const uint4 *cell = reinterpret_cast<const uint4 *>(bytes + r * (int64_t)width);
   const uint32_t n16 = width >> 4;
@@ -368,7 +397,10 @@ __global__ __launch_bounds__(256) void k_build_hist(const uint8_t *__restrict__ 
   uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords);
 #pragma unroll
   for (int w = 0; w < kHistWords / 4; w++)
-    dst[w] = make_uint4(s_h[4 * w][t], s_h[4 * w + 1][t], s_h[4 * w + 2][t], s_h[4 * w + 3][t]);
+    dst[w] = make_uint4(sat_nibbles(s_h[8 * w][t], s_h[8 * w + 1][t]),
+                        sat_nibbles(s_h[8 * w + 2][t], s_h[8 * w + 3][t]),
+                        sat_nibbles(s_h[8 * w + 4][t], s_h[8 * w + 5][t]),
+                        sat_nibbles(s_h[8 * w + 6][t], s_h[8 * w + 7][t]));
 }
 
 __global__ void k_build_planes(const uint8_t *__restrict__ bytes, uint32_t width, int64_t n,
@@ -425,11 +457,11 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
                                                   SigRow *__restrict__ sig_out,
                                                   uint32_t *__restrict__ hist,
                                                   uint32_t *__restrict__ planes) {
-  __shared__ uint32_t s_h[kHistWords][64];  // [class word][row of the work-group]
+  __shared__ uint32_t s_h[kClassWords][64];  // [class word][row of the work-group]
   const uint32_t t = threadIdx.x, q = t & 3u, rl = t >> 2;
   const int64_t r = (int64_t)blockIdx.x * 64 + rl;
   const bool valid = r < n;
-  for (uint32_t i = t; i < kHistWords * 64; i += 256) (&s_h[0][0])[i] = 0;
+  for (uint32_t i = t; i < kClassWords * 64; i += 256) (&s_h[0][0])[i] = 0;
   __syncthreads();
   uint32_t L = 0, pre = 0;
   if (valid) {
@@ -477,13 +509,13 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
   }
   __syncthreads();
   if (!valid) return;
-  {  // lane q writes class words 8q .. 8q + 7 of the row
+  {  // lane q writes classes 32q .. 32q + 31 of the row: nibble words 4q .. 4q + 3
     uint32_t o[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) o[j] = s_h[8 * q + j][rl];
-    uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords + 8 * q);
-    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords + 4 * q);
+    dst[0] = make_uint4(sat_nibbles(o[0], o[1]), sat_nibbles(o[2], o[3]), sat_nibbles(o[4], o[5]),
+                        sat_nibbles(o[6], o[7]));
   }
   if (q == 0) {
     len_out[r] = (uint16_t)min(L, 65535u);
@@ -1056,9 +1088,8 @@ __device__ __forceinline__ uint32_t find_le_u32(const uint32_t *off, uint32_t n,
 
 // One thread per group of the batch: left run `local` of JR jr meets the
 // right run of the same code (binary search over the right runs).
-__global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
-  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi > g.n) return;
+__device__ __forceinline__ void join_group(const BatchDev &b, const GroupArrays &g, uint32_t gi,
+                                           uint32_t &job, unsigned long long &jp) {
   if (gi == g.n) {
     g.l_off[gi] = g.l_cnt[gi] = g.r_off[gi] = g.r_cnt[gi] = g.jr[gi] = 0;
     g.small_pairs[gi] = 0;
@@ -1096,11 +1127,18 @@ __global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
   g.r_cnt[gi] = r_cnt;
   g.jr[gi] = jr;
   set_group_work(g, gi, l_cnt, r_cnt);
-  const unsigned long long jp = (unsigned long long)l_cnt * r_cnt;
-  if (jp)
-    atomicAdd(b.job_cnt + (size_t)job_row(kJobJoined, gi & (kStatStripes - 1)) * b.cnt_stride +
-                  d.job,
-              jp);
+  job = d.job;
+  jp = (unsigned long long)l_cnt * r_cnt;
+}
+
+// One thread per group (and one more for the end entry).  The candidate
+// pairs are summed per job over the wave, one atomic per job and wave.
+__global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
+  const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t job = 0;
+  unsigned long long jp = 0;
+  if (gi <= g.n) join_group(b, g, gi, job, jp);
+  job_add(b, kJobJoined, jp != 0, job, jp);
 }
 
 // Largest g in [0, n) with off[g] <= x (off non-decreasing, off[0] = 0).
@@ -1933,13 +1971,17 @@ __device__ __forceinline__ bool hist_pass(const RefineDesc &sd, uint32_t l, uint
   const uint4 *ha = reinterpret_cast<const uint4 *>(sd.hist_l + (uint64_t)l * kHistWords);
   const uint4 *hb = reinterpret_cast<const uint4 *>(sd.hist_r + (uint64_t)r * kHistWords);
   uint32_t sad = 0;
+  auto sad_nib = [](uint32_t a, uint32_t b, uint32_t acc) {
+    acc = __builtin_amdgcn_sad_u8(a & 0x0f0f0f0fu, b & 0x0f0f0f0fu, acc);
+    return __builtin_amdgcn_sad_u8((a >> 4) & 0x0f0f0f0fu, (b >> 4) & 0x0f0f0f0fu, acc);
+  };
 #pragma unroll
   for (int k = 0; k < kHistWords / 4; k++) {
     const uint4 u = ha[k], v = hb[k];
-    sad = __builtin_amdgcn_sad_u8(u.x, v.x, sad);
-    sad = __builtin_amdgcn_sad_u8(u.y, v.y, sad);
-    sad = __builtin_amdgcn_sad_u8(u.z, v.z, sad);
-    sad = __builtin_amdgcn_sad_u8(u.w, v.w, sad);
+    sad = sad_nib(u.x, v.x, sad);
+    sad = sad_nib(u.y, v.y, sad);
+    sad = sad_nib(u.z, v.z, sad);
+    sad = sad_nib(u.w, v.w, sad);
   }
   const uint32_t la = sa.y & 0xffffu, lb = sb.y & 0xffffu;
   const uint32_t x = sa.x ^ sb.x;
