@@ -864,46 +864,34 @@ __global__ void k_lsmall_count(LayoutSmall a) {
   }
 }
 
-// The same with the counts of a work-group's 4096 rows taken in LDS first
-// (keys of up to kLayoutLdsCodes codes): one global atomic per bucket the
-// work-group uses.
-constexpr uint32_t kLayoutRowsPerGroup = 4096;
+// The same with the counts of a work-group's 1024 rows (one per thread)
+// taken in LDS first (keys of up to kLayoutLdsCodes codes): one global
+// atomic per bucket the work-group uses.
+constexpr uint32_t kLayoutRowsPerGroup = 1024;
 __global__ __launch_bounds__(1024) void k_lsmall_count_lds(LayoutSmall a) {
   extern __shared__ uint32_t s_cnt[];  // buckets of both orders
   const uint32_t t = threadIdx.x;
   const uint32_t nb = a.n_codes * 256u, nb2 = a.sig ? 2u * nb : nb;
   for (uint32_t k = t; k < nb2; k += 1024) s_cnt[k] = 0;
   __syncthreads();
-  const uint32_t i0 = blockIdx.x * kLayoutRowsPerGroup + t;
-  uint32_t b1[4], b2[4], r1[4], r2[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const uint32_t i = i0 + 1024u * u;
-    b1[u] = b2[u] = 0xffffffffu;
-    if (i < a.n) {
-      const uint32_t base = lsmall_code(a, i) << 8;
-      const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
-      b1[u] = base + l;
-      if (a.sig) b2[u] = nb + base + (a.sig[i].pre & 0xffu);
-    }
+  const uint32_t i = blockIdx.x * kLayoutRowsPerGroup + t;
+  uint32_t b1 = 0xffffffffu, b2 = 0xffffffffu, r1 = 0, r2 = 0;
+  if (i < a.n) {
+    const uint32_t base = lsmall_code(a, i) << 8;
+    const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+    b1 = base + l;
+    if (a.sig) b2 = nb + base + (a.sig[i].pre & 0xffu);
   }
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    if (b1[u] != 0xffffffffu) r1[u] = atomicAdd(s_cnt + b1[u], 1u);
-    if (b2[u] != 0xffffffffu) r2[u] = atomicAdd(s_cnt + b2[u], 1u);
-  }
+  if (b1 != 0xffffffffu) r1 = atomicAdd(s_cnt + b1, 1u);
+  if (b2 != 0xffffffffu) r2 = atomicAdd(s_cnt + b2, 1u);
   __syncthreads();
   for (uint32_t k = t; k < nb2; k += 1024) {  // the work-group's base in each bucket
     const uint32_t c = s_cnt[k];
     s_cnt[k] = c ? atomicAdd(a.hist + k, c) : 0u;
   }
   __syncthreads();
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const uint32_t i = i0 + 1024u * u;
-    if (b1[u] != 0xffffffffu) a.rank[i] = s_cnt[b1[u]] + r1[u];
-    if (b2[u] != 0xffffffffu) a.rank[a.n + i] = s_cnt[b2[u]] + r2[u];
-  }
+  if (b1 != 0xffffffffu) a.rank[i] = s_cnt[b1] + r1;
+  if (b2 != 0xffffffffu) a.rank[a.n + i] = s_cnt[b2] + r2;
 }
 
 // One work-group of 1024 threads: exclusive scans of the two count arrays
@@ -2438,9 +2426,10 @@ void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows
 void launch_layout_small(const LayoutSmall &a, hipStream_t stream) {
   if (!a.n) return;
   const uint32_t blocks = (a.n + 255) / 256;
+  const size_t lds = (a.sig ? 2 : 1) * a.n_codes * 256 * sizeof(uint32_t);
   if (a.n_codes <= kLayoutLdsCodes)
-    k_lsmall_count_lds<<<(a.n + kLayoutRowsPerGroup - 1) / kLayoutRowsPerGroup, 1024,
-                         (a.sig ? 2 : 1) * a.n_codes * 256 * sizeof(uint32_t), stream>>>(a);
+    k_lsmall_count_lds<<<(a.n + kLayoutRowsPerGroup - 1) / kLayoutRowsPerGroup, 1024, lds,
+                         stream>>>(a);
   else
     k_lsmall_count<<<blocks, 256, 0, stream>>>(a);
   k_lsmall_scan<<<1, 1024, 0, stream>>>(a);
