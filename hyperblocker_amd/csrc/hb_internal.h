@@ -49,15 +49,21 @@ static_assert(sizeof(SigRow) == 48, "SigRow layout");
 
 constexpr uint32_t kLongFlag = 0x80000000u;
 
-// 128-class byte counters of a similarity column (class = byte & 127),
-// saturated at 15 and kept as nibbles: word k holds classes 8k .. 8k + 7,
-// class 8k + e (e < 4) in the low nibble of byte e and class 8k + 4 + e in
-// its high nibble, so that (w & 0x0f0f0f0f) and ((w >> 4) & 0x0f0f0f0f) are
-// u8 counters for v_sad_u8.  Saturation only lowers |a - b| per class, so
-// the bound built on these counters stays an upper bound.  Counters of cells
-// longer than 255 bytes may be wrong; such cells carry kLongFlag in their
-// signature and no bound rejects them.
+// The refine pass's row of a similarity column, 64 bytes: word 0 the
+// cell's first 4 bytes and word 1 its length (with kLongFlag), as in its
+// SigRow; words 2-3 zero; words 4-15 byte-class counters.  The class of a
+// byte is byte & 127, except that classes 0-31 are counted in class + 32
+// (any partition of the bytes into classes gives a sound bound, and this
+// one frees the first 16 bytes).  Counters are saturated at 15 and kept as
+// nibbles: word k holds classes 8k .. 8k + 7, class 8k + e (e < 4) in the
+// low nibble of byte e and class 8k + 4 + e in its high nibble, so that
+// (w & 0x0f0f0f0f) and ((w >> 4) & 0x0f0f0f0f) are u8 counters for
+// v_sad_u8.  Saturation only lowers |a - b| per class, so the bound built
+// on these counters stays an upper bound.  Counters of cells longer than
+// 255 bytes may be wrong; such cells carry kLongFlag and no bound rejects
+// them.
 constexpr int kHistWords = 16;
+constexpr int kHistFirstClassWord = 4;
 // The u8 counters the builders accumulate per row (class k in byte k & 3 of
 // word k >> 2).
 constexpr int kClassWords = 32;

@@ -356,6 +356,14 @@ __global__ void k_build_sig(const uint8_t *__restrict__ bytes, uint32_t width, i
   sig_out[r] = s;
 }
 
+// The counter class of a non-zero byte (kHistWords layout): byte & 127,
+// with classes 0-31 counted in class + 32.  Its low 5 bits are the byte's,
+// so the signatures' 32 classes are sums of these.
+__device__ __forceinline__ uint32_t byte_class(uint32_t c) {
+  const uint32_t k = c & 127u;
+  return k < 32u ? k + 32u : k;
+}
+
 // Two words of u8 class counters (classes 8k .. 8k + 3 and 8k + 4 .. 8k + 7)
 // as one word of nibble counters saturated at 15 (the kHistWords layout).
 __device__ __forceinline__ uint32_t sat_nibbles(uint32_t lo4, uint32_t hi4) {
@@ -381,8 +389,10 @@ __global__ __launch_bounds__(256) void k_build_hist(const uint8_t *__restrict__ 
   const uint4 *cell = reinterpret_cast<const uint4 *>(bytes + r * (int64_t)width);
   const uint32_t n16 = width >> 4;
   bool end = false;
+  uint32_t L = 0, pre = 0;
   for (uint32_t k = 0; k < n16 && !end; k++) {
     const uint4 v = cell[k];
+    if (k == 0) pre = v.x;
     const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -390,13 +400,15 @@ __global__ __launch_bounds__(256) void k_build_hist(const uint8_t *__restrict__ 
       for (int b = 0; b < 4; b++) {
         const uint32_t c = (w4[q] >> (8 * b)) & 0xffu;
         end |= (c == 0);
-        if (!end) s_h[(c & 127u) >> 2][t] += 1u << ((c & 3u) * 8u);
+        L += !end;
+        if (!end) s_h[byte_class(c) >> 2][t] += 1u << ((c & 3u) * 8u);
       }
     }
   }
   uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords);
+  dst[0] = make_uint4(pre, L | (L > 255u ? kLongFlag : 0u), 0u, 0u);
 #pragma unroll
-  for (int w = 0; w < kHistWords / 4; w++)
+  for (int w = 1; w < kHistWords / 4; w++)
     dst[w] = make_uint4(sat_nibbles(s_h[8 * w][t], s_h[8 * w + 1][t]),
                         sat_nibbles(s_h[8 * w + 2][t], s_h[8 * w + 3][t]),
                         sat_nibbles(s_h[8 * w + 4][t], s_h[8 * w + 5][t]),
@@ -481,7 +493,7 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
         for (int e = 0; e < 4; e++) {
           const uint32_t c = (w8[d] >> (8 * e)) & 0xffu;
           const uint32_t inc = c ? 1u << ((c & 3u) * 8u) : 0u;
-          atomicAdd(&s_h[(c & 127u) >> 2][rl], inc);
+          atomicAdd(&s_h[byte_class(c) >> 2][rl], inc);
         }
       }
       if (blk < nwc) {
@@ -509,13 +521,15 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
   }
   __syncthreads();
   if (!valid) return;
-  {  // lane q writes classes 32q .. 32q + 31 of the row: nibble words 4q .. 4q + 3
+  {  // lane q writes nibble words 4q .. 4q + 3 of the row (classes 32q .. 32q + 31;
+     // lane 0's classes are never counted, so it writes the prefix and length)
     uint32_t o[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) o[j] = s_h[8 * q + j][rl];
     uint4 *dst = reinterpret_cast<uint4 *>(hist + r * kHistWords + 4 * q);
-    dst[0] = make_uint4(sat_nibbles(o[0], o[1]), sat_nibbles(o[2], o[3]), sat_nibbles(o[4], o[5]),
-                        sat_nibbles(o[6], o[7]));
+    dst[0] = q == 0 ? make_uint4(pre, L | (L > 255u ? kLongFlag : 0u), 0u, 0u)
+                    : make_uint4(sat_nibbles(o[0], o[1]), sat_nibbles(o[2], o[3]),
+                                 sat_nibbles(o[4], o[5]), sat_nibbles(o[6], o[7]));
   }
   if (q == 0) {
     len_out[r] = (uint16_t)min(L, 65535u);
@@ -1936,44 +1950,43 @@ __device__ __forceinline__ int lane_matches(const uint32_t *row_planes, int nwc,
 }
 
 // The refine test of one pair (l, r) against similarity predicate sd: the
-// bound of may_pass with 128-class counters, which is false only when JW is
-// certainly below thr - 2e-6 (a matched pair of bytes is a pair of equal
-// bytes, so m <= sum over classes of min(count_a, count_b) for any class
-// partition).  Both rows' counters are loaded before the early outs (long
-// or empty cells, equal first 4 bytes), so the loads go out together.
+// bound of may_pass with the 96-class counters of the refine rows, which is
+// false only when JW is certainly below thr - 2e-6 (a matched pair of bytes
+// is a pair of equal bytes, so m <= sum over classes of min(count_a,
+// count_b) for any class partition).  A refine row holds the cell's prefix
+// and length too, so a pair reads one 64-byte row per side, and both rows
+// are loaded before the early outs (long or empty cells, equal first 4
+// bytes).
 struct RefineDesc {
-  const SigRow *sig_l, *sig_r;
   const uint32_t *hist_l, *hist_r;
   float c0, c1, c2, c3;
 };
 
 template <class SD>
 __device__ __forceinline__ RefineDesc refine_desc(SD &sd) {
-  return RefineDesc{sd.col_l.sig, sd.col_r.sig, sd.col_l.hist, sd.col_r.hist,
-                    sd.c0,        sd.c1,        sd.c2,         sd.c3};
+  return RefineDesc{sd.col_l.hist, sd.col_r.hist, sd.c0, sd.c1, sd.c2, sd.c3};
 }
 
 __device__ __forceinline__ bool hist_pass(const RefineDesc &sd, uint32_t l, uint32_t r) {
-  const uint2 sa = *reinterpret_cast<const uint2 *>(&sd.sig_l[l].pre);  // (pre, len)
-  const uint2 sb = *reinterpret_cast<const uint2 *>(&sd.sig_r[r].pre);
   const uint4 *ha = reinterpret_cast<const uint4 *>(sd.hist_l + (uint64_t)l * kHistWords);
   const uint4 *hb = reinterpret_cast<const uint4 *>(sd.hist_r + (uint64_t)r * kHistWords);
+  const uint4 a0 = ha[0], b0 = hb[0];  // (prefix, length, 0, 0)
   uint32_t sad = 0;
   auto sad_nib = [](uint32_t a, uint32_t b, uint32_t acc) {
     acc = __builtin_amdgcn_sad_u8(a & 0x0f0f0f0fu, b & 0x0f0f0f0fu, acc);
     return __builtin_amdgcn_sad_u8((a >> 4) & 0x0f0f0f0fu, (b >> 4) & 0x0f0f0f0fu, acc);
   };
 #pragma unroll
-  for (int k = 0; k < kHistWords / 4; k++) {
+  for (int k = kHistFirstClassWord / 4; k < kHistWords / 4; k++) {
     const uint4 u = ha[k], v = hb[k];
     sad = sad_nib(u.x, v.x, sad);
     sad = sad_nib(u.y, v.y, sad);
     sad = sad_nib(u.z, v.z, sad);
     sad = sad_nib(u.w, v.w, sad);
   }
-  const uint32_t la = sa.y & 0xffffu, lb = sb.y & 0xffffu;
-  const uint32_t x = sa.x ^ sb.x;
-  if (((sa.y | sb.y) & kLongFlag) || la == 0 || lb == 0 || x == 0) return true;
+  const uint32_t la = a0.y & 0xffffu, lb = b0.y & 0xffffu;
+  const uint32_t x = a0.x ^ b0.x;
+  if (((a0.y | b0.y) & kLongFlag) || la == 0 || lb == 0 || x == 0) return true;
   const uint32_t mub = (la + lb - sad) >> 1;
   const uint32_t S = min(la, lb), T = max(la, lb);
   const uint32_t p = ((uint32_t)(__ffs(x) - 1)) >> 3;
