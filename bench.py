@@ -10,7 +10,8 @@ tables unpacked from those rows (hb_table_from_device), every per-table
 structure built (dictionary codes, signatures, join layouts), every kernel
 of the job, and the matched pairs handed back to the caller (copied into a
 numpy array).  Steps are issued with up to --inflight jobs outstanding, as
-the partition pipeline issues partitions.
+the partition pipeline issues partitions: one thread submits while another
+collects results in order (the reference's producer and reducer threads).
 
 Also reported: the same steps with the rows handed over from host memory
 (PCIe-inclusive), the repeated-job rate of one pair of warm resident tables,
@@ -22,7 +23,9 @@ import argparse
 import collections
 import json
 import os
+import queue
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -252,16 +255,42 @@ def main():
         return ctx.submit(dp, tl, tr), (tl, tr)
 
     def run_steps(make, steps, inflight):
-        stats, pending, last = [], collections.deque(), None
+        """`steps` jobs, at most `inflight` of them submitted and not yet
+        finished.  A producer thread submits (tables and job) while this
+        thread collects results in submission order, as the reference's
+        producer and reducer threads do (host_producer.h, host_reducer.h);
+        the library's calls release the interpreter lock."""
+        stats, last = [], None
+        ready = queue.Queue()
+        slots = threading.Semaphore(max(1, inflight))
+        failure, stop = [], threading.Event()
+
+        def produce():
+            try:
+                for _ in range(steps):
+                    slots.acquire()
+                    if stop.is_set():
+                        return
+                    ready.put(make())
+            except BaseException as e:  # handed to the collecting thread
+                failure.append(e)
+                ready.put(None)
+
         t0 = time.perf_counter()
-        for _ in range(steps):
-            pending.append(make())
-            if len(pending) >= max(1, inflight):
-                last, st = finish(pending.popleft())
+        producer = threading.Thread(target=produce)
+        producer.start()
+        try:
+            for _ in range(steps):
+                item = ready.get()
+                if item is None:
+                    raise failure[0]
+                last, st = finish(item)
                 stats.append(st)
-        while pending:
-            last, st = finish(pending.popleft())
-            stats.append(st)
+                slots.release()
+        finally:
+            stop.set()
+            slots.release()
+            producer.join()
         ctx.synchronize()
         return time.perf_counter() - t0, stats, last
 

@@ -2185,9 +2185,13 @@ __global__ __launch_bounds__(kExactThreads) void k_exact_two(ExactArgs a) {
       }
     }
     __syncthreads();
-    {  // the chains: half h = tid / 128 evaluates predicate st + h
+    {  // the chains: half h = tid / 256 evaluates predicate st + h.  Wave w
+       // of the first half takes the w-th shortest quarter of its pairs and
+       // wave 4 + w the w-th longest quarter of the second half's (both run
+       // on SIMD w), so that every SIMD gets a similar share of steps.
       const int h = (int)(tid / kPass);
-      const uint32_t i = s_order[h][tid % kPass];
+      const uint32_t k = tid % kPass;
+      const uint32_t i = s_order[h][h ? kPass - 1 - k : k];
       const ExactSlot &sl = s_slot[i];
       const bool act = (sl.act >> h) & 1u;
       SimDesc sd{};
