@@ -246,6 +246,7 @@ struct DictDev {
   uint32_t max_probe;
   uint32_t hash_bits;              // 64; fewer only to force collisions in tests
   uint32_t pad;
+  uint32_t *done;                  // work-groups of the running k_dict_verify done (0 at rest)
 };
 constexpr unsigned long long kDictEmpty = ~0ull;
 constexpr unsigned long long kDictNoRep = ~0ull;
@@ -278,10 +279,13 @@ hipError_t noalloc_scratch(void **ptr);
 // Dictionary codes of n cells (each `width` bytes, 16-byte aligned, NUL
 // padded: a cell ends at its first NUL, or has fixed_len bytes when
 // fixed_len > 0).  Rows left to the host get kNoCode in `code` and are listed
-// in fb_rows[0 .. *fb_count) (fb_rows holds n entries; *fb_count must be 0).
+// in fb_rows[0 .. *fb_count) (fb_rows holds n entries; the kernels zero
+// *fb_count first).  When done, the last work-group writes (that count, the
+// codes handed out) to status[0..1] (pinned host memory, device address).
 void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, uint32_t n,
                       DictDev d, uint32_t *code, uint32_t *fb_rows,
-                      unsigned long long *fb_count, hipStream_t stream);
+                      unsigned long long *fb_count, unsigned long long *status,
+                      hipStream_t stream);
 // Key cells of a composite key: byte 0xC0 + k, then the k column codes
 // (little endian), zero padded to width (>= 1 + 4k, multiple of 16).
 void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t width,
@@ -308,7 +312,8 @@ void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, u
 void launch_build_hist(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t *hist,
                        hipStream_t stream);
 // The three above in one pass over the cells (what the runtime uses; the
-// separate kernels stay for the tests that compare them).
+// separate kernels stay for the tests that compare them); hist may be null
+// (no refine rows).
 void launch_build_col(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,
                       uint16_t *len, SigRow *sig, uint32_t *hist, uint32_t *planes,
                       hipStream_t stream);
@@ -348,15 +353,18 @@ struct LayoutSmall {
   uint32_t *perm, *perm_fb;
   SigRow *jsig, *jsig_fb;
   uint32_t *run_code, *run_off, *n_runs;
+  uint32_t *n_runs_host;  // the run count also written here (pinned, device address)
 };
 void launch_layout_small(const LayoutSmall &a, hipStream_t stream);
 void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
                       hipStream_t stream);
 // run_code / run_off of every run start (flags, run_id = exclusive scan of
-// flags) and run_off[n_runs] = n, with n_runs = run_id[n-1] + flags[n-1].
+// flags) and run_off[n_runs] = n, with n_runs = run_id[n-1] + flags[n-1]
+// (also written to n_runs_host, pinned host memory at its device address).
 void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
                         const uint32_t *run_id, uint32_t n, uint32_t *run_code,
-                        uint32_t *run_off, uint32_t *n_runs, hipStream_t stream);
+                        uint32_t *run_off, uint32_t *n_runs, uint32_t *n_runs_host,
+                        hipStream_t stream);
 // Groups of every JR of a batch (one per left run), with their work.
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream);
 

@@ -458,11 +458,12 @@ __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
 // Every per-row structure of a similarity column from one read of its
 // cells, four lanes per row: lane q takes the row's 32-byte blocks q, q + 4,
 // ...  Bit-planes of a block: four 8 x 8 bit transposes.  Byte classes: one
-// LDS atomic add per byte into the row's 128 counters (byte & 127, u8 each;
+// LDS atomic add per byte into the row's 128 counters (byte_class, u8 each;
 // zero bytes, which follow the cell's NUL, add nothing), from which the
 // signature's 32 counters (byte & 31) are sums of four.  The length counts
 // the non-zero bytes (every byte after the NUL is zero).  Results equal
-// those of k_build_sig, k_build_hist and k_build_planes.
+// those of k_build_sig, k_build_hist and k_build_planes; the refine rows
+// are written only when `hist` is not null.
 __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ bytes,
                                                   uint32_t width, int64_t n, uint32_t nwc,
                                                   uint16_t *__restrict__ len_out,
@@ -521,8 +522,8 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
   }
   __syncthreads();
   if (!valid) return;
-  {  // lane q writes nibble words 4q .. 4q + 3 of the row (classes 32q .. 32q + 31;
-     // lane 0's classes are never counted, so it writes the prefix and length)
+  if (hist) {  // lane q writes nibble words 4q .. 4q + 3 of the refine row (classes
+               // 32q .. 32q + 31; lane 0's are never counted, so it writes the prefix and length)
     uint32_t o[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) o[j] = s_h[8 * q + j][rl];
@@ -618,8 +619,9 @@ __device__ __forceinline__ uint64_t dict_hash_cell(const uint4 *cell, uint32_t w
 // holds the slot until k_dict_verify replaces it by the value's code.
 __global__ void k_dict_insert(const uint8_t *__restrict__ cells, uint32_t width,
                               uint32_t fixed_len, uint32_t n, DictDev d,
-                              uint32_t *__restrict__ code) {
+                              uint32_t *__restrict__ code, unsigned long long *fb_count) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) *fb_count = 0ull;  // k_dict_verify counts after this kernel
   if (r >= n) return;
   const uint4 *cell = reinterpret_cast<const uint4 *>(cells + (size_t)r * width);
   uint32_t len, chunks;
@@ -657,12 +659,11 @@ __global__ void k_dict_insert(const uint8_t *__restrict__ cells, uint32_t width,
 
 // One thread per row: the cell must equal its slot's representative; rows
 // that do not (or have no slot) are listed for the host.
-__global__ void k_dict_verify(const uint8_t *__restrict__ cells, uint32_t width,
-                              uint32_t fixed_len, uint32_t n, DictDev d,
-                              uint32_t *__restrict__ code, uint32_t *__restrict__ fb_rows,
-                              unsigned long long *fb_count) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+__device__ __forceinline__ void dict_verify_row(const uint8_t *__restrict__ cells, uint32_t width,
+                                                uint32_t fixed_len, uint32_t r, const DictDev &d,
+                                                uint32_t *__restrict__ code,
+                                                uint32_t *__restrict__ fb_rows,
+                                                unsigned long long *fb_count) {
   const uint32_t c = code[r];
   bool ok = c != kNoCode;
   if (ok) {
@@ -692,6 +693,26 @@ __global__ void k_dict_verify(const uint8_t *__restrict__ cells, uint32_t width,
   } else {
     code[r] = kNoCode;
     fb_rows[atomicAdd(fb_count, 1ull)] = r;
+  }
+}
+
+__global__ void k_dict_verify(const uint8_t *__restrict__ cells, uint32_t width,
+                              uint32_t fixed_len, uint32_t n, DictDev d,
+                              uint32_t *__restrict__ code, uint32_t *__restrict__ fb_rows,
+                              unsigned long long *fb_count, unsigned long long *status) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dict_verify_row(cells, width, fixed_len, r, d, code, fb_rows, fb_count);
+  // The last work-group to finish reports (rows left to the host, codes
+  // handed out): every group's counts are in once it has seen them all done.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(d.done, 1u) == gridDim.x - 1) {
+      __threadfence();
+      status[0] = __hip_atomic_load(fb_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      status[1] = __hip_atomic_load(d.n_codes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *d.done = 0u;  // for the next coding (codings run in order on one stream)
+    }
   }
 }
 
@@ -980,6 +1001,7 @@ __global__ __launch_bounds__(1024) void k_lsmall_scan(LayoutSmall a) {
       const uint32_t nr = (uint32_t)__popcll(m);
       a.run_off[nr] = a.n;
       *a.n_runs = nr;
+      *a.n_runs_host = nr;
     }
   }
 }
@@ -1064,7 +1086,7 @@ __device__ __forceinline__ void set_group_work(const GroupArrays &g, uint32_t gi
 __global__ void k_layout_runs(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ flags,
                               const uint32_t *__restrict__ run_id, uint32_t n,
                               uint32_t *__restrict__ run_code, uint32_t *__restrict__ run_off,
-                              uint32_t *__restrict__ n_runs) {
+                              uint32_t *__restrict__ n_runs, uint32_t *n_runs_host) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (flags[i]) {
@@ -1075,6 +1097,7 @@ __global__ void k_layout_runs(const uint64_t *__restrict__ keys, const uint32_t 
     const uint32_t nr = run_id[i] + flags[i];
     run_off[nr] = n;
     *n_runs = nr;
+    *n_runs_host = nr;
   }
 }
 
@@ -2405,12 +2428,12 @@ hipError_t noalloc_scratch(void **ptr) {
 
 void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, uint32_t n,
                       DictDev d, uint32_t *code, uint32_t *fb_rows,
-                      unsigned long long *fb_count, hipStream_t stream) {
-  if (!n) return;
-  const uint32_t blocks = (n + 255) / 256;
-  k_dict_insert<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code);
+                      unsigned long long *fb_count, unsigned long long *status,
+                      hipStream_t stream) {
+  const uint32_t blocks = std::max<uint32_t>(1, (n + 255) / 256);
+  k_dict_insert<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code, fb_count);
   k_dict_verify<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code, fb_rows,
-                                            fb_count);
+                                            fb_count, status);
 }
 
 void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t width,
@@ -2487,10 +2510,11 @@ void launch_run_flags(const uint64_t *sorted_keys, uint32_t n, uint32_t *flags,
 
 void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
                         const uint32_t *run_id, uint32_t n, uint32_t *run_code,
-                        uint32_t *run_off, uint32_t *n_runs, hipStream_t stream) {
+                        uint32_t *run_off, uint32_t *n_runs, uint32_t *n_runs_host,
+                        hipStream_t stream) {
   if (!n) return;
   hipLaunchKernelGGL(k_layout_runs, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted_keys,
-                     flags, run_id, n, run_code, run_off, n_runs);
+                     flags, run_id, n, run_code, run_off, n_runs, n_runs_host);
 }
 
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream) {

@@ -662,7 +662,8 @@ struct Column {
   uint16_t *d_len = nullptr;
   hb::SigRow *d_sig = nullptr;
   uint32_t *d_planes = nullptr;  // bit-planes of the first 32*nwc bytes
-  uint32_t *d_hist = nullptr;    // 128-class counters
+  Lazy hist;                     // the refine rows, built when a refining rule first needs them
+  uint32_t *d_hist = nullptr;
   uint32_t nwc = 0;
   // equality predicates
   Lazy code;
@@ -673,7 +674,7 @@ struct Column {
   bool validated = false;
   uint32_t *d_fb_rows = nullptr;
   unsigned long long *d_fb_count = nullptr;
-  unsigned long long *h_status = nullptr;
+  unsigned long long *h_status = nullptr, *d_status = nullptr;  // the same pinned words
   uint64_t bound_at_launch = 0;  // codes handed out when the coding was launched
 };
 
@@ -693,7 +694,7 @@ struct Composite {
   uint32_t width = 0, key_len = 0;
   uint32_t *d_fb_rows = nullptr;
   unsigned long long *d_fb_count = nullptr;
-  unsigned long long *h_status = nullptr;
+  unsigned long long *h_status = nullptr, *d_status = nullptr;
   uint64_t bound_at_launch = 0;
 };
 
@@ -711,11 +712,11 @@ struct Layout {
   hb::SigRow *d_jsig_fb = nullptr;
   uint32_t *d_run_code = nullptr;
   uint32_t *d_run_off = nullptr;  // n_runs + 1
-  // Number of runs: counted on the device by the build (d_nruns) and copied
-  // to h_nruns (pinned); -1 until the host has read it there.  runs_bound
+  // Number of runs: counted on the device by the build (d_nruns) and written
+  // to h_nruns (pinned) too; -1 until the host has read it there.  runs_bound
   // bounds it before then (rows, or the codes handed out).
   std::atomic<int64_t> n_runs{-1};
-  uint32_t *h_nruns = nullptr;
+  uint32_t *h_nruns = nullptr, *d_hnruns = nullptr;  // the same pinned word
   uint32_t *d_nruns = nullptr;
   uint32_t runs_bound = 0;
   uint32_t *d_bad = nullptr;  // set by the build when a code was not below its bound
@@ -1017,41 +1018,60 @@ hb_status record_lazy(Lazy &lz, hipStream_t stream) {
 
 hb_status retire_after(hb_ctx *ctx, hb_ctx::Grave &&g, hipStream_t stream);
 
-hb_status ensure_sig(hb_table *t, int c, hipStream_t stream) {
+// The signatures, bit-planes (and with `refine` the refine rows) of
+// similarity column c, built on first use; a slice borrows its root's.  The
+// refine rows are built only for columns a refining rule reads (first
+// threshold below kRefineBelow): with the signatures in one pass, or alone
+// when the signatures exist already.
+hb_status ensure_sig(hb_table *t, int c, hipStream_t stream, bool refine) {
   Column &col = t->cols[c];
   if (t->parent) {  // borrow the parent's signatures at the slice's offset
-    HB_TRY(ensure_sig(t->parent, c, stream));
+    HB_TRY(ensure_sig(t->parent, c, stream, refine));
     const Column &pc = t->parent->cols[c];
+    const size_t off = (size_t)t->row_off;
     std::lock_guard<std::mutex> lk(t->mu);
     if (!col.sig.built) {
-      const size_t off = (size_t)t->row_off;
       col.d_len = pc.d_len + off;
       col.d_sig = pc.d_sig + off;
       col.nwc = pc.nwc;
       col.d_planes = pc.d_planes + off * pc.nwc * 8;
-      col.d_hist = pc.d_hist + off * hb::kHistWords;
       col.sig.built = true;
+    }
+    if (refine && !col.hist.built) {
+      col.d_hist = pc.d_hist + off * hb::kHistWords;
+      col.hist.built = true;
     }
     return HB_OK;
   }
   std::lock_guard<std::mutex> lk(t->mu);
-  if (col.sig.built) return wait_lazy(col.sig, stream);
+  const bool need_sig = !col.sig.built, need_hist = refine && !col.hist.built;
+  if (!need_sig && !need_hist) {
+    HB_TRY(wait_lazy(col.sig, stream));
+    return refine ? wait_lazy(col.hist, stream) : HB_OK;
+  }
   const size_t n = (size_t)std::max<int64_t>(t->n_rows, 1);
-  HB_TRY(t->dev_alloc(n * sizeof(uint16_t), (void **)&col.d_len));
-  HB_TRY(t->dev_alloc(n * sizeof(hb::SigRow), (void **)&col.d_sig));
-  col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
-  HB_TRY(t->dev_alloc(n * col.nwc * 32, (void **)&col.d_planes));
-  HB_TRY(t->dev_alloc(n * hb::kHistWords * 4, (void **)&col.d_hist));
-  if (t->ctx->separate_col_builds) {
-    hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
-    hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
-    hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
+  if (need_hist) HB_TRY(t->dev_alloc(n * hb::kHistWords * 4, (void **)&col.d_hist));
+  if (need_sig) {
+    HB_TRY(t->dev_alloc(n * sizeof(uint16_t), (void **)&col.d_len));
+    HB_TRY(t->dev_alloc(n * sizeof(hb::SigRow), (void **)&col.d_sig));
+    col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
+    HB_TRY(t->dev_alloc(n * col.nwc * 32, (void **)&col.d_planes));
+    if (t->ctx->separate_col_builds) {
+      hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
+      hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
+      if (need_hist) hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
+    } else {
+      hb::launch_build_col(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_len, col.d_sig,
+                           need_hist ? col.d_hist : nullptr, col.d_planes, stream);
+    }
   } else {
-    hb::launch_build_col(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_len, col.d_sig,
-                         col.d_hist, col.d_planes, stream);
+    HB_TRY(wait_lazy(col.sig, stream));
+    hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
   }
   HIP_OK(hipGetLastError());
-  return record_lazy(col.sig, stream);
+  if (need_sig) HB_TRY(record_lazy(col.sig, stream));
+  if (need_hist) HB_TRY(record_lazy(col.hist, stream));
+  return HB_OK;
 }
 
 // Host time of the phases of a call, printed to stderr when the environment
@@ -1094,6 +1114,7 @@ hb_status dict_create(hb_ctx *ctx, hipStream_t stream) {
   char *counters = p + 2 * slot_bytes + dd.slots * 4;
   dd.dev.arena_used = (unsigned long long *)counters;
   dd.dev.n_codes = (uint32_t *)(counters + 64);
+  dd.dev.done = (uint32_t *)(counters + 128);
   dd.dev.arena = (uint8_t *)(counters + 256);
   dd.dev.arena_cap = dd.arena_bytes;
   dd.dev.mask = (uint32_t)(dd.slots - 1);
@@ -1109,23 +1130,19 @@ hb_status dict_create(hb_ctx *ctx, hipStream_t stream) {
 // Launches the dictionary coding of n cells (device memory, `width` bytes
 // each; a cell ends at its first NUL, or has fixed_len bytes) into d_code on
 // `stream`, with the rows left to the host listed in fb_rows (n entries) and
-// counted in *fb_count, and (that count, the codes handed out) copied to the
-// pinned h_status[0..1] when done.  Nothing waits here.
+// counted in *fb_count; the kernels write (that count, the codes handed out)
+// to the pinned h_status[0..1] (device address d_status) when done.  Nothing
+// waits here.
 hb_status dict_launch(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t fixed_len,
                       size_t n, uint32_t *d_code, uint32_t *fb_rows,
                       unsigned long long *fb_count, unsigned long long *h_status,
-                      hipStream_t stream) {
+                      unsigned long long *d_status, hipStream_t stream) {
   HB_TRY(dict_create(ctx, stream));
   h_status[0] = h_status[1] = 0;
   if (n == 0) return HB_OK;
-  hb::launch_fill_u64(fb_count, 1, 0ull, stream);
   hb::launch_dict_code(cells, width, fixed_len, (uint32_t)n, ctx->ddict.dev, d_code, fb_rows,
-                       fb_count, stream);
+                       fb_count, d_status, stream);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(h_status, fb_count, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                        stream));
-  HIP_OK(hipMemcpyAsync(h_status + 1, ctx->ddict.dev.n_codes, sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, stream));
   return HB_OK;
 }
 
@@ -1173,14 +1190,15 @@ hb_status dict_code(hb_ctx *ctx, const uint8_t *cells, uint32_t width, uint32_t 
   if (n == 0) return HB_OK;
   DevicePool &pool = ctx->pool;
   uint32_t *fb_rows = nullptr;
-  unsigned long long *fb_count = nullptr, *h_status = nullptr;
+  unsigned long long *fb_count = nullptr, *h_status = nullptr, *d_status = nullptr;
   HB_TRY(pool.alloc(n * sizeof(uint32_t), (void **)&fb_rows));
   hb_status st = pool.alloc(sizeof(unsigned long long), (void **)&fb_count);
-  if (st == HB_OK) st = ctx->hpool.alloc(2 * sizeof(unsigned long long), (void **)&h_status, nullptr);
+  if (st == HB_OK)
+    st = ctx->hpool.alloc(2 * sizeof(unsigned long long), (void **)&h_status, (void **)&d_status);
   hipEvent_t ev = nullptr;
   if (st == HB_OK) st = ctx->events.get(&ev);
   if (st == HB_OK) st = dict_launch(ctx, cells, width, fixed_len, n, d_code, fb_rows, fb_count,
-                                    h_status, stream);
+                                    h_status, d_status, stream);
   if (st == HB_OK) {
     hipError_t e = hipEventRecord(ev, stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev);
@@ -1246,7 +1264,8 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream, bool may_defer) {
   HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&col.d_code));
   HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&col.d_fb_rows));
   HB_TRY(t->dev_alloc(sizeof(unsigned long long), (void **)&col.d_fb_count));
-  HB_TRY(t->host_alloc(2 * sizeof(unsigned long long), (void **)&col.h_status));
+  HB_TRY(t->host_alloc(2 * sizeof(unsigned long long), (void **)&col.h_status,
+                       (void **)&col.d_status));
   bool defer = false;
   if (may_defer) {
     std::lock_guard<std::mutex> sl(ctx->ddict.stable_mu);
@@ -1257,7 +1276,7 @@ hb_status ensure_code(hb_table *t, int c, hipStream_t stream, bool may_defer) {
   HB_TRY(wait_ready(t, cs));
   col.bound_at_launch = ctx->ddict.code_bound.load();
   HB_TRY(dict_launch(ctx, col.d_bytes, col.width, 0, n, col.d_code, col.d_fb_rows,
-                     col.d_fb_count, col.h_status, cs));
+                     col.d_fb_count, col.h_status, col.d_status, cs));
   HB_TRY(record_lazy(col.code, cs));
   if (!defer) HB_TRY(validate_code_locked(t, c));
   return wait_lazy(col.code, stream);
@@ -1355,7 +1374,8 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
   HB_TRY(t->dev_alloc(n1 * cp.width, (void **)&cp.d_cells));
   HB_TRY(t->dev_alloc(n1 * sizeof(uint32_t), (void **)&cp.d_fb_rows));
   HB_TRY(t->dev_alloc(sizeof(unsigned long long), (void **)&cp.d_fb_count));
-  HB_TRY(t->host_alloc(2 * sizeof(unsigned long long), (void **)&cp.h_status));
+  HB_TRY(t->host_alloc(2 * sizeof(unsigned long long), (void **)&cp.h_status,
+                       (void **)&cp.d_status));
   hb::KeyColumns kc{};
   kc.k = (uint32_t)cols.size();
   cp.cols_final = true;
@@ -1366,7 +1386,7 @@ hb_status ensure_key(hb_table *t, const std::vector<int> &cols, hipStream_t stre
   cp.bound_at_launch = ctx->ddict.code_bound.load();
   if (n) hb::launch_compose_keys(kc, (uint32_t)n, cp.d_cells, cp.width, cs);
   HB_TRY(dict_launch(ctx, cp.d_cells, cp.width, cp.key_len, n, cp.d_code, cp.d_fb_rows,
-                     cp.d_fb_count, cp.h_status, cs));
+                     cp.d_fb_count, cp.h_status, cp.d_status, cs));
   if (n && !cp.cols_final) hb::launch_mask_keys(kc, (uint32_t)n, cp.d_code, cs);
   HIP_OK(hipGetLastError());
   HB_TRY(record_lazy(cp.code, cs));
@@ -1423,7 +1443,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   const uint32_t *key_codes = nullptr;
   if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes));
   trace.mark("key");
-  if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream));
+  if (sim >= 0) HB_TRY(ensure_sig(t, sim, stream, false));
   trace.mark("sig");
   uint64_t n_codes = 1;
   if (!eq.empty())
@@ -1453,11 +1473,11 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   if (n == 0 || eq.empty()) ly.n_runs.store(n ? 1 : 0);
   if (n == 0) hb::launch_fill_u64((unsigned long long *)ly.d_nruns, 1, 0ull, stream);
   if (n) {
-    // The run count is counted on the device and copied to pinned memory;
+    // The run count is counted on the device and written to pinned memory;
     // the first batch that needs it waits for the build there.  The
     // build's temporaries return to the pool once the build stream passes
     // them.
-    HB_TRY(t->host_alloc(sizeof(uint32_t), (void **)&ly.h_nruns));
+    HB_TRY(t->host_alloc(sizeof(uint32_t), (void **)&ly.h_nruns, (void **)&ly.d_hnruns));
     DevicePool &pool = t->ctx->pool;
     hb_ctx::Grave temps;
     auto temp = [&](size_t bytes, void **p) -> hb_status {
@@ -1484,10 +1504,9 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       a.run_code = ly.d_run_code;
       a.run_off = ly.d_run_off;
       a.n_runs = ly.d_nruns;
+      a.n_runs_host = ly.d_hnruns;
       hb::launch_fill_u64((unsigned long long *)a.hist, nb / 2, 0ull, stream);
       hb::launch_layout_small(a, stream);
-      HIP_OK(hipMemcpyAsync(ly.h_nruns, ly.d_nruns, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                            stream));
       HIP_OK(hipGetLastError());
       HB_TRY(retire_after(t->ctx, std::move(temps), stream));
       HB_TRY(record_lazy(ly.lz, stream));
@@ -1523,8 +1542,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     hb::launch_run_flags(ly.d_keys, n, flags, stream);
     HIP_OK(hb::exscan_u32(tmp, ts, flags, run_id, n, stream));
     hb::launch_layout_runs(ly.d_keys, flags, run_id, n, ly.d_run_code, ly.d_run_off, d_nruns,
-                           stream);
-    HIP_OK(hipMemcpyAsync(ly.h_nruns, d_nruns, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+                           ly.d_hnruns, stream);
     if (sim >= 0) {
       hb::launch_gather_sig(ly.d_perm, n, t->cols[sim].d_sig, ly.d_jsig, stream);
       // the first-byte order: keys_in is free again once the first sort has run
@@ -1660,8 +1678,17 @@ hb_status enqueue_batch(Batch *b) {
     for (int k = 0; k < (int)it.plan->rules.size(); k++) {
       const PlanRule &rule = it.plan->rules[k];
       for (const auto &s : rule.sims) {
-        HB_TRY(ensure_sig(it.L, s.cl, st));
-        HB_TRY(ensure_sig(it.R, s.cr, st));
+        // the refine rows of a column some rule of the plan refines on (its
+        // first similarity, threshold below kRefineBelow), built with the
+        // signatures in one pass
+        bool refine_l = false, refine_r = false;
+        for (const PlanRule &o : it.plan->rules)
+          if (!o.sims.empty() && o.sims[0].thr < kRefineBelow) {
+            refine_l |= o.sims[0].cl == s.cl;
+            refine_r |= o.sims[0].cr == s.cr;
+          }
+        HB_TRY(ensure_sig(it.L, s.cl, st, refine_l));
+        HB_TRY(ensure_sig(it.R, s.cr, st, refine_r));
       }
       trace.mark("sig");
       JR jr{j, k, hb::kFilterEq, {nullptr, nullptr}, nullptr, {}, {-1, -1}};
@@ -2419,12 +2446,19 @@ hb_status block_noalloc(hb_ctx *ctx, const hb_plan *plan, const hb_table *lc, co
     std::memset(&d, 0, sizeof(d));
     const int sl = rule.sims.empty() ? -1 : rule.sims[0].cl;
     const int sr = rule.sims.empty() ? -1 : rule.sims[0].cr;
-    for (const auto &s : rule.sims) {
-      if (!L->cols[s.cl].sig.built || !R->cols[s.cr].sig.built)
+    for (size_t k = 0; k < rule.sims.size(); k++) {
+      const auto &s = rule.sims[k];
+      const bool refine = k == 0 && s.thr < kRefineBelow;
+      const Column &cl = L->cols[s.cl], &cr = R->cols[s.cr];
+      if (!cl.sig.built || !cr.sig.built || (refine && (!cl.hist.built || !cr.hist.built)))
         return fail(HB_ERR_INVALID_ARG, "hb_block_noalloc: the tables are not prepared for "
                     "this plan (run a job of the plan on them first)");
-      HB_TRY(wait_lazy(L->cols[s.cl].sig, st));
-      HB_TRY(wait_lazy(R->cols[s.cr].sig, st));
+      HB_TRY(wait_lazy(cl.sig, st));
+      HB_TRY(wait_lazy(cr.sig, st));
+      if (refine) {
+        HB_TRY(wait_lazy(cl.hist, st));
+        HB_TRY(wait_lazy(cr.hist, st));
+      }
     }
     for (int side = 0; side < 2; side++) {
       hb_table *t = side ? R : L;
@@ -3229,7 +3263,7 @@ hb_status hb_table_release(hb_ctx *ctx, hb_table *t) {
     if (ly->lz.ev) g.owned.push_back(ly->lz.ev);
   if (!t->parent) {
     for (Column &c : t->cols)
-      for (Lazy *lz : {&c.sig, &c.code})
+      for (Lazy *lz : {&c.sig, &c.hist, &c.code})
         if (lz->ev) g.owned.push_back(lz->ev);
     for (auto &kv : t->composites)
       if (kv.second.code.ev) g.owned.push_back(kv.second.code.ev);
