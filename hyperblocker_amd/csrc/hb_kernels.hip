@@ -1864,20 +1864,44 @@ __device__ __forceinline__ int lane_prefix(const uint8_t *a, const uint8_t *b, i
   return min(p, n);
 }
 
+// Removes the lowest set bit of the NW-word concatenation `cand` from
+// `avail` (x & -x, one borrow chain over the words: v_sub_co / v_subb_co).
+template <int NW>
+__device__ __forceinline__ void take_lowest(uint32_t (&avail)[NW], const uint32_t (&cand)[NW]) {
+  if constexpr (NW == 1) {
+    avail[0] ^= cand[0] & (0u - cand[0]);
+  } else if constexpr (NW == 2) {
+    const uint64_t x = ((uint64_t)cand[1] << 32) | cand[0];
+    const uint64_t n = 0ull - x;
+    avail[0] ^= cand[0] & (uint32_t)n;
+    avail[1] ^= cand[1] & (uint32_t)(n >> 32);
+  } else {
+    static_assert(NW <= 4, "at most 128 inner bytes in lanes");
+    unsigned __int128 x = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) x |= (unsigned __int128)cand[w] << (32 * w);
+    const unsigned __int128 n = (unsigned __int128)0 - x;
+#pragma unroll
+    for (int w = 0; w < NW; w++) avail[w] ^= cand[w] & (uint32_t)(n >> (32 * w));
+  }
+}
+
 // Greedy match count (device_func.cuh:81-111) for one pair per lane, inner
 // strings of at most 32*NW bytes.  The inner string is held as 8 bit-planes
 // (plane k, word w: bit b set iff bit k of inner byte 32w+b is set), so the
 // positions equal to an outer byte c are the AND over k of
 // (plane k XNOR bit k of c), one 3-input bit operation per plane.  `win` is
 // the window [i-h, i+h] of outer position i, advanced by a one-bit left
-// shift per step; `avail` holds inner positions not yet matched.  The lowest
-// candidate is x & -x over the NW-word concatenation (one borrow chain), and
-// the match count is S minus the positions left in `avail` at the end (each
-// match takes exactly one).  No per-step liveness test is needed: cells are
-// NUL padded and hold no NUL within their length, outer blocks past n_out
-// read as zero, and from i >= S + h on the window lies past the inner
-// string.  With HIGH false no inner byte of the wave has bit 7 set, so an
-// outer byte with bit 7 set matches nothing (its bit-7 mask clears `e`).
+// shift per step (position 0 stays inside while i + 1 <= h: the bits shifted
+// in over a block of 16 steps are the low min(16, h - base) bits of `ent`);
+// `avail` holds inner positions not yet matched.  The lowest candidate is
+// x & -x over the NW-word concatenation (one borrow chain), and the match
+// count is S minus the positions left in `avail` at the end (each match
+// takes exactly one).  No per-step liveness test is needed: cells are NUL
+// padded and hold no NUL within their length, outer blocks past n_out read
+// as zero, and from i >= S + h on the window lies past the inner string.
+// With HIGH false no inner byte of the wave has bit 7 set, so an outer byte
+// with bit 7 set matches nothing (its bit-7 mask clears `e`).
 template <int NW, bool HIGH>
 __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, int h, int n_out,
                                            const uint8_t *outer, int n_steps) {
@@ -1898,13 +1922,14 @@ __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, in
     v_next = make_uint4(0, 0, 0, 0);
     if (base + 16 < n_out) v_next = o16[(base >> 4) + 1];
     const uint32_t vw[4] = {v.x, v.y, v.z, v.w};
+    const int left_in = h - base;  // steps q of this block with base + q + 1 <= h: q < left_in
+    const uint32_t ent = left_in >= 16 ? 0xffffu : (left_in > 0 ? (1u << left_in) - 1u : 0u);
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int i = base + q;
-      const uint32_t c = __builtin_amdgcn_ubfe(vw[q >> 2], 8 * (q & 3), 8);
       uint32_t sb[8];
 #pragma unroll
-      for (int k = 0; k < 8; k++) sb[k] = (uint32_t)__builtin_amdgcn_sbfe((int)c, k, 1);
+      for (int k = 0; k < 8; k++)
+        sb[k] = (uint32_t)__builtin_amdgcn_sbfe((int)vw[q >> 2], 8 * (q & 3) + k, 1);
       uint32_t cand[NW];
 #pragma unroll
       for (int w = 0; w < NW; w++) {
@@ -1914,19 +1939,11 @@ __device__ __forceinline__ int lane_greedy(const uint32_t (&P)[NW][8], int S, in
         for (int k = 0; k < (HIGH ? 8 : 7); k++) e = __builtin_amdgcn_bitop3_b32(e, P[w][k], sb[k], 0x90);
         cand[w] = e;
       }
-      unsigned borrow = 0;
-#pragma unroll
-      for (int w = 0; w < NW; w++) {
-        unsigned bo;
-        const uint32_t neg = __builtin_subc(0u, cand[w], borrow, &bo);
-        borrow = bo;
-        avail[w] ^= cand[w] & neg;
-      }
-      // advance the window: positions shift up by one, and position 0 stays
-      // inside while i + 1 <= h
+      take_lowest<NW>(avail, cand);
+      // advance the window: positions shift up by one
 #pragma unroll
       for (int w = NW - 1; w > 0; w--) win[w] = __builtin_amdgcn_alignbit(win[w], win[w - 1], 31);
-      win[0] = (win[0] << 1) | (uint32_t)(i + 1 <= h);
+      win[0] = (win[0] << 1) | __builtin_amdgcn_ubfe(ent, q, 1);
     }
   }
   int left = 0;
@@ -2131,8 +2148,10 @@ __device__ __forceinline__ void lane_chain(const SimDesc &sd, const PairEntry &e
   p = act ? lane_prefix(ca, cb, S) : 0;
   const bool live = act && S > 0;
   const bool gt32 = __ballot(live && S > 32) != 0, gt64 = __ballot(live && S > 64) != 0;
+  const bool gt96 = __ballot(live && S > 96) != 0;
   if (!gt32) m = lane_matches<1>(planes, nwc, S, T, outer, live);
   else if (!gt64) m = lane_matches<2>(planes, nwc, S, T, outer, live);
+  else if (!gt96) m = lane_matches<3>(planes, nwc, S, T, outer, live);
   else m = lane_matches<4>(planes, nwc, S, T, outer, live);  // S <= 128 here
   if (!live) m = 0;
 }
