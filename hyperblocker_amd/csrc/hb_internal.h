@@ -338,14 +338,19 @@ void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigR
 // The whole layout of a table whose join keys take at most kLayoutMaxCodes
 // codes, by counting sorts: join order (perm, jsig) by (code, length), the
 // first-byte order (perm_fb, jsig_fb; with a similarity column) by (code,
-// first byte), the runs of equal code and their count.  hist (2 * n_codes *
-// 256 counters) must be zero; rank holds 2 * n entries, off 2 * n_codes *
-// 256.  Within a bucket rows come in no particular order.
+// first byte), the runs of equal code and their count, in two launches.
+// hist (2 * kLayoutMaxCodes * 256 counters), *bad and *done must be zero
+// and are left zero (layouts built one after another on one stream can
+// share them); the layout's own flag *bad_out is written (1 when a code was
+// not below n_codes).  rank holds 2 * n entries, off 2 * n_codes * 256.
+// Within a bucket rows come in no particular order.
 constexpr uint32_t kLayoutMaxCodes = 64;
-constexpr uint32_t kLayoutLdsCodes = 32;  // counts taken in LDS (64 KiB) up to this many codes
+constexpr uint32_t kLayoutLdsCodes = 28;  // counts taken in LDS (56 KiB) up to this many codes
 struct LayoutSmall {
   const uint32_t *code;  // null: a rule without equalities (one code, 0)
   uint32_t *bad;         // set when a code is not below n_codes (the row then counts as code 0)
+  uint32_t *bad_out;     // the layout's flag: *bad as the counting launch ends
+  uint32_t *done;        // work-groups of the counting launch done
   const uint16_t *len;   // null: no similarity column
   const SigRow *sig;     // null: no similarity column
   uint32_t n, n_codes;

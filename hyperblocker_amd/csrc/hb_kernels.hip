@@ -873,66 +873,43 @@ __global__ void k_layout_keys(const uint32_t *__restrict__ code, const uint16_t 
 // ---- join layouts by counting sort (keys of few codes) ------------------
 //
 // Buckets (code, min(length, 255)) for the join order and (code, first
-// byte) for the first-byte order, n_codes * 256 of each.  k_lsmall_count
-// takes each row's rank in its two buckets with global atomics (the order
-// within a bucket is arbitrary: its rows share the key the layout orders
-// by), k_lsmall_scan turns the counts into bucket offsets and the runs of
-// equal code, and k_lsmall_scatter places every row and its signature.
+// byte) for the first-byte order, n_codes * 256 of each.  One counting
+// launch takes each row's rank in its two buckets (the order within a
+// bucket is arbitrary: its rows share the key the layout orders by); its
+// last work-group to finish turns the counts into bucket offsets and the
+// runs of equal code and leaves the counters zero again; k_lsmall_scatter
+// places every row and its signature.
 
-__device__ __forceinline__ uint32_t lsmall_code(const LayoutSmall &a, uint32_t i) {
+__device__ __forceinline__ uint32_t lsmall_code(const LayoutSmall &a, uint32_t i, uint32_t *bad) {
   const uint32_t c = a.code ? a.code[i] : 0u;
   if (c < a.n_codes) return c;
-  *a.bad = 1u;
+  *bad = 1u;
   return 0u;
 }
 
-// Ranks from global atomics (keys of up to kLayoutMaxCodes codes).
-__global__ void k_lsmall_count(LayoutSmall a) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const uint32_t base = lsmall_code(a, i) << 8;
-  const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
-  a.rank[i] = atomicAdd(a.hist + base + l, 1u);
-  if (a.sig) {
-    const uint32_t fb = a.sig[i].pre & 0xffu;
-    a.rank[a.n + i] = atomicAdd(a.hist + a.n_codes * 256u + base + fb, 1u);
+// Whether this work-group is the last of the launch to get here: each
+// group's earlier global writes and atomics are visible to the last one
+// (every group fences before counting itself done, the last fences after).
+__device__ __forceinline__ bool last_group(uint32_t *done) {
+  __shared__ uint32_t s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(done, 1u) == gridDim.x - 1 ? 1u : 0u;
   }
+  __syncthreads();
+  const bool last = s_last != 0u;
+  if (last) __threadfence();
+  return last;
 }
 
-// The same with the counts of a work-group's 1024 rows (one per thread)
-// taken in LDS first (keys of up to kLayoutLdsCodes codes): one global
-// atomic per bucket the work-group uses.
-constexpr uint32_t kLayoutRowsPerGroup = 1024;
-__global__ __launch_bounds__(1024) void k_lsmall_count_lds(LayoutSmall a) {
-  extern __shared__ uint32_t s_cnt[];  // buckets of both orders
-  const uint32_t t = threadIdx.x;
-  const uint32_t nb = a.n_codes * 256u, nb2 = a.sig ? 2u * nb : nb;
-  for (uint32_t k = t; k < nb2; k += 1024) s_cnt[k] = 0;
-  __syncthreads();
-  const uint32_t i = blockIdx.x * kLayoutRowsPerGroup + t;
-  uint32_t b1 = 0xffffffffu, b2 = 0xffffffffu, r1 = 0, r2 = 0;
-  if (i < a.n) {
-    const uint32_t base = lsmall_code(a, i) << 8;
-    const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
-    b1 = base + l;
-    if (a.sig) b2 = nb + base + (a.sig[i].pre & 0xffu);
-  }
-  if (b1 != 0xffffffffu) r1 = atomicAdd(s_cnt + b1, 1u);
-  if (b2 != 0xffffffffu) r2 = atomicAdd(s_cnt + b2, 1u);
-  __syncthreads();
-  for (uint32_t k = t; k < nb2; k += 1024) {  // the work-group's base in each bucket
-    const uint32_t c = s_cnt[k];
-    s_cnt[k] = c ? atomicAdd(a.hist + k, c) : 0u;
-  }
-  __syncthreads();
-  if (b1 != 0xffffffffu) a.rank[i] = s_cnt[b1] + r1;
-  if (b2 != 0xffffffffu) a.rank[a.n + i] = s_cnt[b2] + r2;
-}
-
-// One work-group of 1024 threads: exclusive scans of the two count arrays
-// (16 buckets per thread, so thread 16 c starts code c), then the runs of
-// the join order, one per code with rows (codes < 64: one wave's ballot).
-__global__ __launch_bounds__(1024) void k_lsmall_scan(LayoutSmall a) {
+// Run by the counting launch's last work-group (1024 threads): exclusive
+// scans of the two count arrays (16 buckets per thread, so thread 16 c
+// starts code c), the runs of the join order (one per code with rows;
+// codes < 64: one wave's ballot), the layout's bad flag; then the counters,
+// the flag accumulator and the done count are cleared for the next layout
+// built on the same stream.
+__device__ void lsmall_scan(const LayoutSmall &a) {
   __shared__ uint32_t s_part[1024];
   __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
   const uint32_t t = threadIdx.x;
@@ -970,6 +947,7 @@ __global__ __launch_bounds__(1024) void k_lsmall_scan(LayoutSmall a) {
     if (o == 0 && mine && (t & 15u) == 0) s_code_start[t >> 4] = run;
     if (mine) {
       uint4 *o4 = reinterpret_cast<uint4 *>(a.off + o * nb + 16u * t);
+      uint4 *z4 = reinterpret_cast<uint4 *>(a.hist + o * nb + 16u * t);
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         uint4 x;
@@ -982,6 +960,7 @@ __global__ __launch_bounds__(1024) void k_lsmall_scan(LayoutSmall a) {
         x.w = run;
         run += v[4 * q + 3];
         o4[q] = x;
+        z4[q] = make_uint4(0u, 0u, 0u, 0u);
       }
     }
     __syncthreads();
@@ -1002,14 +981,63 @@ __global__ __launch_bounds__(1024) void k_lsmall_scan(LayoutSmall a) {
       a.run_off[nr] = a.n;
       *a.n_runs = nr;
       *a.n_runs_host = nr;
+      *a.bad_out = *a.bad;
+      *a.bad = 0u;
+      *a.done = 0u;
     }
   }
+}
+
+// Ranks from global atomics (keys of up to kLayoutMaxCodes codes).
+__global__ __launch_bounds__(1024) void k_lsmall_count(LayoutSmall a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) {
+    const uint32_t base = lsmall_code(a, i, a.bad) << 8;
+    const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+    a.rank[i] = atomicAdd(a.hist + base + l, 1u);
+    if (a.sig) {
+      const uint32_t fb = a.sig[i].pre & 0xffu;
+      a.rank[a.n + i] = atomicAdd(a.hist + a.n_codes * 256u + base + fb, 1u);
+    }
+  }
+  if (last_group(a.done)) lsmall_scan(a);
+}
+
+// The same with the counts of a work-group's 1024 rows (one per thread)
+// taken in LDS first (keys of up to kLayoutLdsCodes codes): one global
+// atomic per bucket the work-group uses.
+constexpr uint32_t kLayoutRowsPerGroup = 1024;
+__global__ __launch_bounds__(1024) void k_lsmall_count_lds(LayoutSmall a) {
+  extern __shared__ uint32_t s_cnt[];  // buckets of both orders
+  const uint32_t t = threadIdx.x;
+  const uint32_t nb = a.n_codes * 256u, nb2 = a.sig ? 2u * nb : nb;
+  for (uint32_t k = t; k < nb2; k += 1024) s_cnt[k] = 0;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kLayoutRowsPerGroup + t;
+  uint32_t b1 = 0xffffffffu, b2 = 0xffffffffu, r1 = 0, r2 = 0;
+  if (i < a.n) {
+    const uint32_t base = lsmall_code(a, i, a.bad) << 8;
+    const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+    b1 = base + l;
+    if (a.sig) b2 = nb + base + (a.sig[i].pre & 0xffu);
+  }
+  if (b1 != 0xffffffffu) r1 = atomicAdd(s_cnt + b1, 1u);
+  if (b2 != 0xffffffffu) r2 = atomicAdd(s_cnt + b2, 1u);
+  __syncthreads();
+  for (uint32_t k = t; k < nb2; k += 1024) {  // the work-group's base in each bucket
+    const uint32_t c = s_cnt[k];
+    s_cnt[k] = c ? atomicAdd(a.hist + k, c) : 0u;
+  }
+  __syncthreads();
+  if (b1 != 0xffffffffu) a.rank[i] = s_cnt[b1] + r1;
+  if (b2 != 0xffffffffu) a.rank[a.n + i] = s_cnt[b2] + r2;
+  if (last_group(a.done)) lsmall_scan(a);
 }
 
 __global__ void k_lsmall_scatter(LayoutSmall a) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  const uint32_t base = lsmall_code(a, i) << 8;
+  const uint32_t base = lsmall_code(a, i, a.bad_out) << 8;
   const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
   const uint32_t pos = a.off[base + l] + a.rank[i];
   a.perm[pos] = i;
@@ -2486,12 +2514,11 @@ void launch_layout_small(const LayoutSmall &a, hipStream_t stream) {
   if (!a.n) return;
   const uint32_t blocks = (a.n + 255) / 256;
   const size_t lds = (a.sig ? 2 : 1) * a.n_codes * 256 * sizeof(uint32_t);
+  const uint32_t groups = (a.n + kLayoutRowsPerGroup - 1) / kLayoutRowsPerGroup;
   if (a.n_codes <= kLayoutLdsCodes)
-    k_lsmall_count_lds<<<(a.n + kLayoutRowsPerGroup - 1) / kLayoutRowsPerGroup, 1024, lds,
-                         stream>>>(a);
+    k_lsmall_count_lds<<<groups, 1024, lds, stream>>>(a);
   else
-    k_lsmall_count<<<blocks, 256, 0, stream>>>(a);
-  k_lsmall_scan<<<1, 1024, 0, stream>>>(a);
+    k_lsmall_count<<<groups, 1024, 0, stream>>>(a);
   k_lsmall_scatter<<<blocks, 256, 0, stream>>>(a);
 }
 

@@ -607,6 +607,12 @@ struct hb_ctx {
   // Test hook: build a similarity column's structures with the three
   // separate kernels instead of the fused one (HB_SEPARATE_COLUMN_BUILDS).
   bool separate_col_builds = false;
+  // Counters shared by the counting-sort layout builds, which run one after
+  // another on the build stream (each build leaves them zero): bucket
+  // counts (2 * kLayoutMaxCodes * 256), then a bad-code flag and a done count.
+  // layout_mu keeps one build's two launches together on the stream.
+  uint32_t *layout_hist = nullptr;
+  std::mutex layout_mu;
   // hb_block_noalloc: the device address of this GPU's copy of the
   // library's static scratch array (looked up at creation) and the stream
   // its work runs on.
@@ -1468,7 +1474,9 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   }
   HB_TRY(t->dev_alloc(sizeof(uint64_t), (void **)&ly.d_nruns));
   HB_TRY(t->dev_alloc(sizeof(uint64_t), (void **)&ly.d_bad));
-  hb::launch_fill_u64((unsigned long long *)ly.d_bad, 1, 0ull, stream);
+  // the counting sort writes the flag itself; the other builds set it only
+  if (n == 0 || n_codes > hb::kLayoutMaxCodes)
+    hb::launch_fill_u64((unsigned long long *)ly.d_bad, 1, 0ull, stream);
   ly.runs_bound = (uint32_t)std::min<uint64_t>(n, n_codes);
   if (n == 0 || eq.empty()) ly.n_runs.store(n ? 1 : 0);
   if (n == 0) hb::launch_fill_u64((unsigned long long *)ly.d_nruns, 1, 0ull, stream);
@@ -1485,16 +1493,15 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       temps.dev.push_back({*p, bytes});
       return HB_OK;
     };
-    if (n_codes <= hb::kLayoutMaxCodes) {  // counting sorts: 4 launches
+    if (n_codes <= hb::kLayoutMaxCodes) {  // counting sorts: 2 launches
       hb::LayoutSmall a{};
       a.code = key_codes;
-      a.bad = ly.d_bad;
+      a.bad_out = ly.d_bad;
       a.len = sim >= 0 ? t->cols[sim].d_len : nullptr;
       a.sig = sim >= 0 ? t->cols[sim].d_sig : nullptr;
       a.n = n;
       a.n_codes = (uint32_t)n_codes;
       const size_t nb = 2 * (size_t)n_codes * 256;
-      HB_TRY(temp(nb * sizeof(uint32_t), (void **)&a.hist));
       HB_TRY(temp(nb * sizeof(uint32_t), (void **)&a.off));
       HB_TRY(temp(2 * n1 * sizeof(uint32_t), (void **)&a.rank));
       a.perm = ly.d_perm;
@@ -1505,9 +1512,19 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       a.run_off = ly.d_run_off;
       a.n_runs = ly.d_nruns;
       a.n_runs_host = ly.d_hnruns;
-      hb::launch_fill_u64((unsigned long long *)a.hist, nb / 2, 0ull, stream);
-      hb::launch_layout_small(a, stream);
-      HIP_OK(hipGetLastError());
+      {
+        std::lock_guard<std::mutex> ll(t->ctx->layout_mu);
+        if (!t->ctx->layout_hist) {  // created zero, on first use
+          const size_t words = 2 * (size_t)hb::kLayoutMaxCodes * 256 + 64;
+          HIP_OK(hipMalloc((void **)&t->ctx->layout_hist, words * sizeof(uint32_t)));
+          hb::launch_fill_u64((unsigned long long *)t->ctx->layout_hist, words / 2, 0ull, stream);
+        }
+        a.hist = t->ctx->layout_hist;
+        a.bad = a.hist + 2 * (size_t)hb::kLayoutMaxCodes * 256;
+        a.done = a.bad + 32;
+        hb::launch_layout_small(a, stream);
+        HIP_OK(hipGetLastError());
+      }
       HB_TRY(retire_after(t->ctx, std::move(temps), stream));
       HB_TRY(record_lazy(ly.lz, stream));
       trace.mark("launch");
@@ -3012,6 +3029,7 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->build_stream) (void)hipStreamDestroy(ctx->build_stream);
+  if (ctx->layout_hist) (void)hipFree(ctx->layout_hist);
   if (ctx->code_stream) (void)hipStreamDestroy(ctx->code_stream);
   if (ctx->noalloc_stream) (void)hipStreamDestroy(ctx->noalloc_stream);
   delete ctx;
