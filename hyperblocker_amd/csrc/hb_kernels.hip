@@ -748,8 +748,8 @@ __global__ void k_mask_keys(KeyColumns cols, uint32_t n, uint32_t *__restrict__ 
 // stride).  A block is one contiguous span of the input: it is loaded into
 // LDS (16-byte loads when the rows are 16-byte aligned), each cell's length
 // (up to its first NUL, at most `size`) is found there, and each column's
-// cells of the block, one contiguous span of that column, are written as
-// dwords with every byte from the cell's end on zero.
+// cells of the block, one contiguous span of that column, are written in
+// 16-byte chunks with every byte from the cell's end on zero.
 template <bool ALIGNED16>
 __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__ rows,
                                                     uint64_t row_bytes, uint32_t n_rows,
@@ -763,9 +763,18 @@ __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__
   // set where the byte is zero; the length of every cell of the block
   const uint32_t rows_room = ((span_max + 31u) & ~31u) + 32u;
   uint32_t *s_zero = reinterpret_cast<uint32_t *>(s_rows + rows_room);
-  uint8_t *s_zero8 = reinterpret_cast<uint8_t *>(s_zero);
   uint16_t *s_len = reinterpret_cast<uint16_t *>(s_zero + (span_max + 31u) / 32u + 1u);
   const uint32_t tid = threadIdx.x;
+  // s_keep[k]: a mask of the first k bytes of a 16-byte chunk (k <= 16)
+  __shared__ uint4 s_keep[17];
+  if (tid < 17) {
+    const int k = (int)tid;
+    auto dw = [k](int d) -> uint32_t {
+      const int nb = k - 4 * d;
+      return nb >= 4 ? ~0u : (nb <= 0 ? 0u : (1u << (8 * nb)) - 1u);
+    };
+    s_keep[tid] = make_uint4(dw(0), dw(1), dw(2), dw(3));
+  }
   const uint32_t n_blocks = (n_rows + rows_per_group - 1) / rows_per_group;
   for (uint32_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
     const uint32_t r0 = blk * rows_per_group;
@@ -783,17 +792,21 @@ __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__
     // the bytes from span to the next 32-byte boundary read as zero
     for (uint32_t i = span + tid; i < ((span + 31u) & ~31u) + 32u; i += 256) s_rows[i] = 0;
     __syncthreads();
-    // zero-byte bitmap: each thread takes 8 bytes (one 8-bit piece)
-    for (uint32_t i = tid; i < ((span + 31u) / 32u) * 4u; i += 256) {  // to the word's end
-      const uint2 x = reinterpret_cast<const uint2 *>(s_rows32)[i];
+    // zero-byte bitmap: each thread takes 16 bytes (one 16-bit piece).  The
+    // flags of a dword's zero bytes sit at bits 7, 15, 23 and 31 of m; one
+    // multiply gathers them into bits 21-24 (each lands on its own bit, so
+    // nothing carries).
+    for (uint32_t i = tid; i < ((span + 31u) / 32u) * 2u; i += 256) {  // to the word's end
+      const uint4 x = s_mem4[i];
+      const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
       uint32_t z = 0;
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const uint32_t v = h ? x.y : x.x;
+      for (int h = 0; h < 4; h++) {
+        const uint32_t v = xs[h];
         const uint32_t m = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // zero bytes
-        z |= (((m >> 7) & 1u) | ((m >> 14) & 2u) | ((m >> 21) & 4u) | ((m >> 28) & 8u)) << (4 * h);
+        z |= (((m >> 7) * 0x00204081u) >> 21 & 0xfu) << (4 * h);
       }
-      s_zero8[i] = (uint8_t)z;
+      reinterpret_cast<uint16_t *>(s_zero)[i] = (uint16_t)z;
     }
     __syncthreads();
     // each cell's length: its first zero byte, at most `size`
@@ -814,35 +827,41 @@ __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__
     }
     __syncthreads();
     for (uint32_t c = 0; c < n_cols; c++) {
-      const uint32_t wdw = cols[c].width >> 2;  // dwords per cell
+      const uint32_t cpc = cols[c].width >> 4;  // 16-byte chunks per cell
       const uint32_t off = (uint32_t)cols[c].offset;
-      uint32_t *dst = reinterpret_cast<uint32_t *>(cols[c].out + (size_t)r0 * cols[c].width);
-      // thread t takes dword k of cells j, j + per, ... (one division per column)
-      const uint32_t per = wdw <= 256u ? 256u / wdw : 0u;
-      const uint32_t tj = per ? tid / wdw : 0u, tk = per ? tid - tj * wdw : 0u;
+      uint4 *dst = reinterpret_cast<uint4 *>(cols[c].out + (size_t)r0 * cols[c].width);
+      // thread t takes chunk q of cells j, j + per, ... (one division per column)
+      const uint32_t per = cpc <= 256u ? 256u / cpc : 0u;
+      const uint32_t tj = per ? tid / cpc : 0u, tq = per ? tid - tj * cpc : 0u;
       for (uint32_t it = 0;; it++) {
-        uint32_t j, k;
+        uint32_t j, q;
         if (per) {
           if (tj >= per) break;
           j = tj + it * per;
-          k = tk;
+          q = tq;
           if (j >= nr) break;
-        } else {  // cells wider than 1 KiB
+        } else {  // cells wider than 4 KiB
           const uint32_t i = tid + it * 256u;
-          if (i >= nr * wdw) break;
-          j = i / wdw;
-          k = i - j * wdw;
+          if (i >= nr * cpc) break;
+          j = i / cpc;
+          q = i - j * cpc;
         }
         const uint32_t len = s_len[j * n_cols + c];
-        uint32_t v = 0;
-        if (4 * k < len) {
-          const uint32_t pos = j * (uint32_t)row_bytes + off + 4 * k;
-          const uint32_t a = s_rows32[pos >> 2], b = s_rows32[(pos >> 2) + 1];
-          v = __builtin_amdgcn_alignbyte(b, a, pos & 3u);
-          const uint32_t keep = len - 4 * k;  // bytes of this dword inside the cell
-          if (keep < 4) v &= (1u << (8 * keep)) - 1u;
-        }
-        dst[(size_t)j * wdw + k] = v;
+        const uint32_t base = 16u * q;
+        // 16 bytes from any byte position: five dwords and four byte shifts
+        // (a chunk past the cell's end reads bytes of the block, all masked)
+        const uint32_t pos = j * (uint32_t)row_bytes + off + base;
+        const uint32_t w = pos >> 2, sh = pos & 3u;
+        const uint32_t d0 = s_rows32[w], d1 = s_rows32[w + 1], d2 = s_rows32[w + 2],
+                       d3 = s_rows32[w + 3], d4 = s_rows32[w + 4];
+        // the bytes of this chunk inside the cell: min(16, len - base), or 0
+        const uint4 m = s_keep[len > base ? min(len - base, 16u) : 0u];
+        uint4 v;
+        v.x = __builtin_amdgcn_alignbyte(d1, d0, sh) & m.x;
+        v.y = __builtin_amdgcn_alignbyte(d2, d1, sh) & m.y;
+        v.z = __builtin_amdgcn_alignbyte(d3, d2, sh) & m.z;
+        v.w = __builtin_amdgcn_alignbyte(d4, d3, sh) & m.w;
+        dst[j * cpc + q] = v;
       }
     }
   }

@@ -71,19 +71,18 @@ hb_status fail(hb_status st, const char *fmt, ...) {
 class DevicePool {
  public:
   ~DevicePool() { release_all(); }
+  // Gives back memory of released tables and batches whose streams have
+  // passed their release (set by the context); tried before a new block.
+  std::function<void()> on_miss;
   hb_status alloc(size_t bytes, void **out) {
     bytes = round(bytes);
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      auto it = free_.find(bytes);
-      if (it != free_.end() && !it->second.empty()) {
-        *out = it->second.back();
-        it->second.pop_back();
-        cached_ -= bytes;
-        return HB_OK;
-      }
+    if (take(bytes, out)) return HB_OK;
+    if (on_miss) {
+      on_miss();
+      if (take(bytes, out)) return HB_OK;
     }
     void *p = nullptr;
+    misses_++;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       // Give cached blocks back to the driver and retry once.
@@ -129,6 +128,20 @@ class DevicePool {
     return cached_;
   }
   size_t live() const { return live_.load(); }
+  std::atomic<uint64_t> misses_{0};  // allocations the cache could not serve
+
+ private:
+  bool take(size_t bytes, void **out) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.find(bytes);
+    if (it == free_.end() || it->second.empty()) return false;
+    *out = it->second.back();
+    it->second.pop_back();
+    cached_ -= bytes;
+    return true;
+  }
+
+ public:
 
  private:
   static size_t round(size_t b) {
@@ -150,19 +163,16 @@ class DevicePool {
 class HostPool {
  public:
   ~HostPool() { release_all(); }
+  std::function<void()> on_miss;  // as DevicePool::on_miss
   hb_status alloc(size_t bytes, void **host, void **dev) {
     bytes = round(bytes);
-    void *p = nullptr;
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      auto it = free_.find(bytes);
-      if (it != free_.end() && !it->second.empty()) {
-        p = it->second.back();
-        it->second.pop_back();
-        cached_ -= bytes;
-      }
+    void *p = take(bytes);
+    if (!p && on_miss) {
+      on_miss();
+      p = take(bytes);
     }
     if (!p) {
+      misses_++;
       hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocMapped);
       if (e != hipSuccess) {
         release_all();
@@ -210,8 +220,18 @@ class HostPool {
     return p;
   }
   static constexpr size_t kLimit = 4ull << 30;  // cached (free) pinned bytes kept at most
+  void *take(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.find(bytes);
+    if (it == free_.end() || it->second.empty()) return nullptr;
+    void *p = it->second.back();
+    it->second.pop_back();
+    cached_ -= bytes;
+    return p;
+  }
 
  public:
+  std::atomic<uint64_t> misses_{0};  // allocations the cache could not serve
   size_t cached() {
     std::lock_guard<std::mutex> lk(mu_);
     return cached_;
@@ -1024,6 +1044,28 @@ hb_status record_lazy(Lazy &lz, hipStream_t stream) {
 
 hb_status retire_after(hb_ctx *ctx, hb_ctx::Grave &&g, hipStream_t stream);
 
+// Host time of the phases of a call, printed to stderr when the environment
+// sets HB_HOST_TRACE (a diagnostic for host-bound pipelines).
+struct HostTrace {
+  const char *label;
+  bool on = std::getenv("HB_HOST_TRACE") != nullptr;
+  Clock::time_point t0 = Clock::now(), last = t0;
+  std::string line;
+  explicit HostTrace(const char *l) : label(l) {}
+  void mark(const char *what) {
+    if (!on) return;
+    const auto now = Clock::now();
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), " %s %.3f", what,
+                  std::chrono::duration<double, std::milli>(now - last).count());
+    line += buf;
+    last = now;
+  }
+  ~HostTrace() {
+    if (on) std::fprintf(stderr, "%s ms:%s\n", label, line.c_str());
+  }
+};
+
 // The signatures, bit-planes (and with `refine` the refine rows) of
 // similarity column c, built on first use; a slice borrows its root's.  The
 // refine rows are built only for columns a refining rule reads (first
@@ -1055,6 +1097,7 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream, bool refine) {
     HB_TRY(wait_lazy(col.sig, stream));
     return refine ? wait_lazy(col.hist, stream) : HB_OK;
   }
+  HostTrace trace("ensure_sig");
   const size_t n = (size_t)std::max<int64_t>(t->n_rows, 1);
   if (need_hist) HB_TRY(t->dev_alloc(n * hb::kHistWords * 4, (void **)&col.d_hist));
   if (need_sig) {
@@ -1062,6 +1105,7 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream, bool refine) {
     HB_TRY(t->dev_alloc(n * sizeof(hb::SigRow), (void **)&col.d_sig));
     col.nwc = std::min<uint32_t>((col.width + 31) / 32, hb::kMaxPlaneWords);
     HB_TRY(t->dev_alloc(n * col.nwc * 32, (void **)&col.d_planes));
+    trace.mark("alloc");
     if (t->ctx->separate_col_builds) {
       hb::launch_build_sig(col.d_bytes, col.width, t->n_rows, col.d_len, col.d_sig, stream);
       hb::launch_build_planes(col.d_bytes, col.width, t->n_rows, col.nwc, col.d_planes, stream);
@@ -1075,32 +1119,13 @@ hb_status ensure_sig(hb_table *t, int c, hipStream_t stream, bool refine) {
     hb::launch_build_hist(col.d_bytes, col.width, t->n_rows, col.d_hist, stream);
   }
   HIP_OK(hipGetLastError());
+  trace.mark("launch");
   if (need_sig) HB_TRY(record_lazy(col.sig, stream));
   if (need_hist) HB_TRY(record_lazy(col.hist, stream));
+  trace.mark("event");
   return HB_OK;
 }
 
-// Host time of the phases of a call, printed to stderr when the environment
-// sets HB_HOST_TRACE (a diagnostic for host-bound pipelines).
-struct HostTrace {
-  const char *label;
-  bool on = std::getenv("HB_HOST_TRACE") != nullptr;
-  Clock::time_point t0 = Clock::now(), last = t0;
-  std::string line;
-  explicit HostTrace(const char *l) : label(l) {}
-  void mark(const char *what) {
-    if (!on) return;
-    const auto now = Clock::now();
-    char buf[64];
-    std::snprintf(buf, sizeof(buf), " %s %.3f", what,
-                  std::chrono::duration<double, std::milli>(now - last).count());
-    line += buf;
-    last = now;
-  }
-  ~HostTrace() {
-    if (on) std::fprintf(stderr, "%s ms:%s\n", label, line.c_str());
-  }
-};
 
 // Creates the context's device dictionary on first use (sizes from
 // HB_DICT_SLOTS, HB_DICT_ARENA_MB, HB_DICT_MAX_PROBE and HB_DICT_HASH_BITS at
@@ -1474,6 +1499,7 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
   }
   HB_TRY(t->dev_alloc(sizeof(uint64_t), (void **)&ly.d_nruns));
   HB_TRY(t->dev_alloc(sizeof(uint64_t), (void **)&ly.d_bad));
+  trace.mark("alloc");
   // the counting sort writes the flag itself; the other builds set it only
   if (n == 0 || n_codes > hb::kLayoutMaxCodes)
     hb::launch_fill_u64((unsigned long long *)ly.d_bad, 1, 0ull, stream);
@@ -1522,9 +1548,11 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
         a.hist = t->ctx->layout_hist;
         a.bad = a.hist + 2 * (size_t)hb::kLayoutMaxCodes * 256;
         a.done = a.bad + 32;
+        trace.mark("temps");
         hb::launch_layout_small(a, stream);
         HIP_OK(hipGetLastError());
       }
+      trace.mark("kernels");
       HB_TRY(retire_after(t->ctx, std::move(temps), stream));
       HB_TRY(record_lazy(ly.lz, stream));
       trace.mark("launch");
@@ -2362,8 +2390,16 @@ hb_status submit_batch(hb_ctx *ctx, uint32_t n, const hb_plan *const *plans,
   std::vector<Job *> jobs;
   HB_TRY(register_jobs(ctx, n, plans, ls, rs, ids, jobs));
   trace.mark("register");
+  const uint64_t dm = ctx->pool.misses_.load(), hm = ctx->hpool.misses_.load();
   const hb_status st = launch_batch(ctx, jobs);
   trace.mark("launch");
+  if (trace.on) {  // allocations the pools could not serve
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), " | device misses %llu, pinned misses %llu",
+                  (unsigned long long)(ctx->pool.misses_.load() - dm),
+                  (unsigned long long)(ctx->hpool.misses_.load() - hm));
+    trace.line += buf;
+  }
   return st;
 }
 
@@ -2954,6 +2990,11 @@ hb_status hb_ctx_create(int32_t device, hb_ctx **out) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b)
       ctx->pool.set_limit(std::max<size_t>(8ull << 30, total_b / 8));
+  }
+  {
+    hb_ctx *c = ctx.get();
+    ctx->pool.on_miss = [c] { (void)reap_graves(c, false); };
+    ctx->hpool.on_miss = [c] { (void)reap_graves(c, false); };
   }
   HIP_OK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&ctx->build_stream, hipStreamNonBlocking));
