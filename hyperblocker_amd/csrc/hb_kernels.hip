@@ -455,28 +455,71 @@ __device__ __forceinline__ uint32_t nonzero_bytes(uint32_t x) {
   return (uint32_t)__popc((y | x) & 0x80808080u);
 }
 
+// Adds the 32-class counts of one 32-byte block to the signature counters
+// h (counter b in byte b & 3 of word b >> 2), from the block's bit-planes:
+// the bytes of class k (byte & 31 == k) are the AND over bits b < 5 of
+// (P[b] if bit b of k is set, else ~P[b]), built as a tree over bits 4..0
+// (62 two-input operations), counted with one popcount each.  Class 0 also
+// needs a set bit among bits 5-7, so zero bytes (the NUL padding) count
+// nowhere.
+__device__ __forceinline__ void count_classes(const uint32_t (&P)[8], uint32_t (&h)[8]) {
+  uint32_t t1[2] = {~P[4], P[4]};
+  uint32_t t2[4], t3[8], t4[16];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    t2[2 * i] = t1[i] & ~P[3];
+    t2[2 * i + 1] = t1[i] & P[3];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    t3[2 * i] = t2[i] & ~P[2];
+    t3[2 * i + 1] = t2[i] & P[2];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    t4[2 * i] = t3[i] & ~P[1];
+    t4[2 * i + 1] = t3[i] & P[1];
+  }
+  // t4[i]: bits 4..1 of the byte equal i's bits 3..0 (i = class >> 1)
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    uint32_t m0 = t4[i] & ~P[0];
+    if (i == 0) m0 &= P[5] | P[6] | P[7];  // class 0 of a non-zero byte
+    const uint32_t m1 = t4[i] & P[0];
+    const int k0 = 2 * i, k1 = 2 * i + 1;  // classes; counter k in byte k & 3 of word k >> 2
+    h[k0 >> 2] += (uint32_t)__popc(m0) << (8 * (k0 & 3));
+    h[k1 >> 2] += (uint32_t)__popc(m1) << (8 * (k1 & 3));
+  }
+}
+
 // Every per-row structure of a similarity column from one read of its
 // cells, four lanes per row: lane q takes the row's 32-byte blocks q, q + 4,
-// ...  Bit-planes of a block: four 8 x 8 bit transposes.  Byte classes: one
-// LDS atomic add per byte into the row's 128 counters (byte_class, u8 each;
-// zero bytes, which follow the cell's NUL, add nothing), from which the
-// signature's 32 counters (byte & 31) are sums of four.  The length counts
-// the non-zero bytes (every byte after the NUL is zero).  Results equal
-// those of k_build_sig, k_build_hist and k_build_planes; the refine rows
-// are written only when `hist` is not null.
+// ...  Bit-planes of a block: four 8 x 8 bit transposes.  Without refine
+// rows (HIST false) the signature's 32 counters are counted from the
+// block's bit-planes in registers and summed over the row's four lanes;
+// with them, one LDS atomic add per byte goes into the row's 128 counters
+// (byte_class, u8 each; zero bytes, which follow the cell's NUL, add
+// nothing), from which the signature's 32 counters (byte & 31) are sums of
+// four.  The length counts the non-zero bytes (every byte after the NUL is
+// zero).  Results equal those of k_build_sig, k_build_hist and
+// k_build_planes.
+template <bool HIST>
 __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ bytes,
                                                   uint32_t width, int64_t n, uint32_t nwc,
                                                   uint16_t *__restrict__ len_out,
                                                   SigRow *__restrict__ sig_out,
                                                   uint32_t *__restrict__ hist,
                                                   uint32_t *__restrict__ planes) {
-  __shared__ uint32_t s_h[kClassWords][64];  // [class word][row of the work-group]
+  __shared__ uint32_t s_h[HIST ? kClassWords : 1][64];  // [class word][row of the work-group]
   const uint32_t t = threadIdx.x, q = t & 3u, rl = t >> 2;
   const int64_t r = (int64_t)blockIdx.x * 64 + rl;
   const bool valid = r < n;
-  for (uint32_t i = t; i < kClassWords * 64; i += 256) (&s_h[0][0])[i] = 0;
-  __syncthreads();
+  if (HIST) {
+    for (uint32_t i = t; i < kClassWords * 64; i += 256) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+  }
   uint32_t L = 0, pre = 0;
+  uint32_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // HIST false: this lane's blocks' counters
   if (valid) {
     const uint8_t *cell = bytes + r * (int64_t)width;
     uint32_t *pl = planes + r * (int64_t)nwc * 8;
@@ -490,14 +533,16 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
 #pragma unroll
       for (int d = 0; d < 8; d++) {
         L += nonzero_bytes(w8[d]);
+        if (HIST) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const uint32_t c = (w8[d] >> (8 * e)) & 0xffu;
-          const uint32_t inc = c ? 1u << ((c & 3u) * 8u) : 0u;
-          atomicAdd(&s_h[byte_class(c) >> 2][rl], inc);
+          for (int e = 0; e < 4; e++) {
+            const uint32_t c = (w8[d] >> (8 * e)) & 0xffu;
+            const uint32_t inc = c ? 1u << ((c & 3u) * 8u) : 0u;
+            atomicAdd(&s_h[byte_class(c) >> 2][rl], inc);
+          }
         }
       }
-      if (blk < nwc) {
+      if (blk < nwc || !HIST) {
         uint64_t X[4];
 #pragma unroll
         for (int g = 0; g < 4; g++) X[g] = transpose8(((uint64_t)w8[2 * g + 1] << 32) | w8[2 * g]);
@@ -508,9 +553,12 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
           P[k] = (uint32_t)((X[0] >> sh) & 0xffu) | ((uint32_t)((X[1] >> sh) & 0xffu) << 8) |
                  ((uint32_t)((X[2] >> sh) & 0xffu) << 16) | ((uint32_t)((X[3] >> sh) & 0xffu) << 24);
         }
-        uint4 *d4 = reinterpret_cast<uint4 *>(pl + 8 * blk);
-        d4[0] = make_uint4(P[0], P[1], P[2], P[3]);
-        d4[1] = make_uint4(P[4], P[5], P[6], P[7]);
+        if (!HIST) count_classes(P, h);
+        if (blk < nwc) {
+          uint4 *d4 = reinterpret_cast<uint4 *>(pl + 8 * blk);
+          d4[0] = make_uint4(P[0], P[1], P[2], P[3]);
+          d4[1] = make_uint4(P[4], P[5], P[6], P[7]);
+        }
       }
     }
   }
@@ -519,10 +567,14 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
   for (int m = 1; m <= 2; m <<= 1) {
     L += (uint32_t)__shfl_xor((int)L, m);
     pre |= (uint32_t)__shfl_xor((int)pre, m);  // only lane 0 of the row holds it
+    if (!HIST) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) h[j] += (uint32_t)__shfl_xor((int)h[j], m);
+    }
   }
-  __syncthreads();
+  if (HIST) __syncthreads();
   if (!valid) return;
-  if (hist) {  // lane q writes nibble words 4q .. 4q + 3 of the refine row (classes
+  if (HIST) {  // lane q writes nibble words 4q .. 4q + 3 of the refine row (classes
                // 32q .. 32q + 31; lane 0's are never counted, so it writes the prefix and length)
     uint32_t o[8];
 #pragma unroll
@@ -537,7 +589,7 @@ __global__ __launch_bounds__(256) void k_build_col(const uint8_t *__restrict__ b
     SigRow sg;
 #pragma unroll
     for (int j = 0; j < 8; j++)  // classes 4j .. 4j+3 of 32 = four of the 128 each
-      sg.h[j] = s_h[j][rl] + s_h[j + 8][rl] + s_h[j + 16][rl] + s_h[j + 24][rl];
+      sg.h[j] = HIST ? s_h[j][rl] + s_h[j + 8][rl] + s_h[j + 16][rl] + s_h[j + 24][rl] : h[j];
     sg.pre = pre;
     sg.len = L | (L > 255u ? kLongFlag : 0u);
     sg.rlen = (L >= 1u && L <= 255u) ? 1.0f / (float)L : INFINITY;
@@ -2459,8 +2511,11 @@ void launch_build_col(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint
                       uint16_t *len, SigRow *sig, uint32_t *hist, uint32_t *planes,
                       hipStream_t stream) {
   if (n_rows <= 0) return;
-  k_build_col<<<(uint32_t)((n_rows + 63) / 64), 256, 0, stream>>>(bytes, width, n_rows, nwc, len,
-                                                                  sig, hist, planes);
+  const uint32_t groups = (uint32_t)((n_rows + 63) / 64);
+  if (hist)
+    k_build_col<true><<<groups, 256, 0, stream>>>(bytes, width, n_rows, nwc, len, sig, hist, planes);
+  else
+    k_build_col<false><<<groups, 256, 0, stream>>>(bytes, width, n_rows, nwc, len, sig, hist, planes);
 }
 
 void launch_iota(uint32_t *out, uint32_t n, hipStream_t stream) {
