@@ -981,7 +981,7 @@ __device__ __forceinline__ bool last_group(uint32_t *done) {
 // the flag accumulator and the done count are cleared for the next layout
 // built on the same stream.
 __device__ void lsmall_scan(const LayoutSmall &a) {
-  __shared__ uint32_t s_part[1024];
+  __shared__ uint32_t s_part[32];  // wave totals, then their inclusive scan
   __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
   const uint32_t t = threadIdx.x;
   const uint32_t nb = a.n_codes * 256u;
@@ -1006,15 +1006,28 @@ __device__ void lsmall_scan(const LayoutSmall &a) {
     uint32_t sum = 0;
 #pragma unroll
     for (int q = 0; q < 16; q++) sum += v[q];
-    s_part[t] = sum;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {  // inclusive scan of the partial sums
-      const uint32_t x = t >= d ? s_part[t - d] : 0u;
-      __syncthreads();
-      s_part[t] += x;
-      __syncthreads();
+    // exclusive scan of the partial sums: within each wave by shuffles, then
+    // over the 16 wave totals
+    const uint32_t lane = lane_id(), wv = t >> 6;
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)inc, d);
+      if (lane >= (uint32_t)d) inc += y;
     }
-    uint32_t run = t ? s_part[t - 1] : 0u;
+    if (lane == 63) s_part[wv] = inc;
+    __syncthreads();
+    if (t < 64) {
+      uint32_t w = t < 16 ? s_part[t] : 0u;
+#pragma unroll
+      for (int d = 1; d < 16; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)w, d);
+        if (t >= (uint32_t)d) w += y;
+      }
+      if (t < 16) s_part[16 + t] = w;  // inclusive over waves
+    }
+    __syncthreads();
+    uint32_t run = inc - sum + (wv ? s_part[16 + wv - 1] : 0u);
     if (o == 0 && mine && (t & 15u) == 0) s_code_start[t >> 4] = run;
     if (mine) {
       uint4 *o4 = reinterpret_cast<uint4 *>(a.off + o * nb + 16u * t);
