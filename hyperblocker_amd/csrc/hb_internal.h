@@ -46,6 +46,7 @@ struct __attribute__((aligned(16))) SigRow {
   uint32_t pad;
 };
 static_assert(sizeof(SigRow) == 48, "SigRow layout");
+static_assert(__builtin_offsetof(SigRow, pre) == 32, "SigRow: pre opens the third 16-byte word");
 
 constexpr uint32_t kLongFlag = 0x80000000u;
 

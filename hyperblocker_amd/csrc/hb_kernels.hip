@@ -974,9 +974,11 @@ __device__ __forceinline__ bool last_group(uint32_t *done) {
   return last;
 }
 
-// Run by the counting launch's last work-group (1024 threads): exclusive
-// scans of the two count arrays (16 buckets per thread, so thread 16 c
-// starts code c), the runs of the join order (one per code with rows;
+// Run by the counting launch's last work-group (1024 threads): one
+// exclusive scan over both orders' counters, which lie one after the other
+// (32 buckets per thread, so thread 8 c starts code c of the join order;
+// the join order counts every row, so the first-byte order's offsets are
+// the scan less n), the runs of the join order (one per code with rows;
 // codes < 64: one wave's ballot), the layout's bad flag; then the counters,
 // the flag accumulator and the done count are cleared for the next layout
 // built on the same stream.
@@ -985,70 +987,69 @@ __device__ void lsmall_scan(const LayoutSmall &a) {
   __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
   const uint32_t t = threadIdx.x;
   const uint32_t nb = a.n_codes * 256u;
-  const uint32_t orders = a.sig ? 2u : 1u;
-  const bool mine = 16u * t < nb;
-  for (uint32_t o = 0; o < orders; o++) {
-    uint32_t v[16];
-    if (mine) {
-      const uint4 *h4 = reinterpret_cast<const uint4 *>(a.hist + o * nb + 16u * t);
+  const uint32_t total = (a.sig ? 2u : 1u) * nb;  // a multiple of 256
+  const uint32_t first = 32u * t;
+  const bool mine = first < total;
+  uint32_t v[32];
+  if (mine) {
+    const uint4 *h4 = reinterpret_cast<const uint4 *>(a.hist + first);
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const uint4 x = h4[q];
-        v[4 * q] = x.x;
-        v[4 * q + 1] = x.y;
-        v[4 * q + 2] = x.z;
-        v[4 * q + 3] = x.w;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 16; q++) v[q] = 0;
+    for (int q = 0; q < 8; q++) {
+      const uint4 x = h4[q];
+      v[4 * q] = x.x;
+      v[4 * q + 1] = x.y;
+      v[4 * q + 2] = x.z;
+      v[4 * q + 3] = x.w;
     }
-    uint32_t sum = 0;
+  } else {
 #pragma unroll
-    for (int q = 0; q < 16; q++) sum += v[q];
-    // exclusive scan of the partial sums: within each wave by shuffles, then
-    // over the 16 wave totals
-    const uint32_t lane = lane_id(), wv = t >> 6;
-    uint32_t inc = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = (uint32_t)__shfl_up((int)inc, d);
-      if (lane >= (uint32_t)d) inc += y;
-    }
-    if (lane == 63) s_part[wv] = inc;
-    __syncthreads();
-    if (t < 64) {
-      uint32_t w = t < 16 ? s_part[t] : 0u;
-#pragma unroll
-      for (int d = 1; d < 16; d <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)w, d);
-        if (t >= (uint32_t)d) w += y;
-      }
-      if (t < 16) s_part[16 + t] = w;  // inclusive over waves
-    }
-    __syncthreads();
-    uint32_t run = inc - sum + (wv ? s_part[16 + wv - 1] : 0u);
-    if (o == 0 && mine && (t & 15u) == 0) s_code_start[t >> 4] = run;
-    if (mine) {
-      uint4 *o4 = reinterpret_cast<uint4 *>(a.off + o * nb + 16u * t);
-      uint4 *z4 = reinterpret_cast<uint4 *>(a.hist + o * nb + 16u * t);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        uint4 x;
-        x.x = run;
-        run += v[4 * q];
-        x.y = run;
-        run += v[4 * q + 1];
-        x.z = run;
-        run += v[4 * q + 2];
-        x.w = run;
-        run += v[4 * q + 3];
-        o4[q] = x;
-        z4[q] = make_uint4(0u, 0u, 0u, 0u);
-      }
-    }
-    __syncthreads();
+    for (int q = 0; q < 32; q++) v[q] = 0;
   }
+  uint32_t sum = 0;
+#pragma unroll
+  for (int q = 0; q < 32; q++) sum += v[q];
+  // exclusive scan of the partial sums: within each wave by shuffles, then
+  // over the 16 wave totals
+  const uint32_t lane = lane_id(), wv = t >> 6;
+  uint32_t inc = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)inc, d);
+    if (lane >= (uint32_t)d) inc += y;
+  }
+  if (lane == 63) s_part[wv] = inc;
+  __syncthreads();
+  if (t < 64) {
+    uint32_t w = t < 16 ? s_part[t] : 0u;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)w, d);
+      if (t >= (uint32_t)d) w += y;
+    }
+    if (t < 16) s_part[16 + t] = w;  // inclusive over waves
+  }
+  __syncthreads();
+  uint32_t run = inc - sum + (wv ? s_part[16 + wv - 1] : 0u) - (first >= nb ? a.n : 0u);
+  if (first < nb && (t & 7u) == 0) s_code_start[t >> 3] = run;
+  if (mine) {
+    uint4 *o4 = reinterpret_cast<uint4 *>(a.off + first);
+    uint4 *z4 = reinterpret_cast<uint4 *>(a.hist + first);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      uint4 x;
+      x.x = run;
+      run += v[4 * q];
+      x.y = run;
+      run += v[4 * q + 1];
+      x.z = run;
+      run += v[4 * q + 2];
+      x.w = run;
+      run += v[4 * q + 3];
+      o4[q] = x;
+      z4[q] = make_uint4(0u, 0u, 0u, 0u);
+    }
+  }
+  __syncthreads();
   if (t < 64) {
     const uint32_t c = t;
     const uint32_t lo = c < a.n_codes ? s_code_start[c] : a.n;
@@ -1126,12 +1127,22 @@ __global__ void k_lsmall_scatter(LayoutSmall a) {
   const uint32_t pos = a.off[base + l] + a.rank[i];
   a.perm[pos] = i;
   if (a.sig) {
-    const SigRow sg = a.sig[i];
-    a.jsig[pos] = sg;
-    const uint32_t fb = sg.pre & 0xffu;
-    const uint32_t pos2 = a.off[a.n_codes * 256u + base + fb] + a.rank[a.n + i];
+    // The row as three 16-byte words (word 8, the first 4 bytes, opens the
+    // third), in clang vector types: a SigRow or uint4 temporary copied to
+    // two places is given a per-thread copy in LDS by the compiler.
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(a.sig + i);
+    const u32x4 w0 = src[0], w1 = src[1], w2 = src[2];
+    u32x4 *d = reinterpret_cast<u32x4 *>(a.jsig + pos);
+    d[0] = w0;
+    d[1] = w1;
+    d[2] = w2;
+    const uint32_t pos2 = a.off[a.n_codes * 256u + base + (w2.x & 0xffu)] + a.rank[a.n + i];
     a.perm_fb[pos2] = i;
-    a.jsig_fb[pos2] = sg;
+    d = reinterpret_cast<u32x4 *>(a.jsig_fb + pos2);
+    d[0] = w0;
+    d[1] = w1;
+    d[2] = w2;
   }
 }
 
