@@ -303,9 +303,16 @@ inline size_t unpack_lds_bytes(size_t span, size_t cells) {
   const size_t rows_room = ((span + 31) & ~size_t(31)) + 32;
   return rows_room + ((span + 31) / 32 + 1) * 4 + cells * 2;
 }
+// The column parameters of a table of at most kUnpackArgCols columns travel
+// as kernel arguments (no copy ahead of the kernel); those of a wider table
+// are read from device memory (cols_dev).
+constexpr uint32_t kUnpackArgCols = 24;
+struct UnpackColumns {
+  UnpackColumn c[kUnpackArgCols];
+};
 void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,
-                        const UnpackColumn *cols, uint32_t n_cols, uint32_t rows_per_group,
-                        int n_cu, hipStream_t stream);
+                        const UnpackColumns &cols, const UnpackColumn *cols_dev, uint32_t n_cols,
+                        uint32_t rows_per_group, int n_cu, hipStream_t stream);
 void launch_build_sig(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint16_t *len,
                       SigRow *sig, hipStream_t stream);
 void launch_build_planes(const uint8_t *bytes, uint32_t width, int64_t n_rows, uint32_t nwc,

@@ -748,10 +748,13 @@ __device__ __forceinline__ void dict_verify_row(const uint8_t *__restrict__ cell
   }
 }
 
-__global__ void k_dict_verify(const uint8_t *__restrict__ cells, uint32_t width,
-                              uint32_t fixed_len, uint32_t n, DictDev d,
-                              uint32_t *__restrict__ code, uint32_t *__restrict__ fb_rows,
-                              unsigned long long *fb_count, unsigned long long *status) {
+__global__ __launch_bounds__(1024) void k_dict_verify(const uint8_t *__restrict__ cells,
+                                                     uint32_t width, uint32_t fixed_len,
+                                                     uint32_t n, DictDev d,
+                                                     uint32_t *__restrict__ code,
+                                                     uint32_t *__restrict__ fb_rows,
+                                                     unsigned long long *fb_count,
+                                                     unsigned long long *status) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n) dict_verify_row(cells, width, fixed_len, r, d, code, fb_rows, fb_count);
   // The last work-group to finish reports (rows left to the host, codes
@@ -805,8 +808,10 @@ __global__ void k_mask_keys(KeyColumns cols, uint32_t n, uint32_t *__restrict__ 
 template <bool ALIGNED16>
 __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__ rows,
                                                     uint64_t row_bytes, uint32_t n_rows,
-                                                    const UnpackColumn *__restrict__ cols,
+                                                    const UnpackColumns inl,
+                                                    const UnpackColumn *__restrict__ cols_dev,
                                                     uint32_t n_cols, uint32_t rows_per_group) {
+  const UnpackColumn *__restrict__ cols = n_cols <= kUnpackArgCols ? inl.c : cols_dev;
   extern __shared__ uint4 s_mem4[];
   uint8_t *s_rows = reinterpret_cast<uint8_t *>(s_mem4);
   const uint32_t *s_rows32 = reinterpret_cast<const uint32_t *>(s_mem4);
@@ -2577,8 +2582,11 @@ void launch_dict_code(const uint8_t *cells, uint32_t width, uint32_t fixed_len, 
                       hipStream_t stream) {
   const uint32_t blocks = std::max<uint32_t>(1, (n + 255) / 256);
   k_dict_insert<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code, fb_count);
-  k_dict_verify<<<blocks, 256, 0, stream>>>(cells, width, fixed_len, n, d, code, fb_rows,
-                                            fb_count, status);
+  // Work-groups of 1024: each group's fence before its done count writes
+  // back its L2, so fewer groups end the launch sooner.
+  const uint32_t vblocks = std::max<uint32_t>(1, (n + 1023) / 1024);
+  k_dict_verify<<<vblocks, 1024, 0, stream>>>(cells, width, fixed_len, n, d, code, fb_rows,
+                                              fb_count, status);
 }
 
 void launch_compose_keys(KeyColumns cols, uint32_t n, uint8_t *cells, uint32_t width,
@@ -2593,19 +2601,19 @@ void launch_mask_keys(KeyColumns cols, uint32_t n, uint32_t *code, hipStream_t s
 }
 
 void launch_unpack_rows(const uint8_t *rows, uint64_t row_bytes, uint32_t n_rows,
-                        const UnpackColumn *cols, uint32_t n_cols, uint32_t rows_per_group,
-                        int n_cu, hipStream_t stream) {
+                        const UnpackColumns &cols, const UnpackColumn *cols_dev, uint32_t n_cols,
+                        uint32_t rows_per_group, int n_cu, hipStream_t stream) {
   if (!n_rows || !n_cols) return;
   const uint32_t span = (uint32_t)(rows_per_group * row_bytes);
   const size_t lds = unpack_lds_bytes(span, (size_t)rows_per_group * n_cols);
   const uint32_t n_blocks = (n_rows + rows_per_group - 1) / rows_per_group;
   const uint32_t grid = std::min<uint32_t>(n_blocks, (uint32_t)n_cu * 8);
   if ((((uintptr_t)rows) & 15u) == 0 && (row_bytes & 15u) == 0)
-    k_unpack_rows<true><<<grid, 256, lds, stream>>>(rows, row_bytes, n_rows, cols, n_cols,
-                                                     rows_per_group);
+    k_unpack_rows<true><<<grid, 256, lds, stream>>>(rows, row_bytes, n_rows, cols, cols_dev,
+                                                     n_cols, rows_per_group);
   else
-    k_unpack_rows<false><<<grid, 256, lds, stream>>>(rows, row_bytes, n_rows, cols, n_cols,
-                                                      rows_per_group);
+    k_unpack_rows<false><<<grid, 256, lds, stream>>>(rows, row_bytes, n_rows, cols, cols_dev,
+                                                      n_cols, rows_per_group);
 }
 
 void launch_layout_small(const LayoutSmall &a, hipStream_t stream) {

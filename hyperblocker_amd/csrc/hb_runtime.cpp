@@ -2893,12 +2893,17 @@ hb_status unpack_table(hb_ctx *ctx, const hb_serialized_table *s, const uint8_t 
   t->n_rows = (int64_t)s->n_rows;
   t->n_cols = (int32_t)s->n_cols;
   t->cols.resize(s->n_cols);
-  // column parameters: staged in pinned memory (the table's), copied to the
-  // device on the stream ahead of the kernel
-  hb::UnpackColumn *h_cols = nullptr;
+  // column parameters: kernel arguments, or for a table of more than
+  // kUnpackArgCols columns staged in pinned memory (the table's) and copied
+  // to the device on the stream ahead of the kernel
+  const bool in_args = s->n_cols <= hb::kUnpackArgCols;
+  hb::UnpackColumns args{};
+  hb::UnpackColumn *h_cols = args.c;
   void *d_cols = nullptr;
-  HB_TRY(t->host_alloc(s->n_cols * sizeof(hb::UnpackColumn), (void **)&h_cols));
-  HB_TRY(t->dev_alloc(s->n_cols * sizeof(hb::UnpackColumn), &d_cols));
+  if (!in_args) {
+    HB_TRY(t->host_alloc(s->n_cols * sizeof(hb::UnpackColumn), (void **)&h_cols));
+    HB_TRY(t->dev_alloc(s->n_cols * sizeof(hb::UnpackColumn), &d_cols));
+  }
   for (size_t c = 0; c < s->n_cols; c++) {
     Column &col = t->cols[c];
     col.width = (uint32_t)((s->col_size[c] + 1 + 15) & ~size_t(15));
@@ -2907,14 +2912,15 @@ hb_status unpack_table(hb_ctx *ctx, const hb_serialized_table *s, const uint8_t 
     h_cols[c] = hb::UnpackColumn{col.d_bytes, (uint64_t)s->col_offset[c], (uint32_t)s->col_size[c],
                                  col.width};
   }
-  HIP_OK(hipMemcpyAsync(d_cols, h_cols, s->n_cols * sizeof(hb::UnpackColumn),
-                        hipMemcpyHostToDevice, stream));
+  if (!in_args)
+    HIP_OK(hipMemcpyAsync(d_cols, h_cols, s->n_cols * sizeof(hb::UnpackColumn),
+                          hipMemcpyHostToDevice, stream));
   // rows staged per work-group: about 16 KiB of them, within the LDS limit
   const size_t rpg = std::min<size_t>(256, std::max<size_t>(1, (16u << 10) / s->aligned_tuple_size));
   if (hb::unpack_lds_bytes(s->aligned_tuple_size * rpg, rpg * s->n_cols) > hb::kUnpackLds)
     return fail(HB_ERR_CAPACITY, "serialized table: rows of %zu bytes exceed the %u-byte "
                 "staging limit", s->aligned_tuple_size, hb::kUnpackLds);
-  hb::launch_unpack_rows(d_rows, s->aligned_tuple_size, (uint32_t)s->n_rows,
+  hb::launch_unpack_rows(d_rows, s->aligned_tuple_size, (uint32_t)s->n_rows, args,
                          (const hb::UnpackColumn *)d_cols, (uint32_t)s->n_cols, (uint32_t)rpg,
                          ctx->n_cu, stream);
   HIP_OK(hipGetLastError());

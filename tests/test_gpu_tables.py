@@ -114,6 +114,41 @@ def test_unpack_cells_ending_at_nul_or_col_size(ctx, junk):
         dev.free()
 
 
+@pytest.mark.parametrize("n_extra", [20, 26])
+def test_unpack_narrow_and_wide_tables(ctx, n_extra):
+    """Tables of 24 columns (column parameters passed as kernel arguments)
+    and of 30 (read from device memory), the extra columns of assorted
+    sizes between the used ones: both row-major entry points give the
+    oracle's pairs and audit entries."""
+    rng = random.Random(11 + n_extra)
+    alpha = b"abcd "
+    rows = []
+    for i in range(200):
+        extra = tuple(bytes(rng.choice(alpha) for _ in range(rng.randint(0, 6)))
+                      for _ in range(n_extra))
+        rows.append((str(i).encode(), bytes(rng.choice(alpha) for _ in range(rng.randint(1, 10))),
+                     bytes(rng.choice(alpha) for _ in range(rng.randint(0, 6))),
+                     str(rng.randrange(4)).encode()) + extra)
+    cols = H.columns_from_rows(rows)
+    sizes = [4, 11, 7, 2] + [rng.randint(7, 12) for _ in range(n_extra)]
+    raw = _RawTable(cols, sizes)
+    assert raw.n_cols == 4 + n_extra
+    plan = hb.HostPlan.from_yaml([datagen.rule_yaml([3], [(1, 0.7), (2, 0.5)])])
+    want, want_audit, _ = H.run_oracle(plan, cols, cols)
+    dp = ctx.plan(plan)
+    dev = hb.DeviceSerializedTable(raw)
+    try:
+        for how in ("upload", "device"):
+            t = ctx.upload_serialized(raw) if how == "upload" else ctx.from_device(dev)
+            got, audit, _ = ctx.block(dp, t, t)
+            t.release()
+            H.assert_same_pairs(got, want, how)
+            H.assert_same_audit(H.sort_audit(audit), want_audit, how)
+    finally:
+        dp.release()
+        dev.free()
+
+
 @pytest.mark.parametrize("n_pick", [0, 1, 217])
 def test_serialized_partition_rows(ctx, n_pick):
     """hb_table_upload_serialized_rows: rows listed out of order (a
