@@ -379,6 +379,16 @@ void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
                         uint32_t *run_off, uint32_t *n_runs, uint32_t *n_runs_host,
                         hipStream_t stream);
 // Groups of every JR of a batch (one per left run), with their work.
+// The first launch of a batch whose n + 1 groups fit one work-group of
+// 1024 (not captured into a graph): copies desc_bytes of descriptors from
+// pinned host memory (hdesc_dev: its device address; both 16-byte aligned,
+// rounded up to 16 bytes), clears cnt_words counters and set_words dedupe
+// slots, joins and scans the work items, in place of the copy, the fills,
+// launch_batch_join and the scan.
+constexpr uint32_t kPrologueGroups = 1023;
+void launch_batch_prologue(BatchDev b, GroupArrays g, const void *hdesc_dev, void *ddesc,
+                           size_t desc_bytes, unsigned long long *cnt, size_t cnt_words,
+                           uint64_t set_words, hipStream_t stream);
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream);
 
 // Filter variants: JRs without a similarity predicate; with one and an

@@ -1294,6 +1294,44 @@ __global__ void k_batch_join(const BatchDev b, const GroupArrays g) {
   job_add(b, kJobJoined, jp != 0, job, jp);
 }
 
+// A batch whose groups (and end entry) fit one work-group of 1024 starts
+// with this one launch in place of four: its descriptors copied from the
+// pinned host buffer (read over the bus, as the copy would read them), its
+// counters and dedupe sets cleared, the join (one thread per group, as
+// k_batch_join) and the exclusive scan of the groups' work items.  The
+// work-group's barriers order each phase's global writes before the next
+// phase's reads.
+__global__ __launch_bounds__(1024) void k_batch_prologue(const BatchDev b, const GroupArrays g,
+                                                        const uint4 *__restrict__ hdesc,
+                                                        uint4 *__restrict__ ddesc, uint32_t desc16,
+                                                        unsigned long long *__restrict__ cnt,
+                                                        uint32_t cnt_words, uint64_t set_words) {
+  __shared__ unsigned long long s_wave[16];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < desc16; i += 1024) ddesc[i] = hdesc[i];
+  for (uint32_t i = t; i < cnt_words; i += 1024) cnt[i] = 0ull;
+  for (uint64_t i = t; i < set_words; i += 1024) b.sets.slots[i] = ~0ull;
+  __syncthreads();
+  uint32_t job = 0;
+  unsigned long long jp = 0;
+  if (t <= g.n) join_group(b, g, t, job, jp);
+  job_add(b, kJobJoined, jp != 0, job, jp);
+  __syncthreads();
+  const uint32_t lane = lane_id(), wave = t >> 6;
+  const unsigned long long v = t <= g.n ? g.items[t] : 0ull;
+  unsigned long long inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long y = __shfl_up(inc, d);
+    if (lane >= (uint32_t)d) inc += y;
+  }
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  unsigned long long base = 0;
+  for (uint32_t w = 0; w < wave; w++) base += s_wave[w];
+  if (t <= g.n) g.item_off[t] = base + inc - v;
+}
+
 // Largest g in [0, n) with off[g] <= x (off non-decreasing, off[0] = 0).
 __device__ __forceinline__ uint32_t find_group(const uint64_t *off, uint32_t n, uint64_t x) {
   uint32_t lo = 0, hi = n;
@@ -2667,6 +2705,14 @@ void launch_layout_runs(const uint64_t *sorted_keys, const uint32_t *flags,
   if (!n) return;
   hipLaunchKernelGGL(k_layout_runs, dim3(grid_for(n, 256)), dim3(256), 0, stream, sorted_keys,
                      flags, run_id, n, run_code, run_off, n_runs, n_runs_host);
+}
+
+void launch_batch_prologue(BatchDev b, GroupArrays g, const void *hdesc_dev, void *ddesc,
+                           size_t desc_bytes, unsigned long long *cnt, size_t cnt_words,
+                           uint64_t set_words, hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch_prologue, dim3(1), dim3(1024), 0, stream, b, g,
+                     (const uint4 *)hdesc_dev, (uint4 *)ddesc, (uint32_t)((desc_bytes + 15) / 16),
+                     cnt, (uint32_t)cnt_words, set_words);
 }
 
 void launch_batch_join(BatchDev b, GroupArrays g, hipStream_t stream) {

@@ -1863,7 +1863,8 @@ hb_status enqueue_batch(Batch *b) {
   const size_t desc_bytes = n_jr * sizeof(hb::JRDesc) + (n_jr + 1) * sizeof(uint32_t) +
                             std::max<size_t>(sims.size(), 1) * sizeof(hb::SimDesc) +
                             2 * (size_t)n_jobs * sizeof(unsigned long long) + 128;
-  HB_TRY(batch_alloc_host(b, desc_bytes, &hp, nullptr));
+  void *hdesc_dev = nullptr;
+  HB_TRY(batch_alloc_host(b, desc_bytes, &hp, &hdesc_dev));
   HB_TRY(batch_alloc(b, desc_bytes, &p));
   char *hdesc = (char *)hp, *ddesc = (char *)p;
   size_t o = 0;
@@ -1977,11 +1978,13 @@ hb_status enqueue_batch(Batch *b) {
   } else {
     HIP_OK(hipEventRecord(b->ev_start, st));
   }
-  HIP_OK(hipMemcpyAsync(ddesc, hdesc, o, hipMemcpyHostToDevice, st));
-  hb::launch_fill_u64(d_cnt, b->cnt_words, 0ull, st);
-  if (set_total) hb::launch_fill_u64(bd.sets.slots, set_total, ~0ull, st);
-  hb::launch_batch_join(bd, g, st);
-  {
+  if (!cap && n_groups <= hb::kPrologueGroups) {  // one launch
+    hb::launch_batch_prologue(bd, g, hdesc_dev, ddesc, o, d_cnt, b->cnt_words, set_total, st);
+  } else {
+    HIP_OK(hipMemcpyAsync(ddesc, hdesc, o, hipMemcpyHostToDevice, st));
+    hb::launch_fill_u64(d_cnt, b->cnt_words, 0ull, st);
+    if (set_total) hb::launch_fill_u64(bd.sets.slots, set_total, ~0ull, st);
+    hb::launch_batch_join(bd, g, st);
     size_t t = tmp_bytes;
     HIP_OK(hb::exscan_u64(tmp, t, g.items, g.item_off, n_groups + 1, st));
   }
