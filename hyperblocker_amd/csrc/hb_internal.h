@@ -347,22 +347,30 @@ void launch_gather_sig(const uint32_t *perm, uint32_t n, const SigRow *sig, SigR
 // codes, by counting sorts: join order (perm, jsig) by (code, length), the
 // first-byte order (perm_fb, jsig_fb; with a similarity column) by (code,
 // first byte), the runs of equal code and their count, in two launches.
-// hist (2 * kLayoutMaxCodes * 256 counters), *bad and *done must be zero
-// and are left zero (layouts built one after another on one stream can
-// share them); the layout's own flag *bad_out is written (1 when a code was
-// not below n_codes).  rank holds 2 * n entries, off 2 * n_codes * 256.
-// Within a bucket rows come in no particular order.
+// Layouts built one after another on one stream share the counters:
+//   - up to kLayoutLdsCodes codes: hist (2 * kLayoutLdsCodes * 256
+//     counters) must be zero; it is left holding this layout's counts, and
+//     hist_clear (clear_words counters: the other of two such buffers, used
+//     by the previous layout of this kind) is cleared, so consecutive
+//     layouts alternate between the two;
+//   - more codes: hist (2 * kLayoutMaxCodes * 256), *bad and *done must be
+//     zero and are left zero; off holds 2 * n_codes * 256 entries.
+// The layout's own flag *bad_out is written (1 when a code was not below
+// n_codes).  rank holds 2 * n entries.  Within a bucket rows come in no
+// particular order.
 constexpr uint32_t kLayoutMaxCodes = 64;
-constexpr uint32_t kLayoutLdsCodes = 28;  // counts taken in LDS (56 KiB) up to this many codes
+constexpr uint32_t kLayoutLdsCodes = 28;  // counts and offsets in LDS (56 KiB) up to this many codes
 struct LayoutSmall {
   const uint32_t *code;  // null: a rule without equalities (one code, 0)
-  uint32_t *bad;         // set when a code is not below n_codes (the row then counts as code 0)
-  uint32_t *bad_out;     // the layout's flag: *bad as the counting launch ends
-  uint32_t *done;        // work-groups of the counting launch done
+  uint32_t *bad;         // more codes: set when a code is not below n_codes (row counts as code 0)
+  uint32_t *bad_out;     // the layout's flag
+  uint32_t *done;        // more codes: work-groups of the counting launch done
   const uint16_t *len;   // null: no similarity column
   const SigRow *sig;     // null: no similarity column
   uint32_t n, n_codes;
   uint32_t *hist, *rank, *off;
+  uint32_t *hist_clear;  // up to kLayoutLdsCodes codes: counters cleared by this build
+  uint32_t clear_words;
   uint32_t *perm, *perm_fb;
   SigRow *jsig, *jsig_fb;
   uint32_t *run_code, *run_off, *n_runs;

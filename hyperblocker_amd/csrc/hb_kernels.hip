@@ -959,7 +959,7 @@ __global__ void k_layout_keys(const uint32_t *__restrict__ code, const uint16_t 
 __device__ __forceinline__ uint32_t lsmall_code(const LayoutSmall &a, uint32_t i, uint32_t *bad) {
   const uint32_t c = a.code ? a.code[i] : 0u;
   if (c < a.n_codes) return c;
-  *bad = 1u;
+  if (bad) *bad = 1u;
   return 0u;
 }
 
@@ -979,17 +979,16 @@ __device__ __forceinline__ bool last_group(uint32_t *done) {
   return last;
 }
 
-// Run by the counting launch's last work-group (1024 threads): one
-// exclusive scan over both orders' counters, which lie one after the other
-// (32 buckets per thread, so thread 8 c starts code c of the join order;
-// the join order counts every row, so the first-byte order's offsets are
-// the scan less n), the runs of the join order (one per code with rows;
-// codes < 64: one wave's ballot), the layout's bad flag; then the counters,
-// the flag accumulator and the done count are cleared for the next layout
-// built on the same stream.
-__device__ void lsmall_scan(const LayoutSmall &a) {
+// Exclusive scan over both orders' counters, which lie one after the other
+// (32 buckets per thread of a 1024-thread work-group, so thread 8 c starts
+// code c of the join order; the join order counts every row, so the
+// first-byte order's offsets are the scan less n), into dst (global memory
+// or LDS); the start of each code's rows in the join order into
+// s_code_start; with `clear`, the counters are left zero.  Ends with a
+// barrier.
+__device__ void lsmall_offsets(const LayoutSmall &a, const uint32_t *hist, uint32_t *dst,
+                               bool clear, uint32_t *s_code_start) {
   __shared__ uint32_t s_part[32];  // wave totals, then their inclusive scan
-  __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
   const uint32_t t = threadIdx.x;
   const uint32_t nb = a.n_codes * 256u;
   const uint32_t total = (a.sig ? 2u : 1u) * nb;  // a multiple of 256
@@ -997,7 +996,7 @@ __device__ void lsmall_scan(const LayoutSmall &a) {
   const bool mine = first < total;
   uint32_t v[32];
   if (mine) {
-    const uint4 *h4 = reinterpret_cast<const uint4 *>(a.hist + first);
+    const uint4 *h4 = reinterpret_cast<const uint4 *>(hist + first);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const uint4 x = h4[q];
@@ -1037,8 +1036,8 @@ __device__ void lsmall_scan(const LayoutSmall &a) {
   uint32_t run = inc - sum + (wv ? s_part[16 + wv - 1] : 0u) - (first >= nb ? a.n : 0u);
   if (first < nb && (t & 7u) == 0) s_code_start[t >> 3] = run;
   if (mine) {
-    uint4 *o4 = reinterpret_cast<uint4 *>(a.off + first);
-    uint4 *z4 = reinterpret_cast<uint4 *>(a.hist + first);
+    uint4 *o4 = reinterpret_cast<uint4 *>(dst + first);
+    uint4 *z4 = reinterpret_cast<uint4 *>(const_cast<uint32_t *>(hist) + first);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       uint4 x;
@@ -1051,30 +1050,47 @@ __device__ void lsmall_scan(const LayoutSmall &a) {
       x.w = run;
       run += v[4 * q + 3];
       o4[q] = x;
-      z4[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (clear) z4[q] = make_uint4(0u, 0u, 0u, 0u);
     }
   }
   __syncthreads();
-  if (t < 64) {
-    const uint32_t c = t;
-    const uint32_t lo = c < a.n_codes ? s_code_start[c] : a.n;
-    const uint32_t hi = c + 1 < a.n_codes ? s_code_start[c + 1] : a.n;
-    const bool has = c < a.n_codes && hi > lo;
-    const uint64_t m = __ballot(has);
-    const uint32_t id = rank_in(m);
-    if (has) {
-      a.run_code[id] = c;
-      a.run_off[id] = lo;
-    }
-    if (t == 0) {
-      const uint32_t nr = (uint32_t)__popcll(m);
-      a.run_off[nr] = a.n;
-      *a.n_runs = nr;
-      *a.n_runs_host = nr;
-      *a.bad_out = *a.bad;
-      *a.bad = 0u;
-      *a.done = 0u;
-    }
+}
+
+// The runs of the join order from the code starts (one per code with rows;
+// codes < 64: one wave's ballot) and their count, by the first wave.
+__device__ void lsmall_runs(const LayoutSmall &a, const uint32_t *s_code_start) {
+  const uint32_t t = threadIdx.x;
+  if (t >= 64) return;
+  const uint32_t c = t;
+  const uint32_t lo = c < a.n_codes ? s_code_start[c] : a.n;
+  const uint32_t hi = c + 1 < a.n_codes ? s_code_start[c + 1] : a.n;
+  const bool has = c < a.n_codes && hi > lo;
+  const uint64_t m = __ballot(has);
+  const uint32_t id = rank_in(m);
+  if (has) {
+    a.run_code[id] = c;
+    a.run_off[id] = lo;
+  }
+  if (t == 0) {
+    const uint32_t nr = (uint32_t)__popcll(m);
+    a.run_off[nr] = a.n;
+    *a.n_runs = nr;
+    *a.n_runs_host = nr;
+  }
+}
+
+// Run by the global-atomic counting launch's last work-group (1024
+// threads): the offsets into a.off, the runs, the layout's bad flag; the
+// counters, the flag accumulator and the done count are left zero for the
+// next layout built on the same stream.
+__device__ void lsmall_scan(const LayoutSmall &a) {
+  __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
+  lsmall_offsets(a, a.hist, a.off, true, s_code_start);
+  lsmall_runs(a, s_code_start);
+  if (threadIdx.x == 0) {
+    *a.bad_out = *a.bad;
+    *a.bad = 0u;
+    *a.done = 0u;
   }
 }
 
@@ -1093,20 +1109,26 @@ __global__ __launch_bounds__(1024) void k_lsmall_count(LayoutSmall a) {
   if (last_group(a.done)) lsmall_scan(a);
 }
 
-// The same with the counts of a work-group's 1024 rows (one per thread)
-// taken in LDS first (keys of up to kLayoutLdsCodes codes): one global
-// atomic per bucket the work-group uses.
+// Layouts of up to kLayoutLdsCodes codes: the counts of a work-group's 1024
+// rows (one per thread) taken in LDS first, one global atomic per bucket the
+// work-group uses, into a.hist (zero on entry).  The launch also clears
+// a.hist_clear (the counters of the previous such layout, which its
+// k_lsmall_place has read) and the layout's bad flag; the offsets are
+// scanned by every work-group of k_lsmall_place.
 constexpr uint32_t kLayoutRowsPerGroup = 1024;
 __global__ __launch_bounds__(1024) void k_lsmall_count_lds(LayoutSmall a) {
   extern __shared__ uint32_t s_cnt[];  // buckets of both orders
   const uint32_t t = threadIdx.x;
   const uint32_t nb = a.n_codes * 256u, nb2 = a.sig ? 2u * nb : nb;
   for (uint32_t k = t; k < nb2; k += 1024) s_cnt[k] = 0;
+  for (uint32_t k = blockIdx.x * 1024 + t; k < a.clear_words; k += gridDim.x * 1024)
+    a.hist_clear[k] = 0u;
+  if (blockIdx.x == 0 && t == 0) *a.bad_out = 0u;  // k_lsmall_place sets it
   __syncthreads();
   const uint32_t i = blockIdx.x * kLayoutRowsPerGroup + t;
   uint32_t b1 = 0xffffffffu, b2 = 0xffffffffu, r1 = 0, r2 = 0;
   if (i < a.n) {
-    const uint32_t base = lsmall_code(a, i, a.bad) << 8;
+    const uint32_t base = lsmall_code(a, i, nullptr) << 8;
     const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
     b1 = base + l;
     if (a.sig) b2 = nb + base + (a.sig[i].pre & 0xffu);
@@ -1121,15 +1143,13 @@ __global__ __launch_bounds__(1024) void k_lsmall_count_lds(LayoutSmall a) {
   __syncthreads();
   if (b1 != 0xffffffffu) a.rank[i] = s_cnt[b1] + r1;
   if (b2 != 0xffffffffu) a.rank[a.n + i] = s_cnt[b2] + r2;
-  if (last_group(a.done)) lsmall_scan(a);
 }
 
-__global__ void k_lsmall_scatter(LayoutSmall a) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+// One row's places in both orders, its signature copied to each.
+__device__ __forceinline__ void lsmall_put(const LayoutSmall &a, uint32_t i, const uint32_t *off) {
   const uint32_t base = lsmall_code(a, i, a.bad_out) << 8;
   const uint32_t l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
-  const uint32_t pos = a.off[base + l] + a.rank[i];
+  const uint32_t pos = off[base + l] + a.rank[i];
   a.perm[pos] = i;
   if (a.sig) {
     // The row as three 16-byte words (word 8, the first 4 bytes, opens the
@@ -1142,13 +1162,31 @@ __global__ void k_lsmall_scatter(LayoutSmall a) {
     d[0] = w0;
     d[1] = w1;
     d[2] = w2;
-    const uint32_t pos2 = a.off[a.n_codes * 256u + base + (w2.x & 0xffu)] + a.rank[a.n + i];
+    const uint32_t pos2 = off[a.n_codes * 256u + base + (w2.x & 0xffu)] + a.rank[a.n + i];
     a.perm_fb[pos2] = i;
     d = reinterpret_cast<u32x4 *>(a.jsig_fb + pos2);
     d[0] = w0;
     d[1] = w1;
     d[2] = w2;
   }
+}
+
+// Places the rows of a layout counted by k_lsmall_count_lds: every
+// work-group scans the counts (at most 2 * kLayoutLdsCodes * 256) into LDS
+// itself, so no work-group waits for another; the first also writes the
+// runs.
+__global__ __launch_bounds__(1024) void k_lsmall_place(LayoutSmall a) {
+  extern __shared__ uint32_t s_off[];
+  __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
+  lsmall_offsets(a, a.hist, s_off, false, s_code_start);
+  if (blockIdx.x == 0) lsmall_runs(a, s_code_start);
+  for (uint32_t i = blockIdx.x * 1024 + threadIdx.x; i < a.n; i += gridDim.x * 1024)
+    lsmall_put(a, i, s_off);
+}
+
+__global__ void k_lsmall_scatter(LayoutSmall a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) lsmall_put(a, i, a.off);
 }
 
 // Keys of the first-byte order: (code << 16) | (first byte << 8) |
@@ -2659,11 +2697,13 @@ void launch_layout_small(const LayoutSmall &a, hipStream_t stream) {
   const uint32_t blocks = (a.n + 255) / 256;
   const size_t lds = (a.sig ? 2 : 1) * a.n_codes * 256 * sizeof(uint32_t);
   const uint32_t groups = (a.n + kLayoutRowsPerGroup - 1) / kLayoutRowsPerGroup;
-  if (a.n_codes <= kLayoutLdsCodes)
+  if (a.n_codes <= kLayoutLdsCodes) {
     k_lsmall_count_lds<<<groups, 1024, lds, stream>>>(a);
-  else
+    k_lsmall_place<<<groups, 1024, lds, stream>>>(a);
+  } else {
     k_lsmall_count<<<groups, 1024, 0, stream>>>(a);
-  k_lsmall_scatter<<<blocks, 256, 0, stream>>>(a);
+    k_lsmall_scatter<<<blocks, 256, 0, stream>>>(a);
+  }
 }
 
 void launch_layout_keys(const uint32_t *code, const uint16_t *len, uint32_t n,

@@ -628,10 +628,14 @@ struct hb_ctx {
   // separate kernels instead of the fused one (HB_SEPARATE_COLUMN_BUILDS).
   bool separate_col_builds = false;
   // Counters shared by the counting-sort layout builds, which run one after
-  // another on the build stream (each build leaves them zero): bucket
-  // counts (2 * kLayoutMaxCodes * 256), then a bad-code flag and a done count.
+  // another on the build stream: bucket counts of layouts of more than
+  // kLayoutLdsCodes codes (2 * kLayoutMaxCodes * 256, left zero by each
+  // build), a bad-code flag and a done count (64 words), then two buffers
+  // of 2 * kLayoutLdsCodes * 256 counts that layouts of fewer codes use in
+  // turn (layout_turn: the next one's), each build clearing the other.
   // layout_mu keeps one build's two launches together on the stream.
   uint32_t *layout_hist = nullptr;
+  uint32_t layout_turn = 0;
   std::mutex layout_mu;
   // hb_block_noalloc: the device address of this GPU's copy of the
   // library's static scratch array (looked up at creation) and the stream
@@ -1527,8 +1531,9 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       a.sig = sim >= 0 ? t->cols[sim].d_sig : nullptr;
       a.n = n;
       a.n_codes = (uint32_t)n_codes;
-      const size_t nb = 2 * (size_t)n_codes * 256;
-      HB_TRY(temp(nb * sizeof(uint32_t), (void **)&a.off));
+      const bool lds_path = n_codes <= hb::kLayoutLdsCodes;
+      if (!lds_path)
+        HB_TRY(temp(2 * (size_t)n_codes * 256 * sizeof(uint32_t), (void **)&a.off));
       HB_TRY(temp(2 * n1 * sizeof(uint32_t), (void **)&a.rank));
       a.perm = ly.d_perm;
       a.perm_fb = ly.d_perm_fb;
@@ -1540,14 +1545,25 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
       a.n_runs_host = ly.d_hnruns;
       {
         std::lock_guard<std::mutex> ll(t->ctx->layout_mu);
+        const size_t big = 2 * (size_t)hb::kLayoutMaxCodes * 256;
+        const size_t small = 2 * (size_t)hb::kLayoutLdsCodes * 256;
         if (!t->ctx->layout_hist) {  // created zero, on first use
-          const size_t words = 2 * (size_t)hb::kLayoutMaxCodes * 256 + 64;
+          const size_t words = big + 64 + 2 * small;
           HIP_OK(hipMalloc((void **)&t->ctx->layout_hist, words * sizeof(uint32_t)));
           hb::launch_fill_u64((unsigned long long *)t->ctx->layout_hist, words / 2, 0ull, stream);
         }
-        a.hist = t->ctx->layout_hist;
-        a.bad = a.hist + 2 * (size_t)hb::kLayoutMaxCodes * 256;
-        a.done = a.bad + 32;
+        uint32_t *const lh = t->ctx->layout_hist;
+        if (lds_path) {
+          uint32_t &turn = t->ctx->layout_turn;
+          a.hist = lh + big + 64 + turn * small;
+          a.hist_clear = lh + big + 64 + (1 - turn) * small;
+          a.clear_words = (uint32_t)small;
+          turn = 1 - turn;
+        } else {
+          a.hist = lh;
+          a.bad = lh + big;
+          a.done = a.bad + 32;
+        }
         trace.mark("temps");
         hb::launch_layout_small(a, stream);
         HIP_OK(hipGetLastError());

@@ -233,6 +233,33 @@ def test_dictionary_fallback_stays_exact(env):
         c.close()
 
 
+@pytest.mark.parametrize("n_keys", [20, 45, 100])
+def test_layouts_for_each_code_count(n_keys):
+    """Join keys of n_keys values in a fresh context, so that the layouts
+    see that many codes: up to 28 the counting layouts keep counts and
+    offsets in LDS and alternate between two counter buffers, up to 64 they
+    count with global atomics and scan once, beyond that radix sorts build
+    them.  Three table pairs built one after another, each joined under
+    two plans: pairs and audit entries equal the oracle's."""
+    c = hb.Context(0)
+    try:
+        for seed in (11, 12, 13):
+            L, R = _adversarial_tables(seed * 1000 + n_keys, 900, 800, b"abc", 16, n_keys=n_keys)
+            tl, tr = c.upload(hb.ColumnTable(L)), c.upload(hb.ColumnTable(R))
+            for name in ("eq_sim", "multi_rule"):
+                plan = hb.HostPlan.from_yaml(PLANS[name])
+                dp = c.plan(plan)
+                got, audit, _ = c.block(dp, tl, tr)
+                dp.release()
+                want, want_audit, _ = H.run_oracle(plan, L, R)
+                H.assert_same_pairs(got, want, "%d keys, %s, seed %d" % (n_keys, name, seed))
+                H.assert_same_audit(H.sort_audit(audit), want_audit, "%d keys, %s" % (n_keys, name))
+            tl.release()
+            tr.release()
+    finally:
+        c.close()
+
+
 def test_fused_column_build_equals_separate_kernels():
     """A similarity column's signatures, 128-class counters and bit-planes
     come from one fused kernel; a context created with
