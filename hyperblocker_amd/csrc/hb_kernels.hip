@@ -1171,17 +1171,48 @@ __device__ __forceinline__ void lsmall_put(const LayoutSmall &a, uint32_t i, con
   }
 }
 
-// Places the rows of a layout counted by k_lsmall_count_lds: every
-// work-group scans the counts (at most 2 * kLayoutLdsCodes * 256) into LDS
-// itself, so no work-group waits for another; the first also writes the
-// runs.
+// Places the rows of a layout counted by k_lsmall_count_lds, one row per
+// thread (the grid covers n): every work-group scans the counts (at most
+// 2 * kLayoutLdsCodes * 256) into LDS itself, so no work-group waits for
+// another, and the first also writes the runs.  A row's own reads are
+// issued before the scan, so their latency overlaps it.
 __global__ __launch_bounds__(1024) void k_lsmall_place(LayoutSmall a) {
   extern __shared__ uint32_t s_off[];
   __shared__ uint32_t s_code_start[kLayoutMaxCodes + 1];
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+  const bool has = i < a.n;
+  uint32_t base = 0, l = 0, rk = 0, rk2 = 0;
+  u32x4 w0 = {0u, 0u, 0u, 0u}, w1 = w0, w2 = w0;
+  if (has) {
+    base = lsmall_code(a, i, a.bad_out) << 8;
+    l = a.len ? min((uint32_t)a.len[i], 255u) : 0u;
+    rk = a.rank[i];
+    if (a.sig) {
+      rk2 = a.rank[a.n + i];
+      const u32x4 *src = reinterpret_cast<const u32x4 *>(a.sig + i);
+      w0 = src[0];
+      w1 = src[1];
+      w2 = src[2];
+    }
+  }
   lsmall_offsets(a, a.hist, s_off, false, s_code_start);
   if (blockIdx.x == 0) lsmall_runs(a, s_code_start);
-  for (uint32_t i = blockIdx.x * 1024 + threadIdx.x; i < a.n; i += gridDim.x * 1024)
-    lsmall_put(a, i, s_off);
+  if (!has) return;
+  const uint32_t pos = s_off[base + l] + rk;
+  a.perm[pos] = i;
+  if (a.sig) {
+    u32x4 *d = reinterpret_cast<u32x4 *>(a.jsig + pos);
+    d[0] = w0;
+    d[1] = w1;
+    d[2] = w2;
+    const uint32_t pos2 = s_off[a.n_codes * 256u + base + (w2.x & 0xffu)] + rk2;
+    a.perm_fb[pos2] = i;
+    d = reinterpret_cast<u32x4 *>(a.jsig_fb + pos2);
+    d[0] = w0;
+    d[1] = w1;
+    d[2] = w2;
+  }
 }
 
 __global__ void k_lsmall_scatter(LayoutSmall a) {
