@@ -627,15 +627,19 @@ struct hb_ctx {
   // Test hook: build a similarity column's structures with the three
   // separate kernels instead of the fused one (HB_SEPARATE_COLUMN_BUILDS).
   bool separate_col_builds = false;
-  // Counters shared by the counting-sort layout builds, which run one after
-  // another on the build stream: bucket counts of layouts of more than
-  // kLayoutLdsCodes codes (2 * kLayoutMaxCodes * 256, left zero by each
-  // build), a bad-code flag and a done count (64 words), then two buffers
-  // of 2 * kLayoutLdsCodes * 256 counts that layouts of fewer codes use in
-  // turn (layout_turn: the next one's), each build clearing the other.
-  // layout_mu keeps one build's two launches together on the stream.
-  uint32_t *layout_hist = nullptr;
-  uint32_t layout_turn = 0;
+  // Counters of the counting-sort layout builds, one set per stream that
+  // builds layouts (builds on one stream run one after another): bucket
+  // counts of layouts of more than kLayoutLdsCodes codes (2 *
+  // kLayoutMaxCodes * 256, left zero by each build), a bad-code flag and a
+  // done count (64 words), then two buffers of 2 * kLayoutLdsCodes * 256
+  // counts that layouts of fewer codes use in turn (turn: the next one's),
+  // each build clearing the other.  layout_mu guards the map and keeps one
+  // build's two launches together on its stream.
+  struct LayoutCounters {
+    uint32_t *hist = nullptr;
+    uint32_t turn = 0;
+  };
+  std::map<hipStream_t, LayoutCounters> layout_counters;
   std::mutex layout_mu;
   // hb_block_noalloc: the device address of this GPU's copy of the
   // library's static scratch array (looked up at creation) and the stream
@@ -1458,10 +1462,10 @@ hb_status layout_runs(Layout *ly, uint32_t *out, uint64_t *budget) {
 // radix-sorted with the row ids, the runs of equal code, and the similarity
 // signatures gathered into that order.
 //
-// The build runs on the context's build stream; jobs' streams wait on the
-// layout's event, so a build never waits for other batches' kernels, and the
-// host never waits for the build (the run count, which sizes the batch's
-// groups, is counted on the host).
+// The build runs on the caller's stream (the first batch that needs the
+// layout), so the builds of different batches run side by side; later
+// users wait on the layout's event, and the host never waits for the build
+// (the batch's groups are sized by a bound on the run count).
 hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStream_t caller,
                         Layout **out) {
   {
@@ -1473,7 +1477,9 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     }
   }
   HostTrace trace("ensure_layout");
-  const hipStream_t stream = t->ctx->build_stream;
+  // built on the caller's stream (a batch's, or the build stream): builds
+  // for different batches run side by side
+  const hipStream_t stream = caller;
   HB_TRY(wait_ready(t, stream));
   const uint32_t *key_codes = nullptr;
   if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes));
@@ -1547,14 +1553,15 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
         std::lock_guard<std::mutex> ll(t->ctx->layout_mu);
         const size_t big = 2 * (size_t)hb::kLayoutMaxCodes * 256;
         const size_t small = 2 * (size_t)hb::kLayoutLdsCodes * 256;
-        if (!t->ctx->layout_hist) {  // created zero, on first use
+        hb_ctx::LayoutCounters &lc = t->ctx->layout_counters[stream];
+        if (!lc.hist) {  // created zero, on the stream's first use
           const size_t words = big + 64 + 2 * small;
-          HIP_OK(hipMalloc((void **)&t->ctx->layout_hist, words * sizeof(uint32_t)));
-          hb::launch_fill_u64((unsigned long long *)t->ctx->layout_hist, words / 2, 0ull, stream);
+          HIP_OK(hipMalloc((void **)&lc.hist, words * sizeof(uint32_t)));
+          hb::launch_fill_u64((unsigned long long *)lc.hist, words / 2, 0ull, stream);
         }
-        uint32_t *const lh = t->ctx->layout_hist;
+        uint32_t *const lh = lc.hist;
         if (lds_path) {
-          uint32_t &turn = t->ctx->layout_turn;
+          uint32_t &turn = lc.turn;
           a.hist = lh + big + 64 + turn * small;
           a.hist_clear = lh + big + 64 + (1 - turn) * small;
           a.clear_words = (uint32_t)small;
@@ -3095,7 +3102,8 @@ hb_status hb_ctx_destroy(hb_ctx *ctx) {
   for (hipStream_t s : ctx->streams) (void)hipStreamDestroy(s);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->build_stream) (void)hipStreamDestroy(ctx->build_stream);
-  if (ctx->layout_hist) (void)hipFree(ctx->layout_hist);
+  for (auto &kv : ctx->layout_counters)
+    if (kv.second.hist) (void)hipFree(kv.second.hist);
   if (ctx->code_stream) (void)hipStreamDestroy(ctx->code_stream);
   if (ctx->noalloc_stream) (void)hipStreamDestroy(ctx->noalloc_stream);
   delete ctx;
