@@ -678,12 +678,16 @@ struct Lazy {
   hipEvent_t ev = nullptr;
   // ev has completed since it was last recorded: waiting on it is a no-op
   mutable std::atomic<bool> done{false};
+  // the stream ev was last recorded on: work later queued there follows the
+  // build without a wait
+  std::atomic<hipStream_t> stream{nullptr};
   Lazy() = default;
-  Lazy(const Lazy &o) : built(o.built), ev(o.ev), done(o.done.load()) {}
+  Lazy(const Lazy &o) : built(o.built), ev(o.ev), done(o.done.load()), stream(o.stream.load()) {}
   Lazy &operator=(const Lazy &o) {
     built = o.built;
     ev = o.ev;
     done = o.done.load();
+    stream = o.stream.load();
     return *this;
   }
 };
@@ -771,6 +775,9 @@ struct hb_table {
   std::map<std::vector<int>, Composite> composites;             // (equality columns)
   hipEvent_t ready = nullptr;                // upload finished (a slice: its root's)
   mutable std::atomic<bool> ready_done{false};  // `ready` seen complete
+  // the last stream made to wait on `ready` (recorded once, at creation):
+  // later work queued there follows the upload without another wait
+  mutable std::atomic<hipStream_t> ready_waiter{nullptr};
   std::mutex mu;  // guards lazy builds
   // A slice borrows rows [row_off, row_off + n_rows) of its parent's column
   // data (bytes, lengths, signatures, planes, codes); its layouts are its own.
@@ -1021,11 +1028,13 @@ void batch_free_memory(Batch *b) {
 // Makes `stream` wait for the table's upload (nothing once it is seen done).
 hb_status wait_ready(const hb_table *t, hipStream_t stream) {
   if (!t->ready || t->ready_done.load(std::memory_order_acquire)) return HB_OK;
+  if (t->ready_waiter.load(std::memory_order_acquire) == stream) return HB_OK;
   if (hipEventQuery(t->ready) == hipSuccess) {
     t->ready_done.store(true, std::memory_order_release);
     return HB_OK;
   }
   HIP_OK(hipStreamWaitEvent(stream, t->ready, 0));
+  t->ready_waiter.store(stream, std::memory_order_release);
   return HB_OK;
 }
 
@@ -1034,6 +1043,7 @@ hb_status wait_ready(const hb_table *t, hipStream_t stream) {
 // this for every layout they use).
 hb_status wait_lazy(const Lazy &lz, hipStream_t stream) {
   if (!lz.ev || lz.done.load(std::memory_order_acquire)) return HB_OK;
+  if (lz.stream.load(std::memory_order_acquire) == stream) return HB_OK;  // stream order
   if (hipEventQuery(lz.ev) == hipSuccess) {
     lz.done.store(true, std::memory_order_release);
     return HB_OK;
@@ -1046,6 +1056,7 @@ hb_status record_lazy(Lazy &lz, hipStream_t stream) {
   if (!lz.ev) HIP_OK(hipEventCreateWithFlags(&lz.ev, hipEventDisableTiming));
   lz.done.store(false, std::memory_order_release);
   HIP_OK(hipEventRecord(lz.ev, stream));
+  lz.stream.store(stream, std::memory_order_release);
   lz.built = true;
   return HB_OK;
 }
@@ -1477,9 +1488,12 @@ hb_status ensure_layout(hb_table *t, const std::vector<int> &eq, int sim, hipStr
     }
   }
   HostTrace trace("ensure_layout");
-  // built on the caller's stream (a batch's, or the build stream): builds
-  // for different batches run side by side
-  const hipStream_t stream = caller;
+  // built on the caller's stream when the context owns it (a batch stream),
+  // so builds for different batches run side by side; else (a graph
+  // template's temporary stream) on the build stream
+  hipStream_t stream = t->ctx->build_stream;
+  for (hipStream_t s : t->ctx->streams)
+    if (s == caller) stream = caller;
   HB_TRY(wait_ready(t, stream));
   const uint32_t *key_codes = nullptr;
   if (!eq.empty()) HB_TRY(ensure_key(t, eq, stream, &key_codes));
