@@ -839,22 +839,14 @@ __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__
     const uint32_t span = (uint32_t)(nr * row_bytes);
     const uint8_t *src = rows + (size_t)r0 * row_bytes;
     __syncthreads();  // the previous block's reads of LDS are done
-    if (ALIGNED16) {
-      const uint4 *src4 = reinterpret_cast<const uint4 *>(src);
-#pragma unroll 8
-      for (uint32_t i = tid; i < span / 16; i += 256) s_mem4[i] = src4[i];
-    } else {
-      for (uint32_t i = tid; i < span; i += 256) s_rows[i] = src[i];
-    }
-    // the bytes from span to the next 32-byte boundary read as zero
-    for (uint32_t i = span + tid; i < ((span + 31u) & ~31u) + 32u; i += 256) s_rows[i] = 0;
-    __syncthreads();
-    // zero-byte bitmap: each thread takes 16 bytes (one 16-bit piece).  The
-    // flags of a dword's zero bytes sit at bits 7, 15, 23 and 31 of m; one
-    // multiply gathers them into bits 21-24 (each lands on its own bit, so
-    // nothing carries).
-    for (uint32_t i = tid; i < ((span + 31u) / 32u) * 2u; i += 256) {  // to the word's end
-      const uint4 x = s_mem4[i];
+    // Zero-byte bitmap, 16 bytes (one 16-bit piece) at a time.  The flags of
+    // a dword's zero bytes sit at bits 7, 15, 23 and 31 of m; one multiply
+    // gathers them into bits 21-24 (each lands on its own bit, so nothing
+    // carries).  The pieces up to the bitmap word's end past `span` are
+    // zero bytes.
+    uint16_t *s_zero16 = reinterpret_cast<uint16_t *>(s_zero);
+    const uint32_t n16_words = ((span + 31u) / 32u) * 2u;
+    auto zero_bits = [](const uint4 x) -> uint32_t {
       const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
       uint32_t z = 0;
 #pragma unroll
@@ -863,9 +855,27 @@ __global__ __launch_bounds__(256) void k_unpack_rows(const uint8_t *__restrict__
         const uint32_t m = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // zero bytes
         z |= (((m >> 7) * 0x00204081u) >> 21 & 0xfu) << (4 * h);
       }
-      reinterpret_cast<uint16_t *>(s_zero)[i] = (uint16_t)z;
+      return z;
+    };
+    if (ALIGNED16) {  // each 16 bytes' piece of the bitmap as they are loaded
+      const uint4 *src4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll 8
+      for (uint32_t i = tid; i < span / 16; i += 256) {
+        const uint4 x = src4[i];
+        s_mem4[i] = x;
+        s_zero16[i] = (uint16_t)zero_bits(x);
+      }
+      for (uint32_t i = span / 16 + tid; i < n16_words; i += 256) s_zero16[i] = 0xffffu;
+    } else {
+      for (uint32_t i = tid; i < span; i += 256) s_rows[i] = src[i];
     }
+    // the bytes from span to the next 32-byte boundary read as zero
+    for (uint32_t i = span + tid; i < ((span + 31u) & ~31u) + 32u; i += 256) s_rows[i] = 0;
     __syncthreads();
+    if (!ALIGNED16) {
+      for (uint32_t i = tid; i < n16_words; i += 256) s_zero16[i] = (uint16_t)zero_bits(s_mem4[i]);
+      __syncthreads();
+    }
     // each cell's length: its first zero byte, at most `size`
     for (uint32_t i = tid; i < nr * n_cols; i += 256) {
       const uint32_t j = i / n_cols, c = i - j * n_cols;
