@@ -335,7 +335,7 @@ def main():
     cand = stats[-1]["candidates"]
     achieved = cand * FILTER_OPS_PER_CANDIDATE / (filt_ms * 1e-3) / 1e12
     traffic = measured_traffic(args.workload)
-    roofline = {"bound": "valu", "kernel": "k_filter_tiles+k_filter_small",
+    roofline = {"bound": "valu", "kernel": "k_filter_tiles",
                 "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
                 "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
                 "ops_per_candidate": FILTER_OPS_PER_CANDIDATE, "candidates_per_launch": cand,

@@ -9,13 +9,13 @@
 //   1. join (k_batch_join): every left run of equal first-equality code in a
 //      JR's cached layout meets the right run of the same code (binary
 //      search); a rule without equalities is one group covering L x R.
-//   2. k_filter_tiles / k_filter_small: pairs of every group are checked for
+//   2. k_filter_tiles: pairs of every group are checked for
 //      the remaining equalities (integer code compares) and an upper bound of
 //      the first similarity predicate (byte-class histograms, v_sad_u8);
 //      survivors are compacted with wave ballots into a shared pair list
 //      tagged with their JR.  Large groups run as tiles (512 rows in lanes x
-//      256 rows staged in LDS and read as broadcasts), small groups as
-//      explicit pair lists.
+//      256 rows staged in LDS and read as broadcasts), small groups a wave
+//      each, in the launch's last work-groups.
 //   3. k_exact_two: each surviving pair (one per lane) gets the exact greedy
 //      Jaro match count from bit-planes of its shorter string, then the
 //      reference's fp32/fp64 tail, for two consecutive similarity
@@ -1495,10 +1495,87 @@ struct __attribute__((aligned(8))) BRow {
   uint32_t len;
 };
 
+// Read-only data the kernel reads at wave-uniform addresses: through the
+// constant address space such loads become scalar loads.
+typedef const __attribute__((address_space(4))) SigRow ConstSigRow;
+
+// Small groups (at most small_pairs_max pairs each), run by the filter
+// launch's last work-groups (k_filter_tiles): one wave per group, groups
+// taken in a grid-stride loop over those waves (wbuf: the wave's output
+// buffer of kWaveOut entries in LDS).  The larger side sits in lanes, 64
+// rows at a time; the other side's rows are read with scalar loads in join
+// order, where they are ordered by length, so the lanes' allowances are
+// recomputed only where the length changes.
+template <bool SIM>
+__device__ void filter_small_groups(const BatchDev &bd, const GroupArrays &g, uint32_t g_begin,
+                                    uint32_t g_end, const PairList &out, PairEntry *wbuf,
+                                    uint32_t wave_in, uint32_t n_waves) {
+  uint32_t wn = 0;  // entries in wbuf (wave-uniform)
+  const uint32_t lane = lane_id();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(wave_in);
+  for (uint32_t gi = g_begin + wave; gi < g_end; gi += n_waves) {
+    if (__builtin_amdgcn_readfirstlane((uint32_t)g.small_pairs[gi]) == 0 &&
+        __builtin_amdgcn_readfirstlane((uint32_t)(g.small_pairs[gi] >> 32)) == 0)
+      continue;  // tiled or empty
+    const uint32_t jr = __builtin_amdgcn_readfirstlane(g.jr[gi]);
+    const uint32_t l_off = __builtin_amdgcn_readfirstlane(g.l_off[gi]);
+    const uint32_t l_cnt = __builtin_amdgcn_readfirstlane(g.l_cnt[gi]);
+    const uint32_t r_off = __builtin_amdgcn_readfirstlane(g.r_off[gi]);
+    const uint32_t r_cnt = __builtin_amdgcn_readfirstlane(g.r_cnt[gi]);
+    const JRDesc *d = bd.jrs + jr;
+    const bool a_is_l = l_cnt >= r_cnt;
+    const uint32_t a_cnt = a_is_l ? l_cnt : r_cnt, b_cnt = a_is_l ? r_cnt : l_cnt;
+    const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
+    const SideDesc &sa = d->s[a_is_l ? 0 : 1], &sb = d->s[a_is_l ? 1 : 0];
+    const uint32_t *perm_a = wave_ptr(sa.perm) + a_off;
+    const uint32_t *perm_b = wave_ptr(sb.perm) + b_off;
+    const SigRow *sig_a = SIM ? wave_ptr(sa.jsig) + a_off : nullptr;
+    ConstSigRow *sig_b = SIM ? (ConstSigRow *)(wave_ptr(sb.jsig) + b_off) : nullptr;
+    const float k0 = 2.0f * d->c0, k3 = 2.0f * d->c3;
+    for (uint32_t a0 = 0; a0 < a_cnt; a0 += 64) {
+      const uint32_t ai = a0 + lane;
+      const bool av = ai < a_cnt;
+      uint32_t ah[8] = {0, 0, 0, 0, 0, 0, 0, 0}, apre = 0, la = 0;
+      float rla = INFINITY;
+      if (SIM && av) {
+        const SigRow s = sig_a[ai];
+#pragma unroll
+        for (int k = 0; k < 8; k++) ah[k] = s.h[k];
+        apre = __builtin_bswap32(s.pre);
+        la = s.len;
+        rla = s.rlen;
+      }
+      uint32_t cur_len = 0xffffffffu, nal0 = (uint32_t)(-kPassAllow), nal3 = nal0;
+      for (uint32_t bj = 0; bj < b_cnt; bj++) {
+        bool ok = av;
+        if (SIM) {
+          uint32_t bh[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) bh[k] = sig_b[bj].h[k];
+          const uint32_t blen = sig_b[bj].len;
+          if (blen != cur_len) {
+            cur_len = blen;
+            allowances(la, rla, blen, sig_b[bj].rlen, k0, k3, nal0, nal3);
+          }
+          const uint32_t x = apre ^ __builtin_bswap32(sig_b[bj].pre);
+          ok = ok && (bound_acc(ah, nal0, nal3, bh, x) <= 0 || x == 0u);
+        }
+        if (__builtin_amdgcn_ballot_w64(ok)) {
+          const uint32_t b_row = perm_b[bj];
+          const uint32_t a_row = ok ? perm_a[ai] : 0u;
+          wave_append(ok, PairEntry{a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, jr, 0u},
+                      wbuf, wn, out, bd.overflow);
+        }
+      }
+    }
+  }
+  wave_flush(wbuf, wn, out, bd.overflow);
+}
+
 template <bool SIM, bool TAB, int AR, int BU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR == 4 ? 5 : (AR == 2 ? 7 : 8)))) void
 k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_t g_end,
-               PairList out) {
+               PairList out, uint32_t tile_grid) {
   constexpr uint32_t kChunk = 256;
   __shared__ BRow s_b[SIM ? kChunk + 1 : 1];
   __shared__ unsigned long long s_starts[kChunk / 64];  // bit i: B row i starts a run
@@ -1510,6 +1587,14 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
   __shared__ unsigned long long s_slow[kSlices][kChunk / 64];
   __shared__ TileOut to;
   const uint32_t tid = threadIdx.x;
+  if (blockIdx.x >= tile_grid) {  // the launch's last work-groups: small groups
+    static_assert(sizeof(TileOut::e) >= 4 * kWaveOut * sizeof(PairEntry), "small-path buffer");
+    filter_small_groups<SIM>(bd, g, g_begin, g_end, out,
+                             reinterpret_cast<PairEntry *>(to.e) + (tid >> 6) * kWaveOut,
+                             (blockIdx.x - tile_grid) * 4u + (tid >> 6),
+                             (gridDim.x - tile_grid) * 4u);
+    return;
+  }
   if (tid == 0) {
     to.n = 0;
     to.dense = 0;
@@ -1519,7 +1604,7 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
   // so that the per-pair loop keeps its vector registers.
   const uint64_t item_begin = wave_u64(g.item_off[g_begin]);
   const uint64_t item_end = wave_u64(g.item_off[g_end]);
-  for (uint64_t item = item_begin + blockIdx.x; item < item_end; item += gridDim.x) {
+  for (uint64_t item = item_begin + blockIdx.x; item < item_end; item += tile_grid) {
     const uint32_t gi =
         wave_u32(g_begin + find_group(g.item_off + g_begin, g_end - g_begin, item));
     const uint32_t jr = wave_u32(g.jr[gi]);
@@ -1763,84 +1848,6 @@ k_filter_tiles(const BatchDev bd, const GroupArrays g, uint32_t g_begin, uint32_
     __syncthreads();
     if (to.n) tile_flush(to, tc, out, bd.overflow);  // group-uniform after the barrier
   }
-}
-
-// Read-only data the kernel reads at wave-uniform addresses: through the
-// constant address space such loads become scalar loads.
-typedef const __attribute__((address_space(4))) SigRow ConstSigRow;
-
-// Small groups (at most small_pairs_max pairs each): one wave per group,
-// groups taken in a grid-stride loop.  The larger side sits in lanes, 64
-// rows at a time; the other side's rows are read with scalar loads in join
-// order, where they are ordered by length, so the lanes' allowances are
-// recomputed only where the length changes.
-template <bool SIM>
-__global__ __launch_bounds__(256) void k_filter_small(const BatchDev bd, const GroupArrays g,
-                                                      uint32_t g_begin, uint32_t g_end,
-                                                      PairList out) {
-  __shared__ PairEntry s_out[4][kWaveOut];
-  PairEntry *wbuf = s_out[threadIdx.x >> 6];
-  uint32_t wn = 0;  // entries in wbuf (wave-uniform)
-  const uint32_t lane = lane_id();
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t gi = g_begin + wave; gi < g_end; gi += n_waves) {
-    if (__builtin_amdgcn_readfirstlane((uint32_t)g.small_pairs[gi]) == 0 &&
-        __builtin_amdgcn_readfirstlane((uint32_t)(g.small_pairs[gi] >> 32)) == 0)
-      continue;  // tiled or empty
-    const uint32_t jr = __builtin_amdgcn_readfirstlane(g.jr[gi]);
-    const uint32_t l_off = __builtin_amdgcn_readfirstlane(g.l_off[gi]);
-    const uint32_t l_cnt = __builtin_amdgcn_readfirstlane(g.l_cnt[gi]);
-    const uint32_t r_off = __builtin_amdgcn_readfirstlane(g.r_off[gi]);
-    const uint32_t r_cnt = __builtin_amdgcn_readfirstlane(g.r_cnt[gi]);
-    const JRDesc *d = bd.jrs + jr;
-    const bool a_is_l = l_cnt >= r_cnt;
-    const uint32_t a_cnt = a_is_l ? l_cnt : r_cnt, b_cnt = a_is_l ? r_cnt : l_cnt;
-    const uint32_t a_off = a_is_l ? l_off : r_off, b_off = a_is_l ? r_off : l_off;
-    const SideDesc &sa = d->s[a_is_l ? 0 : 1], &sb = d->s[a_is_l ? 1 : 0];
-    const uint32_t *perm_a = wave_ptr(sa.perm) + a_off;
-    const uint32_t *perm_b = wave_ptr(sb.perm) + b_off;
-    const SigRow *sig_a = SIM ? wave_ptr(sa.jsig) + a_off : nullptr;
-    ConstSigRow *sig_b = SIM ? (ConstSigRow *)(wave_ptr(sb.jsig) + b_off) : nullptr;
-    const float k0 = 2.0f * d->c0, k3 = 2.0f * d->c3;
-    for (uint32_t a0 = 0; a0 < a_cnt; a0 += 64) {
-      const uint32_t ai = a0 + lane;
-      const bool av = ai < a_cnt;
-      uint32_t ah[8] = {0, 0, 0, 0, 0, 0, 0, 0}, apre = 0, la = 0;
-      float rla = INFINITY;
-      if (SIM && av) {
-        const SigRow s = sig_a[ai];
-#pragma unroll
-        for (int k = 0; k < 8; k++) ah[k] = s.h[k];
-        apre = __builtin_bswap32(s.pre);
-        la = s.len;
-        rla = s.rlen;
-      }
-      uint32_t cur_len = 0xffffffffu, nal0 = (uint32_t)(-kPassAllow), nal3 = nal0;
-      for (uint32_t bj = 0; bj < b_cnt; bj++) {
-        bool ok = av;
-        if (SIM) {
-          uint32_t bh[8];
-#pragma unroll
-          for (int k = 0; k < 8; k++) bh[k] = sig_b[bj].h[k];
-          const uint32_t blen = sig_b[bj].len;
-          if (blen != cur_len) {
-            cur_len = blen;
-            allowances(la, rla, blen, sig_b[bj].rlen, k0, k3, nal0, nal3);
-          }
-          const uint32_t x = apre ^ __builtin_bswap32(sig_b[bj].pre);
-          ok = ok && (bound_acc(ah, nal0, nal3, bh, x) <= 0 || x == 0u);
-        }
-        if (__builtin_amdgcn_ballot_w64(ok)) {
-          const uint32_t b_row = perm_b[bj];
-          const uint32_t a_row = ok ? perm_a[ai] : 0u;
-          wave_append(ok, PairEntry{a_is_l ? a_row : b_row, a_is_l ? b_row : a_row, jr, 0u},
-                      wbuf, wn, out, bd.overflow);
-        }
-      }
-    }
-  }
-  wave_flush(wbuf, wn, out, bd.overflow);
 }
 
 // ---- exact Jaro-Winkler -----------------------------------------------------
@@ -2808,20 +2815,19 @@ void launch_filter(BatchDev b, GroupArrays g, uint32_t g_begin, uint32_t g_end, 
   // two rounds of the 8 that fit at once), each taking work items in a
   // grid-stride loop; surplus groups find no item and exit.
   const unsigned cus = (unsigned)(n_cu > 0 ? n_cu : 256);
-  const unsigned grid = cus * shape.grid_per_cu, grid_small = cus * 8;
+  // The last cus * 8 work-groups of the launch take the small groups.
+  const unsigned tile_grid = cus * shape.grid_per_cu, grid = tile_grid + cus * 8;
 #define HB_FILTER(SIM, TAB)                                                                  \
   do {                                                                                       \
     if (shape.rows_per_lane == 4)                                                            \
       hipLaunchKernelGGL((k_filter_tiles<SIM, TAB, 4, 1>), dim3(grid), dim3(256), 0, stream, \
-                         b, g, g_begin, g_end, out);                                         \
+                         b, g, g_begin, g_end, out, tile_grid);                              \
     else if (shape.rows_per_lane == 2)                                                       \
       hipLaunchKernelGGL((k_filter_tiles<SIM, TAB, 2, 2>), dim3(grid), dim3(256), 0, stream, \
-                         b, g, g_begin, g_end, out);                                         \
+                         b, g, g_begin, g_end, out, tile_grid);                              \
     else                                                                                     \
       hipLaunchKernelGGL((k_filter_tiles<SIM, TAB, 1, 2>), dim3(grid), dim3(256), 0, stream, \
-                         b, g, g_begin, g_end, out);                                         \
-    hipLaunchKernelGGL((k_filter_small<SIM>), dim3(grid_small), dim3(256), 0, stream, b, g,   \
-                       g_begin, g_end, out);                                                 \
+                         b, g, g_begin, g_end, out, tile_grid);                              \
   } while (0)
   if (variant == kFilterSimTable)
     HB_FILTER(true, true);
