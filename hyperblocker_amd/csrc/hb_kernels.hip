@@ -1318,6 +1318,23 @@ __device__ __forceinline__ uint32_t find_le_u32(const uint32_t *off, uint32_t n,
   return lo;
 }
 
+// Largest j in [start, n) with off[j] <= x, given off[start] <= x: steps of
+// 1, 2, 4, ... from start, then a binary search of the last step.
+__device__ __forceinline__ uint32_t find_le_from(const uint32_t *off, uint32_t n, uint32_t x,
+                                                 uint32_t start) {
+  uint32_t lo = start, step = 1;
+  while (lo + step < n && off[lo + step] <= x) {
+    lo += step;
+    step <<= 1;
+  }
+  uint32_t hi = min(n, lo + step);
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (off[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // One thread per group of the batch: left run `local` of JR jr meets the
 // right run of the same code (binary search over the right runs).
 __device__ __forceinline__ void join_group(const BatchDev &b, const GroupArrays &g, uint32_t gi,
@@ -1328,7 +1345,11 @@ __device__ __forceinline__ void join_group(const BatchDev &b, const GroupArrays 
     g.items[gi] = 0;
     return;
   }
-  const uint32_t jr = find_le_u32(b.jr_group_off, b.n_jr, gi);
+  // The JR of the wave's first active group by a binary search, each lane's
+  // from there: a wave's groups are consecutive, in one JR or a few.
+  const uint32_t g_first = __builtin_amdgcn_readfirstlane(gi);
+  const uint32_t jr0 = find_le_u32(b.jr_group_off, b.n_jr, g_first);
+  const uint32_t jr = find_le_from(b.jr_group_off, b.n_jr, gi, jr0);
   const JRDesc &d = b.jrs[jr];
   const uint32_t local = gi - b.jr_group_off[jr];
   const SideDesc &L = d.s[0], &R = d.s[1];
