@@ -26,6 +26,7 @@
 #include <string>
 #include <string_view>
 #include <sys/stat.h>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -527,19 +528,21 @@ struct hb_partitions {
 
 namespace {
 
+// Cells of CSV lines, row after row, with each row's cell count.
+struct CsvRows {
+  std::vector<std::string> cells;
+  std::vector<uint32_t> n_cells;
+};
+
 // rapidcsv parse (LabelParams(-1, -1), default SeparatorParams with the
-// given separator): quote toggling only at the start of a cell or inside a
-// cell that began with a quote, CR dropped outside quotes, LF ends a row
-// outside quotes, "..." cells unquoted with "" -> ".
-void parse_csv(const std::string &buf, char sep, std::vector<std::vector<std::string>> &rows) {
-  size_t i = 0;
-  if (buf.size() >= 3 && (unsigned char)buf[0] == 0xEF && (unsigned char)buf[1] == 0xBB &&
-      (unsigned char)buf[2] == 0xBF)
-    i = 3;
-  std::vector<std::string> row;
-  std::string cell;
-  bool quoted = false;
-  auto unquote = [](const std::string &c) {
+// given separator) of buf[begin, end): quote toggling only at the start of
+// a cell or inside a cell that began with a quote, CR dropped, every LF ends
+// a row (quoted line breaks are off, so a quote never spans lines), "..."
+// cells unquoted with "" -> ".  A row is the text up to its LF (or the end
+// of the input), so ranges that start after an LF parse independently.
+// Lines with no quote and no CR are split at the separator directly.
+void parse_csv_range(const char *buf, size_t begin, size_t end, char sep, CsvRows &out) {
+  auto unquote = [](std::string &c) {
     if (c.size() >= 2 && c.front() == '"' && c.back() == '"') {
       std::string s;
       s.reserve(c.size());
@@ -547,57 +550,97 @@ void parse_csv(const std::string &buf, char sep, std::vector<std::vector<std::st
         s += c[k];
         if (c[k] == '"' && k + 2 < c.size() && c[k + 1] == '"') k++;
       }
-      return s;
+      c.swap(s);
     }
-    return c;
   };
-  for (; i < buf.size(); i++) {
-    const char ch = buf[i];
-    if (ch == '"') {
-      if (cell.empty() || cell[0] == '"') quoted = !quoted;
-      cell += ch;
-    } else if (ch == sep) {
-      if (!quoted) {
-        row.push_back(unquote(cell));
-        cell.clear();
-      } else {
-        cell += ch;
-      }
-    } else if (ch == '\r') {
-      // dropped (quoted line breaks are off)
-    } else if (ch == '\n') {
-      row.push_back(unquote(cell));
-      cell.clear();
-      rows.push_back(std::move(row));
-      row.clear();
-      quoted = false;
-    } else {
-      cell += ch;
+  size_t i = begin;
+  while (i < end) {
+    const char *nl = (const char *)std::memchr(buf + i, '\n', end - i);
+    const size_t le = nl ? (size_t)(nl - buf) : end;  // line [i, le)
+    if (!nl) {  // the last line, without LF: a row unless it holds only CRs
+      size_t k = i;
+      while (k < le && buf[k] == '\r') k++;
+      if (k == le) break;
     }
-  }
-  if (!cell.empty() || !row.empty()) {
-    row.push_back(unquote(cell));
-    rows.push_back(std::move(row));
+    const size_t before = out.cells.size();
+    if (!std::memchr(buf + i, '"', le - i) && !std::memchr(buf + i, '\r', le - i)) {
+      size_t s = i;
+      for (;;) {
+        const char *q = (const char *)std::memchr(buf + s, sep, le - s);
+        const size_t ce = q ? (size_t)(q - buf) : le;
+        out.cells.emplace_back(buf + s, ce - s);
+        if (!q) break;
+        s = ce + 1;
+      }
+    } else {
+      std::string cell;
+      bool quoted = false;
+      for (size_t k = i; k < le; k++) {
+        const char ch = buf[k];
+        if (ch == '"') {
+          if (cell.empty() || cell[0] == '"') quoted = !quoted;
+          cell += ch;
+        } else if (ch == sep && !quoted) {
+          unquote(cell);
+          out.cells.push_back(std::move(cell));
+          cell.clear();
+        } else if (ch != '\r') {
+          cell += ch;
+        }
+      }
+      unquote(cell);
+      out.cells.push_back(std::move(cell));
+    }
+    out.n_cells.push_back((uint32_t)(out.cells.size() - before));
+    i = le + 1;
   }
 }
 
+unsigned host_threads(size_t work, size_t per_thread) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (unsigned)std::max<size_t>(1, std::min<size_t>({hw, 16, work / per_thread}));
+}
+
 void pack_view(hb_host_table *t) {
-  t->packed.assign(t->n_cols, {});
-  t->bases.assign(t->n_cols, nullptr);
-  t->strides.assign(t->n_cols, 1);
-  for (int c = 0; c < t->n_cols; c++) {
+  const int nc = t->n_cols;
+  const size_t nr = (size_t)t->n_rows;
+  t->packed.assign(nc, {});
+  t->bases.assign(nc, nullptr);
+  t->strides.assign(nc, 1);
+  const unsigned nt = host_threads(nr * nc, 65536);
+  std::vector<std::vector<size_t>> wt(nt, std::vector<size_t>(nc, 1));
+  auto rows_of = [&](unsigned k) { return std::make_pair(nr * k / nt, nr * (k + 1) / nt); };
+  auto run = [&](auto fn) {
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < nt; k++) th.emplace_back(fn, k);
+    fn(0u);
+    for (auto &x : th) x.join();
+  };
+  run([&](unsigned k) {  // widths: longest cell + 1 per column
+    const auto rr = rows_of(k);
+    for (size_t r = rr.first; r < rr.second; r++)
+      for (int c = 0; c < nc; c++)
+        wt[k][c] = std::max(wt[k][c], t->cells[r * nc + c].size() + 1);
+  });
+  for (int c = 0; c < nc; c++) {
     size_t w = 1;
-    for (int64_t r = 0; r < t->n_rows; r++)
-      w = std::max(w, t->cells[(size_t)r * t->n_cols + c].size() + 1);
-    auto &p = t->packed[c];
-    p.assign(std::max<size_t>((size_t)t->n_rows * w, 1), 0);
-    for (int64_t r = 0; r < t->n_rows; r++) {
-      const std::string &s = t->cells[(size_t)r * t->n_cols + c];
-      std::memcpy(p.data() + (size_t)r * w, s.data(), s.size());
-    }
-    t->bases[c] = p.data();
+    for (unsigned k = 0; k < nt; k++) w = std::max(w, wt[k][c]);
+    t->packed[c].resize(std::max<size_t>(nr * w, 1));
+    t->bases[c] = t->packed[c].data();
     t->strides[c] = (int64_t)w;
   }
+  run([&](unsigned k) {  // zero-padded fixed-width copies
+    const auto rr = rows_of(k);
+    for (int c = 0; c < nc; c++) {
+      const size_t w = (size_t)t->strides[c];
+      char *p = t->packed[c].data();
+      std::memset(p + rr.first * w, 0, (rr.second - rr.first) * w);
+      for (size_t r = rr.first; r < rr.second; r++) {
+        const std::string &s = t->cells[r * nc + c];
+        std::memcpy(p + r * w, s.data(), s.size());
+      }
+    }
+  });
 }
 
 }  // namespace
@@ -715,19 +758,80 @@ hb_status hb_csv_load(const char *path, char sep, hb_host_table **out) {
   if (!path || !out) return fail(HB_ERR_INVALID_ARG, "hb_csv_load: null argument");
   *out = nullptr;
   std::string buf;
-  if (!read_file(path, buf)) return fail(HB_ERR_IO, "cannot read '%s'", path);
-  std::vector<std::vector<std::string>> rows;
-  parse_csv(buf, sep, rows);
+  {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(HB_ERR_IO, "cannot read '%s'", path);
+    struct stat sb;
+    if (fstat(fileno(f), &sb) == 0 && sb.st_size > 0) buf.resize((size_t)sb.st_size);
+    const size_t got = buf.empty() ? 0 : std::fread(&buf[0], 1, buf.size(), f);
+    buf.resize(got);
+    char tmp[65536];  // whatever the size did not cover (a file still growing, a pipe)
+    for (size_t n; (n = std::fread(tmp, 1, sizeof tmp, f)) > 0;) buf.append(tmp, n);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) return fail(HB_ERR_IO, "cannot read '%s'", path);
+  }
+  size_t begin = 0;
+  if (buf.size() >= 3 && (unsigned char)buf[0] == 0xEF && (unsigned char)buf[1] == 0xBB &&
+      (unsigned char)buf[2] == 0xBF)
+    begin = 3;
+  // chunks that start after an LF parse independently (parse_csv_range)
+  const unsigned nt = host_threads(buf.size() - begin, 1u << 20);
+  std::vector<size_t> cut(nt + 1, buf.size());
+  cut[0] = begin;
+  for (unsigned k = 1; k < nt; k++) {
+    size_t c = std::max(cut[k - 1], begin + (buf.size() - begin) * k / nt);
+    const void *nl = c < buf.size() ? std::memchr(buf.data() + c, '\n', buf.size() - c) : nullptr;
+    cut[k] = nl ? (size_t)((const char *)nl - buf.data()) + 1 : buf.size();
+  }
+  std::vector<CsvRows> part(nt);
+  {
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < nt; k++)
+      th.emplace_back([&, k] { parse_csv_range(buf.data(), cut[k], cut[k + 1], sep, part[k]); });
+    parse_csv_range(buf.data(), cut[0], cut[1], sep, part[0]);
+    for (auto &x : th) x.join();
+  }
   auto t = std::make_unique<hb_host_table>();
-  t->n_rows = (int64_t)rows.size();
-  t->n_cols = rows.empty() ? 0 : (int32_t)rows[0].size();
-  if (t->n_rows == 0 || t->n_cols == 0) return fail(HB_ERR_PARSE, "'%s' holds no rows", path);
-  t->cells.reserve((size_t)t->n_rows * t->n_cols);
-  for (size_t r = 0; r < rows.size(); r++) {
-    if ((int32_t)rows[r].size() < t->n_cols)
-      return fail(HB_ERR_PARSE, "'%s' line %zu has %zu cells, the first line has %d", path,
-                  r + 1, rows[r].size(), t->n_cols);
-    for (int c = 0; c < t->n_cols; c++) t->cells.push_back(std::move(rows[r][c]));
+  size_t n_rows = 0;
+  for (const auto &p : part) n_rows += p.n_cells.size();
+  uint32_t n_cols = 0;
+  for (const auto &p : part)
+    if (!p.n_cells.empty()) {
+      n_cols = p.n_cells[0];
+      break;
+    }
+  if (n_rows == 0 || n_cols == 0) return fail(HB_ERR_PARSE, "'%s' holds no rows", path);
+  size_t row0 = 0;  // rows before each chunk
+  for (const auto &p : part) {
+    for (size_t r = 0; r < p.n_cells.size(); r++)
+      if (p.n_cells[r] < n_cols)
+        return fail(HB_ERR_PARSE, "'%s' line %zu has %u cells, the first line has %u", path,
+                    row0 + r + 1, p.n_cells[r], n_cols);
+    row0 += p.n_cells.size();
+  }
+  t->n_rows = (int64_t)n_rows;
+  t->n_cols = (int32_t)n_cols;
+  t->cells.resize(n_rows * n_cols);
+  {  // each chunk's first n_cols cells of every row into place (extra cells are dropped)
+    std::vector<std::thread> th;
+    auto place = [&](unsigned k, size_t r0) {
+      CsvRows &p = part[k];
+      size_t src = 0;
+      for (size_t r = 0; r < p.n_cells.size(); r++) {
+        for (uint32_t c = 0; c < n_cols; c++)
+          t->cells[(r0 + r) * n_cols + c] = std::move(p.cells[src + c]);
+        src += p.n_cells[r];
+      }
+      CsvRows().cells.swap(p.cells);
+    };
+    size_t r0 = 0;
+    for (unsigned k = 0; k < nt; k++) {
+      if (k) th.emplace_back(place, k, r0);
+      r0 += part[k].n_cells.size();
+    }
+    place(0, 0);
+    for (auto &x : th) x.join();
   }
   pack_view(t.get());
   *out = t.release();

@@ -213,6 +213,89 @@ def test_csv_edge_cases(tmp_path, data, sep, want):
     assert _read_all(hb.HostTable.from_csv(str(p), sep=sep)) == want
 
 
+def _csv_line_model(line, sep):
+    """rapidcsv's cell rules for one line (its LF removed): a quote toggles
+    the quoted state only at the start of a cell or inside a cell that began
+    with one, a separator outside quotes ends the cell, CR is dropped, and a
+    cell "..." is unquoted with "" -> "."""
+    def unquote(c):
+        if len(c) >= 2 and c[:1] == b'"' and c[-1:] == b'"':
+            out, k = bytearray(), 1
+            while k + 1 < len(c):
+                out.append(c[k])
+                if c[k] == 0x22 and k + 2 < len(c) and c[k + 1] == 0x22:
+                    k += 1
+                k += 1
+            return bytes(out)
+        return bytes(c)
+    cells, cell, quoted = [], bytearray(), False
+    for ch in line:
+        if ch == 0x22:
+            if not cell or cell[0] == 0x22:
+                quoted = not quoted
+            cell.append(ch)
+        elif ch == sep and not quoted:
+            cells.append(unquote(cell))
+            cell = bytearray()
+        elif ch != 0x0D:
+            cell.append(ch)
+    cells.append(unquote(cell))
+    return cells
+
+
+def test_csv_chunked_parse_equals_line_model(tmp_path):
+    """Files above a megabyte are split at line feeds and parsed on several
+    threads (every LF ends a row in this dialect, quoted or not).  A 6 MB
+    file of random lines (quoted cells with separators and "" escapes,
+    stray quotes, CRs, empty lines, a BOM, extra cells, no final LF) gives
+    the rows of the line-by-line model; a short row deep in the file is
+    reported with its own line number."""
+    rng = np.random.default_rng(5)
+    alphabet = np.frombuffer(b'abcdefgh XYZ,;"\r' + bytes(range(0x80, 0x84)), np.uint8)
+
+    def cell():
+        k = rng.integers(0, 6)
+        raw = bytes(rng.choice(alphabet, rng.integers(0, 24)))
+        if k == 0:
+            return b'"' + raw.replace(b'"', b'""') + b'"'  # quoted, escapes kept whole
+        if k == 1:
+            return b""
+        raw = raw.replace(b",", b"")  # may hold stray quotes and CRs
+        # a leading quote (after dropped CRs) would open a quoted cell
+        return b"x" + raw if raw.lstrip(b"\r")[:1] == b'"' else raw
+
+    lines = []
+    for i in range(110000):
+        if i and rng.random() < 0.01:
+            lines.append(b"a,b,c,d" if rng.random() < 0.5 else b"\r")
+            continue
+        n = 4 + int(rng.integers(0, 3))
+        lines.append(b",".join(cell() for _ in range(n)))
+    lines[0] = b"h0,h1,h2,h3"
+    data = b"\xef\xbb\xbf" + b"\n".join(lines)  # no final LF
+    assert len(data) > (4 << 20)
+    want = []
+    for ln in lines:
+        cells = _csv_line_model(ln, 0x2C)
+        assert len(cells) >= 4 or ln == b"\r", ln
+        want.append(cells[:4] if len(cells) >= 4 else None)
+    # "\r" lines give one empty cell: too short, so the file with them fails
+    # at the first one; the file without them parses to the model's rows
+    first_short = next(i for i, w in enumerate(want) if w is None)
+    p = tmp_path / "big.csv"
+    p.write_bytes(data)
+    with pytest.raises(capi.HBError) as e:
+        hb.HostTable.from_csv(str(p))
+    assert e.value.status == capi.HB_ERR_PARSE
+    assert "line %d has 1 cells" % (first_short + 1) in str(e.value), str(e.value)
+    keep = [ln for ln, w in zip(lines, want) if w is not None]
+    p.write_bytes(b"\xef\xbb\xbf" + b"\n".join(keep))
+    t = hb.HostTable.from_csv(str(p))
+    assert t.n_rows == len(keep) and t.n_cols == 4
+    got = _read_all(t)
+    assert got == [w for w in want if w is not None]
+
+
 def test_csv_errors(tmp_path):
     with pytest.raises(capi.HBError) as e:
         hb.HostTable.from_csv(str(tmp_path / "missing.csv"))
