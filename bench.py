@@ -131,7 +131,9 @@ def end_to_end_cli(L, R, rules):
         m = json.load(open(d + "/m.json"))
     return {"pairs_per_s": m["pairs"] / m["total_s"], "total_s": m["total_s"],
             "load_s": m["load_s"], "partition_s": m["partition_s"],
-            "blocking_s": m["blocking_s"], "process_s": process_s, "matches": m["matches"]}
+            "blocking_s": m["blocking_s"], "ctx_wait_s": m["ctx_wait_s"],
+            "upload_s": m["upload_s"], "write_s": m["write_s"], "process_s": process_s,
+            "matches": m["matches"]}
 
 
 def copy_calibration(n_bytes):
